@@ -1,0 +1,1723 @@
+// solver.hip — handle, setup and host control of the HPR iteration behind the
+// C ABI of include/hprsocp.h.
+//
+// Host control mirrors the reference's solve loop (src/algorithm.jl:2765-2905):
+// residual checks, restart tests and the sigma update are scalar host logic fed
+// by device reductions; everything vector-sized stays on the GPU.  Between two
+// checks the stream holds only kernel launches (no host synchronisation): the
+// M-norm needed by the restart test is reduced on the device and read at the
+// next check.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "../../include/hprsocp.h"
+#include "kernels.h"
+#include "plan.h"
+#include "scale_rules.h"
+
+using namespace hpr;
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Julia semantics for host scalars (max/min propagate NaN, clamp = nested ifelse)
+double jmax(double a, double b) {
+    if (a != a || b != b) return NAN;
+    return a > b ? a : (b > a ? b : (signbit(a) ? b : a));
+}
+double jmin(double a, double b) {
+    if (a != a || b != b) return NAN;
+    return a < b ? a : (b < a ? b : (signbit(a) ? a : b));
+}
+double jclamp(double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); }
+
+struct Err {
+    int code;
+    std::string msg;
+};
+
+#define HIPCHK(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) throw Err{HPRSOCP_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+std::string g_create_error;
+
+// SplitMix64 + Box-Muller normals (+1e-8): start vector of the power iterations
+// (eigenvalues.jl:76,159 use randn(MersenneTwister(1)) + 1e-8).
+void start_vector(int64_t len, double *out) {
+    uint64_t st = 1;
+    auto next = [&]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    };
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int64_t i = 0; i < len; i += 2) {
+        const double u1 = ((double)(next() >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+        const double u2 = ((double)(next() >> 11)) * (1.0 / 9007199254740992.0);
+        const double r = sqrt(-2.0 * log(u1));
+        out[i] = r * cos(two_pi * u2) + 1e-8;
+        if (i + 1 < len) out[i + 1] = r * sin(two_pi * u2) + 1e-8;
+    }
+}
+
+struct HostCsr {
+    int32_t nrows = 0, ncols = 0;
+    std::vector<int32_t> rp, ci;
+    std::vector<double> v;
+};
+
+HostCsr transpose(const HostCsr &a) {
+    HostCsr t;
+    t.nrows = a.ncols;
+    t.ncols = a.nrows;
+    t.rp.assign((size_t)a.ncols + 1, 0);
+    t.ci.resize(a.ci.size());
+    t.v.resize(a.v.size());
+    for (int32_t c : a.ci) t.rp[c + 1]++;
+    for (int32_t j = 0; j < a.ncols; ++j) t.rp[j + 1] += t.rp[j];
+    std::vector<int32_t> next(t.rp.begin(), t.rp.end() - 1);
+    for (int32_t i = 0; i < a.nrows; ++i)
+        for (int32_t k = a.rp[i]; k < a.rp[i + 1]; ++k) {
+            const int32_t d = next[a.ci[k]]++;
+            t.ci[d] = i;
+            t.v[d] = a.v[k];
+        }
+    return t;
+}
+
+// rows sorted by column, duplicates summed
+void canonicalize(HostCsr &a) {
+    std::vector<int32_t> rp(1, 0), ci;
+    std::vector<double> v;
+    std::vector<std::pair<int32_t, double>> row;
+    for (int32_t i = 0; i < a.nrows; ++i) {
+        row.clear();
+        for (int32_t k = a.rp[i]; k < a.rp[i + 1]; ++k) row.push_back({a.ci[k], a.v[k]});
+        std::stable_sort(row.begin(), row.end(), [](auto &x, auto &y) { return x.first < y.first; });
+        for (size_t k = 0; k < row.size(); ++k) {
+            if (!ci.empty() && (int32_t)ci.size() > rp.back() && ci.back() == row[k].first) v.back() += row[k].second;
+            else {
+                ci.push_back(row[k].first);
+                v.push_back(row[k].second);
+            }
+        }
+        rp.push_back((int32_t)ci.size());
+    }
+    a.rp.swap(rp);
+    a.ci.swap(ci);
+    a.v.swap(v);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ handle
+struct hprsocp_handle {
+    std::string err;
+    hprsocp_params params{};
+    hipStream_t st = nullptr;
+    int device = 0;
+    int32_t m = 0, n = 0, m1 = 0, n1 = 0, ncon = 0, nvar = 0;
+    int64_t nnzA = 0, nnzQ = 0;
+    std::vector<void *> allocs;
+    // matrices
+    CsrView A{}, AT{}, Q{}, F{}, FT{};
+    double *A_v = nullptr, *AT_v = nullptr, *Q_v = nullptr, *Q_v2 = nullptr;
+    int32_t *Q_tpos = nullptr;
+    int q_kind = HPRSOCP_Q_CSR, has_quad = 0, Q_is_diag = 0, q_struct_diag = 0;
+    int32_t Frows = 0;
+    double *Fdiag = nullptr, *Ftmp = nullptr;
+    DevPlan plan_A, plan_AT, plan_Q, plan_F, plan_FT;
+    int32_t *con_idx = nullptr, *var_idx = nullptr, *row_cone = nullptr;
+    std::vector<int32_t> h_con_idx, h_var_idx;
+    std::vector<int32_t> h_A_rp;  // for nnz bookkeeping
+    // vectors
+    double *c = nullptr, *AL = nullptr, *AU = nullptr, *l = nullptr, *u = nullptr, *rhs = nullptr;
+    double *row_norm = nullptr, *col_norm = nullptr, *diag_Q = nullptr;
+    double *x = nullptr, *x_hat = nullptr, *x_bar = nullptr, *z_bar = nullptr, *dx = nullptr, *last_x = nullptr;
+    double *y = nullptr, *y_bar = nullptr, *dy = nullptr, *last_y = nullptr, *s = nullptr;
+    double *w = nullptr, *w_bar = nullptr, *last_w = nullptr, *dw = nullptr;
+    double *aty = nullptr, *aty_bar = nullptr, *last_aty = nullptr, *atdy = nullptr, *qatdy = nullptr;
+    double *qw = nullptr, *dqw = nullptr, *qx = nullptr, *tempv = nullptr, *ax = nullptr, *rp = nullptr, *rd = nullptr;
+    double *fact1 = nullptr, *fact2 = nullptr, *fact = nullptr, *fact_m = nullptr;
+    double *raw_n = nullptr, *raw_m = nullptr, *part_n = nullptr, *part_m = nullptr;
+    double *cone_t = nullptr, *cone_ns = nullptr;
+    double *red_part = nullptr;  // reduction partials
+    int red_blocks = 1;
+    double *dscal = nullptr;     // device scalar slots
+    double *hscal = nullptr;     // pinned host mirror
+    PowerState *pstate = nullptr;
+    double *tmp_m = nullptr, *tmp_n = nullptr, *tmp_n2 = nullptr;
+    // scalars
+    double sigma = 1, lambda_A = 0, lambda_Q = 0, b_scale = 1, c_scale = 1, obj_const = 0;
+    double norm_b = 0, norm_c = 0, norm_b_org = 0, norm_c_org = 0;
+    bool lambda_set = false;
+    double power_time = 0, setup_time = 0;
+    int to_check = 1;
+    // timing (profiled iterations)
+    bool profile = false;
+    std::vector<hipEvent_t> ev;
+
+    template <class T>
+    T *dalloc(int64_t count) {
+        void *p = nullptr;
+        const size_t bytes = (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) throw Err{HPRSOCP_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+        allocs.push_back(p);
+        HIPCHK(hipMemsetAsync(p, 0, bytes, st));
+        return (T *)p;
+    }
+    // synchronous: the host source may be a temporary
+    template <class T>
+    T *upload(const T *src, int64_t count) {
+        T *d = dalloc<T>(count);
+        if (count > 0 && src) HIPCHK(hipMemcpyAsync(d, src, (size_t)count * sizeof(T), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return d;
+    }
+    void sync() { HIPCHK(hipStreamSynchronize(st)); }
+    ~hprsocp_handle() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (void *p : allocs) (void)hipFree(p);
+        if (hscal) (void)hipHostFree(hscal);
+        for (auto e : ev) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+// device scalar slots
+enum : int {
+    SL_RN = 0,      // residual n-side (6)
+    SL_RM = 8,      // residual m-side (5)
+    SL_BV = 16,     // bounds violation (1)
+    SL_MN0 = 24,    // M-norm dots, slot "last gap" (8)
+    SL_MN1 = 32,    // M-norm dots, slot "current gap" (8)
+    SL_SIG = 40,    // sigma dots (8)
+    SL_PW = 48,     // power iteration scratch (4)
+    SL_GEN = 56,    // generic (8)
+    SL_GOLD = 64,   // golden section evaluations (64)
+    SL_COUNT = 128
+};
+
+namespace {
+
+void upload_plan(hprsocp_handle *h, const HostPlan &hp, DevPlan &dp) {
+    dp.ntiles = (int32_t)hp.tiles.size();
+    dp.first_huge = hp.first_huge;
+    dp.nslots = hp.nslots;
+    dp.nhuge = (int32_t)hp.hstart.size();
+    dp.tiles = h->upload<Tile>(hp.tiles.data(), (int64_t)hp.tiles.size());
+    const int64_t nh = dp.nhuge;
+    std::vector<int32_t> buf;
+    buf.reserve((size_t)nh * 4);
+    buf.insert(buf.end(), hp.hstart.begin(), hp.hstart.end());
+    buf.insert(buf.end(), hp.hend.begin(), hp.hend.end());
+    buf.insert(buf.end(), hp.hslot0.begin(), hp.hslot0.end());
+    buf.insert(buf.end(), hp.hnslot.begin(), hp.hnslot.end());
+    dp.huge_buf = h->upload<int32_t>(buf.data(), (int64_t)buf.size());
+    dp.huge.start = dp.huge_buf;
+    dp.huge.end = dp.huge_buf + nh;
+    dp.huge.slot0 = dp.huge_buf + 2 * nh;
+    dp.huge.nslot = dp.huge_buf + 3 * nh;
+}
+
+DevPlan make_plan(hprsocp_handle *h, const std::vector<int32_t> &rp, int32_t nrows, int32_t lin_end,
+                  const std::vector<int32_t> &cidx, int32_t ncones) {
+    PlanConfig cfg;
+    HostPlan hp = build_plan(rp.empty() ? nullptr : rp.data(), nrows, lin_end, cidx.empty() ? nullptr : cidx.data(),
+                             ncones, cfg);
+    DevPlan dp;
+    upload_plan(h, hp, dp);
+    return dp;
+}
+
+double *slot(hprsocp_handle *h, int k) { return h->dscal + k; }
+
+// copy device slots [k, k+cnt) to host and wait
+const double *read_slots(hprsocp_handle *h, int k, int cnt) {
+    HIPCHK(hipMemcpyAsync(h->hscal + k, h->dscal + k, sizeof(double) * cnt, hipMemcpyDeviceToHost, h->st));
+    h->sync();
+    return h->hscal + k;
+}
+
+void reduce(hprsocp_handle *h, int64_t n, std::initializer_list<RedTerm> terms, int out_slot) {
+    std::vector<RedTerm> t(terms);
+    launch_reduce(h->st, n, t.data(), (int)t.size(), h->red_part, reduce_blocks(n), slot(h, out_slot));
+}
+
+// Q v into out (general Q; Q must have quadratic terms)
+void qmap(hprsocp_handle *h, const double *v, double *out) {
+    if (h->q_kind == HPRSOCP_Q_FACTORED) {
+        launch_spmv(h->st, h->plan_F, h->F, v, h->Ftmp);
+        launch_spmv(h->st, h->plan_FT, h->FT, h->Ftmp, out, h->Fdiag, v);
+    } else if (h->nnzQ == 0) {
+        launch_fill(h->st, h->n, 0.0, out);
+    } else if (h->q_struct_diag) {
+        launch_mul(h->st, h->n, h->diag_Q, v, out);
+    } else {
+        launch_spmv(h->st, h->plan_Q, h->Q, v, out);
+    }
+}
+
+// ------------------------------------------------------------- scaling
+void equilibrate(hprsocp_handle *h, int use_sum, int phase, double *trow, double *tcol) {
+    hipStream_t st = h->st;
+    const CsrView *qv = (h->nnzQ > 0) ? &h->Q : nullptr;
+    launch_row_stat(st, h->AT, qv, use_sum, tcol);
+    if (h->m > 0) {
+        launch_row_stat(st, h->A, nullptr, use_sum, trow);
+        launch_block_scale(st, h->ncon, h->con_idx, trow, h->params.soc_block_scaling_strategy, phase, LOC_CONSTRAINT);
+        launch_clip(st, h->m, trow);
+    }
+    launch_block_scale(st, h->nvar, h->var_idx, tcol, h->params.soc_block_scaling_strategy, phase, LOC_VARIABLE);
+    launch_clip(st, h->n, tcol);
+    launch_scale_rows(st, h->m, trow, h->AL, h->AU, h->rhs, h->row_norm);
+    launch_scale_cols(st, h->n, tcol, h->c, h->l, h->u, h->col_norm);
+    if (h->nnzQ > 0) launch_scale_csr(st, h->n, h->Q.rowptr, h->Q.col, h->Q_v, tcol, tcol);
+    if (h->m > 0) {
+        launch_scale_csr(st, h->m, h->A.rowptr, h->A.col, h->A_v, trow, tcol);
+        launch_scale_csr_t(st, h->n, h->AT.rowptr, h->AT.col, h->AT_v, trow, tcol);
+    }
+}
+
+// bound norms: returns (l2 of max(|AL_nInf|,|AU_nInf|), inf-norm of same)
+void bound_norms(hprsocp_handle *h, double *l2, double *linf) {
+    if (h->m == 0) {
+        *l2 = *linf = 0.0;
+        return;
+    }
+    launch_bound_stats(h->st, h->m, h->AL, h->AU, h->red_part, reduce_blocks(h->m), slot(h, SL_GEN));
+    const double *r = read_slots(h, SL_GEN, 2);
+    *l2 = sqrt(r[0]);
+    *linf = r[1];
+}
+// [sumsq, count, absmax] of finite entries
+void finite_stats(hprsocp_handle *h, int64_t n, const double *v, double out[3]) {
+    if (n <= 0) {
+        out[0] = out[1] = out[2] = 0;
+        return;
+    }
+    launch_finite_stats(h->st, n, v, h->red_part, reduce_blocks(n), slot(h, SL_GEN));
+    const double *r = read_slots(h, SL_GEN, 3);
+    out[0] = r[0];
+    out[1] = r[1];
+    out[2] = r[2];
+}
+double soc_rhs_inf(hprsocp_handle *h) {
+    if (h->ncon == 0 || h->m - h->m1 == 0) return 0.0;
+    reduce(h, h->m - h->m1, {RedTerm{h->rhs + h->m1, nullptr, nullptr, 1}}, SL_GEN);
+    return read_slots(h, SL_GEN, 1)[0];
+}
+double vec_absmax(hprsocp_handle *h, int64_t n, const double *v) {
+    if (n <= 0) return 0.0;
+    reduce(h, n, {RedTerm{v, nullptr, nullptr, 1}}, SL_GEN);
+    return read_slots(h, SL_GEN, 1)[0];
+}
+double vec_nrm2(hprsocp_handle *h, int64_t n, const double *v) {
+    if (n <= 0) return 0.0;
+    reduce(h, n, {RedTerm{v, v, nullptr, 0}}, SL_GEN);
+    return sqrt(read_slots(h, SL_GEN, 1)[0]);
+}
+
+// scaling!, reference src/utils/scaling.jl:785-1289 (CPU-path semantics: no
+// clipping before the SOC block aggregation; A^T scaled bit-identically to A).
+void do_scaling(hprsocp_handle *h) {
+    hipStream_t st = h->st;
+    const int32_t m = h->m, n = h->n;
+    launch_fill(st, m, 1.0, h->row_norm);
+    launch_fill(st, n, 1.0, h->col_norm);
+    h->b_scale = h->c_scale = 1.0;
+    double bl2, blinf;
+    bound_norms(h, &bl2, &blinf);
+    const double rinf = soc_rhs_inf(h);
+    if (h->q_kind == HPRSOCP_Q_FACTORED) {  // operator Q: skip ALL scaling (scaling.jl:797-846)
+        h->norm_b_org = m > 0 ? jmax(blinf, rinf) : 0.0;
+        h->norm_c_org = vec_absmax(h, n, h->c);
+        h->norm_b = m > 0 ? jmax(bl2, rinf) : 0.0;
+        h->norm_c = vec_nrm2(h, n, h->c);
+        return;
+    }
+    h->norm_b_org = jmax(blinf, rinf);
+    h->norm_c_org = vec_absmax(h, n, h->c);
+    double *trow = h->tmp_m, *tcol = h->tmp_n;
+    launch_fill(st, m, 1.0, trow);
+    launch_fill(st, n, 1.0, tcol);
+    const hprsocp_params &p = h->params;
+    if (p.use_Ruiz_scaling)
+        for (int it = 0; it < std::max(p.ruiz_iterations, 0); ++it) equilibrate(h, 0, PHASE_RUIZ, trow, tcol);
+    if (uses_pre_pock(p.soc_block_scaling_strategy))
+        for (int it = 0; it < 2; ++it) equilibrate(h, 0, PHASE_PRE_POCK, trow, tcol);
+    if (p.use_Pock_Chambolle_scaling) equilibrate(h, 1, PHASE_POCK, trow, tcol);
+    if (p.use_bc_scaling) {
+        // compute_bc_scales, scaling.jl:721-760
+        const int t = p.bc_scaling_norm_type;
+        auto finite_norm = [&](const double *v, int64_t len) {
+            double f[3];
+            finite_stats(h, len, v, f);
+            if (t == HPRSOCP_BC_LINF) return f[2];
+            if (f[1] == 0) return 0.0;
+            return t == HPRSOCP_BC_RMS ? sqrt(f[0] / f[1]) : sqrt(f[0]);
+        };
+        auto vec_norm = [&](const double *v, int64_t len) {
+            if (len == 0) return 0.0;
+            double f[3];
+            finite_stats(h, len, v, f);
+            if (t == HPRSOCP_BC_LINF) return f[2];
+            return t == HPRSOCP_BC_RMS ? sqrt(f[0] / (double)len) : sqrt(f[0]);
+        };
+        double hb = 1.0;
+        hb = jmax(hb, finite_norm(h->AL, m));
+        hb = jmax(hb, finite_norm(h->AU, m));
+        hb = jmax(hb, (h->ncon == 0 || m - h->m1 == 0) ? 0.0 : vec_norm(h->rhs + h->m1, m - h->m1));
+        hb = jclamp(hb, 1e-4, 1e4);
+        const double hc = jclamp(jmax(1.0, vec_norm(h->c, n)), 1e-4, 1e4);
+        const double bs = hb, cs = jclamp(hc, hb / 1e2, hb / 1e-2);
+        if (h->nnzQ > 0) launch_scale_scalar(st, h->nnzQ, h->Q_v, bs / cs, 1.0);
+        if (m > 0) {
+            launch_scale_scalar(st, m, h->AL, 1.0, bs);
+            launch_scale_scalar(st, m, h->AU, 1.0, bs);
+            launch_scale_scalar(st, m, h->rhs, 1.0, bs);
+        }
+        launch_scale_scalar(st, n, h->c, 1.0, cs);
+        launch_scale_scalar(st, n, h->l, 1.0, bs);
+        launch_scale_scalar(st, n, h->u, 1.0, bs);
+        h->b_scale = bs;
+        h->c_scale = cs;
+    }
+    // Q = (Q + Q')/2 (scaling.jl:1265-1269)
+    if (h->nnzQ > 0) {
+        launch_sym_q(st, h->nnzQ, h->Q_v, h->Q_tpos, h->Q_v2);
+        launch_copy(st, h->nnzQ, h->Q_v2, h->Q_v);
+    }
+    bound_norms(h, &bl2, &blinf);
+    h->norm_b = jmax(bl2, soc_rhs_inf(h));
+    h->norm_c = vec_nrm2(h, n, h->c);
+}
+
+void update_q_factors(hprsocp_handle *h) {
+    launch_q_factors(h->st, h->n, h->diag_Q, h->sigma, h->fact2, h->fact, h->fact1, h->fact_m);
+}
+
+// ------------------------------------------------------------- power iteration
+double power_iteration(hprsocp_handle *h, bool onA) {
+    const int64_t len = onA ? h->m : h->n;
+    std::vector<double> z0((size_t)len);
+    start_vector(len, z0.data());
+    double *z = onA ? h->tmp_m : h->tmp_n;
+    double *q = onA ? h->ax : h->tmp_n2;
+    HIPCHK(hipMemcpyAsync(z, z0.data(), sizeof(double) * len, hipMemcpyHostToDevice, h->st));
+    PowerState ps0{1.0, 1.0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h->pstate, &ps0, sizeof(PowerState), hipMemcpyHostToDevice, h->st));
+    const int32_t *done = &h->pstate->done;
+    const int max_iter = 5000, batch = 10;
+    PowerState ps{};
+    for (int it = 0; it < max_iter; it += batch) {
+        for (int k = 0; k < batch; ++k) {
+            reduce(h, len, {RedTerm{z, z, nullptr, 0}}, SL_PW);
+            launch_scale_by_rnorm(h->st, len, z, slot(h, SL_PW), q, done);
+            if (onA) {
+                launch_spmv(h->st, h->plan_AT, h->AT, q, h->tempv);
+                launch_spmv(h->st, h->plan_A, h->A, h->tempv, z);
+            } else {
+                qmap(h, q, z);
+            }
+            reduce(h, len, {RedTerm{q, z, nullptr, 0}}, SL_PW + 1);
+            launch_power_residual(h->st, len, z, slot(h, SL_PW + 1), q, done);
+            reduce(h, len, {RedTerm{q, q, nullptr, 0}, RedTerm{z, z, nullptr, 0}}, SL_PW + 2);
+            launch_power_check(h->st, slot(h, SL_PW + 1), slot(h, SL_PW + 2), slot(h, SL_PW + 3), h->pstate,
+                               1e-4, max_iter);
+        }
+        HIPCHK(hipMemcpyAsync(&ps, h->pstate, sizeof(PowerState), hipMemcpyDeviceToHost, h->st));
+        h->sync();
+        if (ps.done) break;
+    }
+    return ps.lambda;
+}
+
+// ------------------------------------------------------------- validation
+int validate_csr(std::string &err, const char *name, int32_t nr, int32_t nc, int64_t nnz, const int32_t *rp,
+                 const int32_t *ci, const double *v) {
+    char buf[256];
+    if (nnz < 0) {
+        err = std::string(name) + ": negative nnz";
+        return HPRSOCP_ERR_INVALID;
+    }
+    if (nnz == 0 && !rp) return 0;
+    if (!rp || (nnz > 0 && (!ci || !v))) {
+        err = std::string(name) + ": missing CSR array";
+        return HPRSOCP_ERR_INVALID;
+    }
+    if (rp[0] != 0 || rp[nr] != nnz) {
+        err = std::string(name) + ": rowptr must start at 0 and end at nnz";
+        return HPRSOCP_ERR_INVALID;
+    }
+    for (int32_t i = 0; i < nr; ++i)
+        if (rp[i + 1] < rp[i]) {
+            snprintf(buf, sizeof buf, "%s: rowptr not monotone at row %d", name, i);
+            err = buf;
+            return HPRSOCP_ERR_INVALID;
+        }
+    for (int64_t k = 0; k < nnz; ++k)
+        if (ci[k] < 0 || ci[k] >= nc) {
+            snprintf(buf, sizeof buf, "%s: column index out of range at %lld", name, (long long)k);
+            err = buf;
+            return HPRSOCP_ERR_INVALID;
+        }
+    return 0;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+void hprsocp_default_params(hprsocp_params *p) {
+    memset(p, 0, sizeof(*p));
+    p->stoptol = 1e-6;
+    p->sigma = -1.0;
+    p->max_iter = 2147483647LL;
+    p->time_limit = 3600.0;
+    p->check_iter = 150;
+    p->warm_up = 0;
+    p->print_frequency = -1;
+    p->device_number = 0;
+    p->use_Ruiz_scaling = 1;
+    p->ruiz_iterations = 10;
+    p->use_bc_scaling = 1;
+    p->bc_scaling_norm_type = HPRSOCP_BC_L2;
+    p->use_l2_scaling = 0;
+    p->use_Pock_Chambolle_scaling = 1;
+    p->soc_block_scaling_strategy = HPRSOCP_SOC_PHASE_TAPER;
+    p->verbose = 1;
+    p->use_gpu = 1;
+    p->precision = HPRSOCP_PREC_FP64;
+    p->refine_every = 1;
+    p->use_graph = 1;
+}
+
+const char *hprsocp_last_error(const hprsocp_handle *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+void hprsocp_destroy(hprsocp_handle *h) { delete h; }
+
+static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, hprsocp_handle *h) {
+    const double t0 = now_s();
+    if (params) h->params = *params;
+    else hprsocp_default_params(&h->params);
+    hprsocp_params &P = h->params;
+    if (!P.use_gpu) throw Err{HPRSOCP_ERR_UNSUPPORTED, "use_gpu = false: this backend runs the HPR iteration on the GPU only"};
+    if (P.auto_save) throw Err{HPRSOCP_ERR_UNSUPPORTED, "auto_save (HDF5) is not provided by this backend"};
+    if (P.precision != HPRSOCP_PREC_FP64) throw Err{HPRSOCP_ERR_UNSUPPORTED, "precision mode not available yet"};
+    if (P.sigma != -1.0 && !(P.sigma > 0.0)) {
+        char b[160];
+        snprintf(b, sizeof b, "Invalid sigma value: %g. It should be a positive number or -1 for automatic.", P.sigma);
+        throw Err{HPRSOCP_ERR_INVALID, b};
+    }
+    if (P.check_iter <= 0) throw Err{HPRSOCP_ERR_INVALID, "check_iter must be positive"};
+    if (P.print_frequency != -1 && P.print_frequency <= 0)
+        throw Err{HPRSOCP_ERR_INVALID, "Invalid print_frequency: it should be a positive integer or -1"};
+    const int32_t m = p->m, n = p->n;
+    if (m < 0 || n < 0) throw Err{HPRSOCP_ERR_INVALID, "negative dimensions"};
+    std::string e;
+    if (int rc = validate_csr(e, "A", m, n, p->A_nnz, p->A_rowptr, p->A_colidx, p->A_val)) throw Err{rc, e};
+    if (p->number_linear_con < 0 || p->number_linear_con > m) throw Err{HPRSOCP_ERR_INVALID, "number_linear_con out of range"};
+    if (p->number_lu_x < 0 || p->number_lu_x > n) throw Err{HPRSOCP_ERR_INVALID, "number_lu_x out of range"};
+    if (!p->c || !p->l || !p->u || (m > 0 && (!p->AL || !p->AU))) throw Err{HPRSOCP_ERR_INVALID, "missing vector"};
+    h->m = m;
+    h->n = n;
+    h->obj_const = p->obj_constant;
+    h->m1 = p->number_linear_con;
+    h->n1 = p->number_lu_x;
+    h->ncon = p->n_soc_con;
+    h->nvar = p->n_soc_var;
+    if (h->ncon < 0 || h->nvar < 0) throw Err{HPRSOCP_ERR_INVALID, "negative cone count"};
+    h->h_con_idx.assign((size_t)h->ncon + 1, m);
+    h->h_var_idx.assign((size_t)h->nvar + 1, n);
+    // SOC index vectors must start at the first SOC row / variable and end at the
+    // sentinel, also when there is no cone (builders.jl:55-67)
+    if (h->ncon > 0 && !p->soc_con_idx) throw Err{HPRSOCP_ERR_INVALID, "missing soc_con_idx"};
+    if (p->soc_con_idx) std::copy(p->soc_con_idx, p->soc_con_idx + h->ncon + 1, h->h_con_idx.begin());
+    else h->h_con_idx[0] = m;
+    if (h->nvar > 0 && !p->soc_var_idx) throw Err{HPRSOCP_ERR_INVALID, "missing soc_var_idx"};
+    if (p->soc_var_idx) std::copy(p->soc_var_idx, p->soc_var_idx + h->nvar + 1, h->h_var_idx.begin());
+    else h->h_var_idx[0] = n;
+    if (h->h_con_idx[0] != h->m1 || h->h_con_idx[h->ncon] != m)
+        throw Err{HPRSOCP_ERR_INVALID, "soc_con_idx must start at number_linear_con and end at m"};
+    if (h->h_var_idx[0] != h->n1 || h->h_var_idx[h->nvar] != n)
+        throw Err{HPRSOCP_ERR_INVALID, "soc_var_idx must start at number_lu_x and end at n"};
+    for (int32_t i = 0; i < h->ncon; ++i)
+        if (h->h_con_idx[i + 1] < h->h_con_idx[i]) throw Err{HPRSOCP_ERR_INVALID, "soc_con_idx not monotone"};
+    for (int32_t i = 0; i < h->nvar; ++i)
+        if (h->h_var_idx[i + 1] < h->h_var_idx[i]) throw Err{HPRSOCP_ERR_INVALID, "soc_var_idx not monotone"};
+
+    // ---- device
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (P.device_number < 0 || P.device_number >= ndev) {
+        char b[128];
+        snprintf(b, sizeof b, "device_number %d not available (%d devices)", P.device_number, ndev);
+        throw Err{HPRSOCP_ERR_DEVICE, b};
+    }
+    h->device = P.device_number;
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+
+    // ---- host-side matrices: A, A^T, Q (symmetrized, canonical), F, F^T
+    HostCsr A;
+    A.nrows = m;
+    A.ncols = n;
+    A.rp.assign(p->A_rowptr ? p->A_rowptr : nullptr, p->A_rowptr ? p->A_rowptr + m + 1 : nullptr);
+    if (A.rp.empty()) A.rp.assign((size_t)m + 1, 0);
+    A.ci.assign(p->A_colidx, p->A_colidx + p->A_nnz);
+    A.v.assign(p->A_val, p->A_val + p->A_nnz);
+    HostCsr AT = transpose(A);
+    h->nnzA = p->A_nnz;
+    h->q_kind = p->Q_kind;
+    HostCsr Qh;
+    if (p->Q_kind == HPRSOCP_Q_FACTORED) {
+        if (int rc = validate_csr(e, "F", p->F_rows, n, p->F_nnz, p->F_rowptr, p->F_colidx, p->F_val)) throw Err{rc, e};
+        h->has_quad = 1;
+    } else if (p->Q_kind == HPRSOCP_Q_CSR) {
+        if (int rc = validate_csr(e, "Q", n, n, p->Q_nnz, p->Q_rowptr, p->Q_colidx, p->Q_val)) throw Err{rc, e};
+        if (p->Q_nnz > 0) {
+            Qh.nrows = Qh.ncols = n;
+            Qh.rp.assign(p->Q_rowptr, p->Q_rowptr + n + 1);
+            Qh.ci.assign(p->Q_colidx, p->Q_colidx + p->Q_nnz);
+            Qh.v.assign(p->Q_val, p->Q_val + p->Q_nnz);
+            canonicalize(Qh);
+            HostCsr Qt = transpose(Qh);
+            bool sym = Qt.rp == Qh.rp && Qt.ci == Qh.ci && Qt.v == Qh.v;
+            if (!sym) {  // (Q + Q^T)/2 up front for non-symmetric input
+                HostCsr S;
+                S.nrows = S.ncols = n;
+                S.rp.push_back(0);
+                for (int32_t i = 0; i < n; ++i) {
+                    int32_t a = Qh.rp[i], ae = Qh.rp[i + 1], b = Qt.rp[i], be = Qt.rp[i + 1];
+                    while (a < ae || b < be) {
+                        const int32_t ca = a < ae ? Qh.ci[a] : INT32_MAX, cb = b < be ? Qt.ci[b] : INT32_MAX;
+                        if (ca == cb) {
+                            S.ci.push_back(ca);
+                            S.v.push_back((Qh.v[a++] + Qt.v[b++]) / 2.0);
+                        } else if (ca < cb) {
+                            S.ci.push_back(ca);
+                            S.v.push_back(Qh.v[a++] / 2.0);
+                        } else {
+                            S.ci.push_back(cb);
+                            S.v.push_back(Qt.v[b++] / 2.0);
+                        }
+                    }
+                    S.rp.push_back((int32_t)S.ci.size());
+                }
+                Qh = S;
+            }
+        }
+        h->has_quad = Qh.v.size() > 0;
+    } else {
+        throw Err{HPRSOCP_ERR_INVALID, "unknown Q_kind"};
+    }
+    h->nnzQ = (int64_t)Qh.v.size();
+
+    // ---- upload
+    int32_t *A_rp = h->upload<int32_t>(A.rp.data(), m + 1);
+    int32_t *A_ci = h->upload<int32_t>(A.ci.data(), (int64_t)A.ci.size());
+    h->A_v = h->upload<double>(A.v.data(), (int64_t)A.v.size());
+    h->A = CsrView{A_rp, A_ci, h->A_v, m};
+    int32_t *AT_rp = h->upload<int32_t>(AT.rp.data(), n + 1);
+    int32_t *AT_ci = h->upload<int32_t>(AT.ci.data(), (int64_t)AT.ci.size());
+    h->AT_v = h->upload<double>(AT.v.data(), (int64_t)AT.v.size());
+    h->AT = CsrView{AT_rp, AT_ci, h->AT_v, n};
+    h->h_A_rp = A.rp;
+    h->con_idx = h->upload<int32_t>(h->h_con_idx.data(), h->ncon + 1);
+    h->var_idx = h->upload<int32_t>(h->h_var_idx.data(), h->nvar + 1);
+    if (h->nnzQ > 0) {
+        int32_t *Q_rp = h->upload<int32_t>(Qh.rp.data(), n + 1);
+        int32_t *Q_ci = h->upload<int32_t>(Qh.ci.data(), h->nnzQ);
+        h->Q_v = h->upload<double>(Qh.v.data(), h->nnzQ);
+        h->Q_v2 = h->dalloc<double>(h->nnzQ);
+        h->Q = CsrView{Q_rp, Q_ci, h->Q_v, n};
+        // transpose partner of every entry (Qh is symmetric in pattern, rows sorted)
+        HostCsr idx = Qh;
+        for (int64_t k = 0; k < h->nnzQ; ++k) idx.v[k] = (double)k;
+        HostCsr idxT = transpose(idx);
+        std::vector<int32_t> tpos((size_t)h->nnzQ);
+        for (int64_t k = 0; k < h->nnzQ; ++k) tpos[k] = (int32_t)idxT.v[k];
+        h->Q_tpos = h->upload<int32_t>(tpos.data(), h->nnzQ);
+        bool sd = true;
+        for (int32_t i = 0; i < n && sd; ++i)
+            for (int32_t k = Qh.rp[i]; k < Qh.rp[i + 1]; ++k)
+                if (Qh.ci[k] != i) {
+                    sd = false;
+                    break;
+                }
+        h->q_struct_diag = sd;
+        h->plan_Q = make_plan(h, Qh.rp, n, n, {}, 0);
+    }
+    if (h->q_kind == HPRSOCP_Q_FACTORED) {
+        HostCsr F;
+        F.nrows = p->F_rows;
+        F.ncols = n;
+        F.rp.assign(p->F_rowptr, p->F_rowptr + p->F_rows + 1);
+        F.ci.assign(p->F_colidx, p->F_colidx + p->F_nnz);
+        F.v.assign(p->F_val, p->F_val + p->F_nnz);
+        HostCsr FT = transpose(F);
+        h->Frows = p->F_rows;
+        int32_t *F_rp = h->upload<int32_t>(F.rp.data(), F.nrows + 1);
+        int32_t *F_ci = h->upload<int32_t>(F.ci.data(), (int64_t)F.ci.size());
+        double *F_v = h->upload<double>(F.v.data(), (int64_t)F.v.size());
+        h->F = CsrView{F_rp, F_ci, F_v, F.nrows};
+        int32_t *FT_rp = h->upload<int32_t>(FT.rp.data(), n + 1);
+        int32_t *FT_ci = h->upload<int32_t>(FT.ci.data(), (int64_t)FT.ci.size());
+        double *FT_v = h->upload<double>(FT.v.data(), (int64_t)FT.v.size());
+        h->FT = CsrView{FT_rp, FT_ci, FT_v, n};
+        h->Fdiag = p->Q_diag ? h->upload<double>(p->Q_diag, n) : nullptr;
+        h->Ftmp = h->dalloc<double>(F.nrows);
+        h->plan_F = make_plan(h, F.rp, F.nrows, F.nrows, {}, 0);
+        h->plan_FT = make_plan(h, FT.rp, n, n, {}, 0);
+    }
+    // plans: A rows (linear | constraint cones), A^T rows (box | variable cones)
+    h->plan_A = make_plan(h, A.rp, m, h->m1, h->h_con_idx, h->ncon);
+    h->plan_AT = make_plan(h, AT.rp, n, h->n1, h->h_var_idx, h->nvar);
+    {
+        std::vector<int32_t> rc((size_t)m, -1);
+        for (int32_t k = 0; k < h->ncon; ++k)
+            for (int32_t i = h->h_con_idx[k]; i < h->h_con_idx[k + 1]; ++i) rc[i] = k;
+        h->row_cone = h->upload<int32_t>(rc.data(), m);
+    }
+    // vectors
+    h->c = h->upload<double>(p->c, n);
+    h->AL = h->upload<double>(p->AL, m);
+    h->AU = h->upload<double>(p->AU, m);
+    h->l = h->upload<double>(p->l, n);
+    h->u = h->upload<double>(p->u, n);
+    {
+        std::vector<double> rf((size_t)m, 0.0);
+        if (p->soc_rhs)
+            for (int32_t i = h->m1; i < m; ++i) rf[i] = p->soc_rhs[i - h->m1];
+        h->rhs = h->upload<double>(rf.data(), m);
+    }
+    double **vn[] = {&h->col_norm, &h->diag_Q, &h->x, &h->x_hat, &h->x_bar, &h->z_bar, &h->dx,
+                     &h->last_x, &h->w, &h->w_bar, &h->last_w, &h->dw, &h->aty, &h->aty_bar, &h->last_aty,
+                     &h->atdy, &h->qatdy, &h->qw, &h->dqw, &h->qx, &h->tempv, &h->rd, &h->fact1, &h->fact2,
+                     &h->fact, &h->fact_m, &h->raw_n, &h->tmp_n, &h->tmp_n2};
+    for (double **pp : vn) *pp = h->dalloc<double>(n);
+    h->row_norm = h->dalloc<double>(m);
+    double **vm[] = {&h->y, &h->y_bar, &h->dy, &h->last_y, &h->s, &h->ax, &h->rp, &h->raw_m, &h->tmp_m};
+    for (double **pp : vm) *pp = h->dalloc<double>(m);
+    h->part_n = h->dalloc<double>(h->plan_AT.nslots);
+    h->part_m = h->dalloc<double>(h->plan_A.nslots);
+    h->cone_t = h->dalloc<double>(h->ncon);
+    h->cone_ns = h->dalloc<double>(h->ncon);
+    h->red_blocks = 1024;
+    h->red_part = h->dalloc<double>((int64_t)h->red_blocks * kMaxGolden);
+    h->dscal = h->dalloc<double>(SL_COUNT);
+    HIPCHK(hipHostMalloc((void **)&h->hscal, sizeof(double) * SL_COUNT, hipHostMallocDefault));
+    h->pstate = (PowerState *)h->dalloc<double>(4);
+
+    do_scaling(h);
+    // check_Q_diagonal (scaling.jl:1312-1353), after scaling as in solve()
+    if (h->q_kind == HPRSOCP_Q_FACTORED) {
+        h->Q_is_diag = 0;
+    } else if (h->nnzQ == 0) {
+        h->Q_is_diag = 1;
+    } else {
+        launch_q_diag(h->st, n, h->Q.rowptr, h->Q.col, h->Q_v, h->diag_Q, h->red_part, reduce_blocks(n),
+                      slot(h, SL_GEN));
+        h->Q_is_diag = read_slots(h, SL_GEN, 2)[0] == 0.0;
+    }
+    // sigma_0 (allocate_workspace, algorithm.jl:1538-1551)
+    if (P.sigma == -1.0) {
+        const double nb = h->norm_b, nc = h->norm_c;
+        h->sigma = (nc > 1e-16 && nb > 1e-16 && nb < 1e16 && nc < 1e16) ? nb / nc : 1.0;
+    } else {
+        h->sigma = P.sigma;
+    }
+    if (h->Q_is_diag) update_q_factors(h);
+    h->to_check = 1;
+    h->sync();
+    h->setup_time = now_s() - t0;
+    return 0;
+}
+
+int hprsocp_create(const hprsocp_problem *prob, const hprsocp_params *params, hprsocp_handle **out) {
+    *out = nullptr;
+    g_create_error.clear();
+    if (!prob) {
+        g_create_error = "null problem";
+        return HPRSOCP_ERR_INVALID;
+    }
+    hprsocp_handle *h = new (std::nothrow) hprsocp_handle();
+    if (!h) return HPRSOCP_ERR_ALLOC;
+    try {
+        create_impl(prob, params, h);
+    } catch (const Err &e) {
+        g_create_error = e.msg;
+        delete h;
+        return e.code;
+    }
+    *out = h;
+    return 0;
+}
+
+#define ABI_TRY(h, ...)                 \
+    try {                               \
+        __VA_ARGS__;                    \
+    } catch (const Err &e_) {           \
+        (h)->err = e_.msg;              \
+        return e_.code;                 \
+    }
+
+int hprsocp_estimate_lambda(hprsocp_handle *h) {
+    ABI_TRY(h, {
+        const double t0 = now_s();
+        h->lambda_A = h->m > 0 ? power_iteration(h, true) * 1.05 : 0.0;
+        if (h->q_kind == HPRSOCP_Q_FACTORED) {
+            h->lambda_Q = power_iteration(h, false) * 1.05;
+        } else if (h->nnzQ > 0) {
+            if (!h->Q_is_diag) h->lambda_Q = power_iteration(h, false) * 1.05;
+            else {
+                reduce(h, h->nnzQ, {RedTerm{h->Q_v, nullptr, nullptr, 3}}, SL_GEN);
+                h->lambda_Q = read_slots(h, SL_GEN, 1)[0];
+            }
+        } else {
+            h->lambda_Q = 0.0;
+        }
+        h->lambda_set = true;
+        h->power_time = now_s() - t0;
+    });
+    return 0;
+}
+
+int hprsocp_set_initial(hprsocp_handle *h, const double *x0, const double *y0) {
+    ABI_TRY(h, {
+        hipStream_t st = h->st;
+        const int32_t m = h->m, n = h->n;
+        if (x0) {
+            HIPCHK(hipMemcpyAsync(h->tmp_n, x0, sizeof(double) * n, hipMemcpyHostToDevice, st));
+            launch_scale_vd(st, n, h->tmp_n, h->col_norm, h->b_scale, h->x);
+            launch_copy(st, n, h->x, h->x_bar);
+            launch_copy(st, n, h->x, h->last_x);
+            if (h->has_quad) {
+                launch_copy(st, n, h->x, h->w);
+                launch_copy(st, n, h->x, h->w_bar);
+                launch_copy(st, n, h->x, h->last_w);
+            }
+        }
+        if (y0) {
+            HIPCHK(hipMemcpyAsync(h->tmp_m, y0, sizeof(double) * m, hipMemcpyHostToDevice, st));
+            launch_scale_vd(st, m, h->tmp_m, h->row_norm, h->c_scale, h->y);
+            launch_copy(st, m, h->y, h->y_bar);
+            launch_copy(st, m, h->y, h->last_y);
+            if (m > 0) {
+                if (h->has_quad) {
+                    launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->aty_bar);
+                    launch_copy(st, n, h->aty_bar, h->aty);
+                    launch_copy(st, n, h->aty_bar, h->last_aty);
+                } else {
+                    launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->aty);
+                }
+            }
+            if (h->has_quad) {
+                qmap(h, h->x_bar, h->qx);
+                launch_init_zbar(st, n, h->x_bar, h->qx, h->aty_bar, h->c, h->sigma, h->z_bar);
+            } else {
+                launch_init_zbar(st, n, h->x_bar, nullptr, h->aty, h->c, h->sigma, h->z_bar);
+            }
+            if (m > 0) {
+                launch_spmv(st, h->plan_A, h->A, h->x_bar, h->ax);
+                launch_init_s(st, m, h->ax, h->y, h->AL, h->AU, h->lambda_A * h->sigma, h->s);
+            }
+        }
+        h->sync();
+    });
+    return 0;
+}
+
+// main_update_gpu! (algorithm.jl:2284-2309)
+static void do_update(hprsocp_handle *h, int64_t inner, int full) {
+    hipStream_t st = h->st;
+    const double h1 = 1.0 / ((double)inner + 2.0), h2 = 1.0 - h1;
+    const int32_t m = h->m, n = h->n;
+    h->to_check = full;
+    auto ev = [&](int k) {
+        if (h->profile) HIPCHK(hipEventRecord(h->ev[k], st));
+    };
+    if (!h->has_quad) {
+        ev(0);
+        NoQPrimalArgs a{h->y, h->x, h->x_hat, h->x_bar, h->z_bar, h->dx, h->last_x, h->c, h->l, h->u,
+                        h->sigma, h1, h2, full};
+        launch_noq_primal(st, h->plan_AT, h->AT, h->var_idx, a, h->raw_n, h->part_n);
+        ev(1);
+        if (m > 0) {
+            const double f1 = h->lambda_A * h->sigma;
+            DualArgs d{h->x_hat, h->y, h->y_bar, h->s, h->dy, h->last_y, h->AL, h->AU, h->rhs,
+                       f1, 1.0 / f1, h1, h2, full, 0};
+            launch_dual(st, h->plan_A, h->A, h->con_idx, d, h->raw_m, h->part_m);
+        }
+        ev(2);
+        return;
+    }
+    const double f2s = 1.0 / (1.0 + h->sigma * h->lambda_Q), f1s = 1.0 - f2s;
+    const int sdiag = h->q_struct_diag && h->q_kind == HPRSOCP_Q_CSR;
+    ev(0);
+    if (!sdiag) qmap(h, h->w, h->qw);
+    ev(1);
+    QPPrimalArgs a{h->x, h->x_hat, h->x_bar, h->z_bar, h->dx, h->w_bar, h->tempv, h->w, h->last_x, h->c,
+                   h->l, h->u, h->aty, h->qw, h->diag_Q, h->fact1, h->fact2, h->sigma, h1, h2, f1s, f2s,
+                   full, sdiag, h->Q_is_diag};
+    launch_qp_primal(st, h->plan_AT, h->var_idx, a, h->raw_n, h->part_n);
+    ev(2);
+    if (m > 0) {
+        if (!sdiag) {
+            if (h->q_kind == HPRSOCP_Q_FACTORED) {
+                launch_spmv(st, h->plan_F, h->F, h->w_bar, h->Ftmp);
+                TempvArgs t{h->x_hat, h->qw, h->tempv, h->Fdiag, h->w_bar, h->sigma};
+                launch_spmv_tempv(st, h->plan_FT, h->FT, h->Ftmp, t);
+            } else {
+                TempvArgs t{h->x_hat, h->qw, h->tempv, nullptr, nullptr, h->sigma};
+                launch_spmv_tempv(st, h->plan_Q, h->Q, h->w_bar, t);
+            }
+        }
+        ev(3);
+        const double f1 = h->lambda_A * h->sigma;
+        DualArgs d{h->tempv, h->y, h->y_bar, h->s, h->dy, h->last_y, h->AL, h->AU, h->rhs,
+                   f1, 1.0 / f1, h1, h2, full, 1};
+        launch_dual(st, h->plan_A, h->A, h->con_idx, d, h->raw_m, h->part_m);
+    } else {
+        ev(3);
+    }
+    ev(4);
+    W2Args w2{h->y_bar, h->w, h->w_bar, h->aty, h->dw, h->atdy, h->aty_bar, h->last_w, h->last_aty, h->fact,
+              h->sigma / (1.0 + h->sigma * h->lambda_Q), h1, h2, full, h->Q_is_diag};
+    launch_w2(st, h->plan_AT, h->AT, w2);
+    ev(5);
+    (void)n;
+}
+
+int hprsocp_update(hprsocp_handle *h, int64_t inner, int32_t to_check) {
+    if (!h->lambda_set) {
+        h->err = "estimate_lambda must run before update";
+        return HPRSOCP_ERR_STATE;
+    }
+    ABI_TRY(h, {
+        do_update(h, inner, to_check ? 1 : 0);
+        HIPCHK(hipGetLastError());
+    });
+    return 0;
+}
+
+// compute_residuals! (algorithm.jl:113-196)
+static void residuals_impl(hprsocp_handle *h, int64_t iter, hprsocp_residuals *r) {
+    hipStream_t st = h->st;
+    const int32_t m = h->m, n = h->n;
+    const double bs = h->b_scale, cs = h->c_scale;
+    const bool skip_q = !h->has_quad;
+    if (!skip_q) qmap(h, h->x_bar, h->qx);
+    if (m > 0) launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->atdy);
+    else launch_fill(st, n, 0.0, h->atdy);
+    const int nbn = reduce_blocks(n);
+    ResidNArgs an{h->c, h->x_bar, h->z_bar, h->col_norm, skip_q ? nullptr : h->qx, h->atdy, h->rd, cs};
+    launch_resid_n(st, n, an, h->red_part, nbn, slot(h, SL_RN));
+    if (m > 0) {
+        launch_spmv(st, h->plan_A, h->A, h->x_bar, h->ax);
+        launch_ax_to_org(st, m, h->ax, h->row_norm, bs);
+        launch_cone_rp_stats(st, h->ncon, h->con_idx, h->ax, h->rhs, h->row_norm, bs, h->cone_t, h->cone_ns);
+        ResidMArgs am{h->y_bar, h->s, h->rhs, h->AL, h->AU, h->row_norm, h->ax, h->rp, h->row_cone, h->con_idx,
+                      h->cone_t, h->cone_ns, bs, h->m1};
+        launch_resid_m(st, m, am, h->red_part + (int64_t)h->red_blocks * kMaxRed * 2, reduce_blocks(m),
+                       slot(h, SL_RM));
+    }
+    if (iter == 0)
+        launch_bounds_violation(st, n, h->nvar, h->var_idx, h->x_bar, h->l, h->u, h->col_norm, bs, h->dx,
+                                h->red_part + (int64_t)h->red_blocks * kMaxRed * 4, reduce_blocks(n),
+                                slot(h, SL_BV));
+    const double *v = read_slots(h, 0, SL_BV + 1);
+    const double *rn = v + SL_RN, *rm = v + SL_RM;
+    r->primal_obj = bs * cs * (rn[0] + 0.5 * rn[1]) + h->obj_const;
+    r->dual_obj = bs * cs * (-0.5 * rn[1] + rn[2]) + h->obj_const;
+    if (m > 0) r->dual_obj += bs * cs * (rm[0] + rm[1]);
+    r->rel_gap = fabs(r->primal_obj - r->dual_obj) / (1.0 + jmax(fabs(r->primal_obj), fabs(r->dual_obj)));
+    const double qx_inf = skip_q ? 0.0 : rn[5];
+    r->err_Rd = rn[3] / (1.0 + jmax(jmax(h->norm_c_org, rn[4]), qx_inf));
+    if (m > 0) {
+        const double denom = 1.0 + jmax(h->norm_b_org, rm[4]);
+        r->err_Rp = jmax(rm[2], rm[3]) / denom;
+        r->err_Rp_linear = h->m1 > 0 ? rm[2] / denom : 0.0;
+        r->err_Rp_soc = h->m1 < m ? rm[3] / denom : 0.0;
+    } else {
+        r->err_Rp = r->err_Rp_linear = r->err_Rp_soc = 0.0;
+    }
+    if (iter == 0) r->err_Rp = jmax(r->err_Rp, v[SL_BV]);
+    r->kkt = jmax(jmax(r->err_Rp, r->err_Rd), r->rel_gap);
+}
+
+int hprsocp_compute_residuals(hprsocp_handle *h, int64_t iter, hprsocp_residuals *out) {
+    ABI_TRY(h, { residuals_impl(h, iter, out); });
+    return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ M-norm
+namespace {
+
+// Enqueues the reductions of compute_M_norm! (algorithm.jl:1302-1359) into 8
+// device slots starting at `base`; no host synchronisation.
+struct MnormCtx {
+    double sigma, lambda_A, lambda_Q;
+    bool pending = false;
+};
+
+void mnorm_enqueue(hprsocp_handle *h, int base) {
+    const int32_t m = h->m, n = h->n;
+    const bool q = h->has_quad;
+    hipStream_t st = h->st;
+    if (q) qmap(h, h->dw, h->dqw);
+    if (m > 0) {
+        launch_spmv(st, h->plan_AT, h->AT, h->dy, h->atdy);
+        if (q) qmap(h, h->atdy, h->qatdy);
+    }
+    std::vector<RedTerm> tn;
+    tn.push_back(RedTerm{h->dx, h->dx, nullptr, 0});                               // d0 = |dx|^2
+    tn.push_back(q ? RedTerm{h->dqw, h->dx, nullptr, 0} : RedTerm{h->dx, h->dx, nullptr, 0});  // d1
+    tn.push_back(m > 0 && q ? RedTerm{h->dqw, h->atdy, nullptr, 0} : RedTerm{h->dx, h->dx, nullptr, 0});  // d3
+    tn.push_back(m > 0 ? RedTerm{h->atdy, h->dx, nullptr, 0} : RedTerm{h->dx, h->dx, nullptr, 0});       // d4
+    if (q && h->Q_is_diag) {
+        tn.push_back(m > 0 ? RedTerm{h->atdy, h->fact_m, h->qatdy, 0} : RedTerm{h->dx, h->dx, nullptr, 0});  // d5
+        tn.push_back(RedTerm{h->dqw, h->dqw, nullptr, 0});                                                  // d6
+    } else if (q) {
+        tn.push_back(m > 0 ? RedTerm{h->atdy, h->qatdy, nullptr, 0} : RedTerm{h->dx, h->dx, nullptr, 0});
+        tn.push_back(RedTerm{h->dw, h->dqw, nullptr, 0});
+    }
+    launch_reduce(st, n, tn.data(), (int)tn.size(), h->red_part, reduce_blocks(n), slot(h, base));
+    if (m > 0) {
+        RedTerm ty{h->dy, h->dy, nullptr, 0};
+        launch_reduce(st, m, &ty, 1, h->red_part + (int64_t)h->red_blocks * kMaxRed * 2, reduce_blocks(m),
+                      slot(h, base + 7));  // d2
+    } else {
+        launch_fill(st, 1, 0.0, slot(h, base + 7));
+    }
+}
+
+double mnorm_finish(hprsocp_handle *h, const double *d, const MnormCtx &c) {
+    const bool q = h->has_quad;
+    const int32_t m = h->m;
+    const double sig = c.sigma;
+    double M1 = 0.0, M2 = (1.0 / sig) * d[0], M3 = 0.0;
+    if (q) M2 -= 2.0 * d[1];
+    if (m > 0) {
+        M1 = sig * c.lambda_A * d[7];
+        if (q) M1 -= 2.0 * sig * d[2];
+        M2 += 2.0 * d[3];
+        if (q) {
+            if (h->Q_is_diag) {
+                M3 = d[4];
+                M1 += sig * d[5];
+            } else {
+                M3 = (sig * sig) / (1.0 + sig * c.lambda_Q) * d[4];
+                M1 += sig * c.lambda_Q * d[5];
+            }
+        }
+    } else if (!h->Q_is_diag && q) {
+        M1 = sig * c.lambda_Q * d[5];
+    }
+    M2 += jmax(M1, 0.0);
+    const double Mn = jmax(M2, 0.0) + jmax(M3, 0.0);
+    return sqrt(Mn);
+}
+
+// ------------------------------------------------------------------ restart
+struct Restart {  // HPRSOCP_restart, structs.jl:600-623
+    int restart_flag = 0;
+    bool first_restart = true;
+    double last_gap = INFINITY, current_gap = INFINITY, save_gap = INFINITY;
+    int64_t inner = 0, times = 0, sufficient = 0, necessary = 0, longr = 0;
+    double weighted_norm = INFINITY, best_gap = INFINITY, best_kkt = INFINITY, best_sigma = 1.0;
+    int64_t best_iter = 0;
+    bool corr_active = false;
+    int corr_hold = 0, corr_dir = 0;
+};
+
+// check_restart, algorithm.jl:1016-1099
+void check_restart(Restart &r, int64_t iter, int64_t check_iter, double sigma, double kkt) {
+    r.restart_flag = 0;
+    if (r.first_restart) {
+        if (iter == check_iter) {
+            r.first_restart = false;
+            r.restart_flag = 1;
+            r.weighted_norm = r.current_gap;
+            r.best_gap = r.current_gap;
+            r.best_kkt = kkt;
+            r.best_sigma = sigma;
+        }
+        return;
+    }
+    if (iter % check_iter != 0) return;
+    if (r.current_gap < 0) r.current_gap = 1e-6;
+    if (r.current_gap <= 0.36 * r.last_gap) {
+        r.sufficient++;
+        r.restart_flag = 1;
+    }
+    if (r.current_gap <= 0.8 * r.last_gap && r.current_gap > 1.00 * r.save_gap) {
+        r.necessary++;
+        r.restart_flag = 2;
+    }
+    const double fact = 0.15;
+    if (((double)r.inner >= jmax(500.0, fact * (double)iter) ||
+         (r.inner >= 10000 && r.current_gap <= 0.75 * r.last_gap)) &&
+        r.current_gap > 0.9 * r.save_gap) {
+        r.longr++;
+        r.restart_flag = 3;
+    }
+    // should_refresh_best_sigma, algorithm.jl:682-693
+    const bool improved = kkt < r.best_kkt;
+    const bool compat = r.current_gap < r.best_gap && kkt <= 1.01 * r.best_kkt;
+    const bool periodic = r.best_iter >= 10 && kkt <= 1.2 * r.best_kkt;
+    if (improved || compat || periodic) {
+        r.best_gap = r.current_gap;
+        r.best_kkt = kkt;
+        r.best_sigma = sigma;
+        r.best_iter = 0;
+    }
+    r.save_gap = r.current_gap;
+}
+
+double f_dev(double x, double a, double b, double c, double d) { return a * x + b / x + c * x * x / (1 + d * x); }
+
+// golden, kernels.jl:6272-6305
+double golden_scalar(double ap, double bp, double cp, double dp, double lo, double hi, double tol) {
+    const double phi = (sqrt(5.0) - 1.0) / 2.0;
+    double a = lo, b = hi, c = b - phi * (b - a), d = a + phi * (b - a);
+    double fc = f_dev(c, ap, bp, cp, dp), fd = f_dev(d, ap, bp, cp, dp);
+    for (int i = 0; i < 200; ++i) {
+        if (fd < fc) {
+            a = c;
+            c = d;
+            fc = fd;
+            d = a + phi * (b - a);
+            fd = f_dev(d, ap, bp, cp, dp);
+        } else {
+            b = d;
+            d = c;
+            fd = fc;
+            c = b - phi * (b - a);
+            fc = f_dev(c, ap, bp, cp, dp);
+        }
+        if ((b - a) < tol) break;
+    }
+    return 0.5 * (a + b);
+}
+
+// unified_golden_Q_diag (unified_operations.jl:411-449) replayed exactly with
+// speculative batches: from a known state, the next L steps can only visit the
+// 2^L - 1 points of a binary decision tree; all of them are evaluated in one
+// device pass (g(x) = sum_i ATdy_i * QATdy_i / (1 + x d_i)), then the host walks
+// the tree with the reference's comparisons.
+struct GState {
+    double lo, hi, x1, x2, f1, f2;
+};
+
+double golden_q_diag(hprsocp_handle *h, double a, double b, double lo, double hi, double tol) {
+    const double phi = (sqrt(5.0) - 1.0) / 2.0;
+    const int L = 5, NP = (1 << L) - 1;
+    const int nb = std::min(reduce_blocks(h->n), 1024);
+    auto eval = [&](const std::vector<double> &xs, std::vector<double> &gs) {
+        memcpy(h->hscal + SL_GOLD, xs.data(), sizeof(double) * xs.size());
+        HIPCHK(hipMemcpyAsync(h->dscal + SL_GOLD, h->hscal + SL_GOLD, sizeof(double) * xs.size(), hipMemcpyHostToDevice, h->st));
+        launch_golden_eval(h->st, h->n, h->atdy, h->qatdy, h->diag_Q, h->dscal + SL_GOLD, (int)xs.size(), h->red_part,
+                           nb, h->dscal + SL_GOLD);
+        const double *r = read_slots(h, SL_GOLD, (int)xs.size());
+        gs.assign(r, r + xs.size());
+    };
+    auto f_of = [&](double x, double g) { return a * x + b / x + x * x * g; };
+    GState S;
+    S.lo = lo;
+    S.hi = hi;
+    S.x1 = hi - phi * (hi - lo);
+    S.x2 = lo + phi * (hi - lo);
+    {
+        std::vector<double> xs{S.x1, S.x2}, gs;
+        eval(xs, gs);
+        S.f1 = f_of(S.x1, gs[0]);
+        S.f2 = f_of(S.x2, gs[1]);
+    }
+    int iter = 0;
+    auto cont = [&](const GState &g, int it) { return fabs(g.hi - g.lo) > tol * jmax(1.0, fabs(g.lo)) && it < 200; };
+    // step of the reference loop with the new point's f unknown (returned via nx)
+    auto step = [&](const GState &g, bool branchA, GState &o, double &nx) {
+        o = g;
+        if (branchA) {  // f1 > f2
+            o.lo = g.x1;
+            o.x1 = g.x2;
+            o.f1 = g.f2;
+            o.x2 = o.lo + phi * (o.hi - o.lo);
+            nx = o.x2;
+        } else {
+            o.hi = g.x2;
+            o.x2 = g.x1;
+            o.f2 = g.f1;
+            o.x1 = o.hi - phi * (o.hi - o.lo);
+            nx = o.x1;
+        }
+    };
+    while (cont(S, iter)) {
+        // build the decision tree: node k's state has one unknown f at point pts[k]
+        std::vector<GState> node(NP);
+        std::vector<double> pts(NP);
+        std::vector<char> isA(NP);
+        step(S, S.f1 > S.f2, node[0], pts[0]);
+        isA[0] = S.f1 > S.f2;
+        for (int k = 0; 2 * k + 2 < NP; ++k) {
+            // child 2k+1: decision "f1 > f2" true at node k; child 2k+2: false
+            step(node[k], true, node[2 * k + 1], pts[2 * k + 1]);
+            isA[2 * k + 1] = 1;
+            step(node[k], false, node[2 * k + 2], pts[2 * k + 2]);
+            isA[2 * k + 2] = 0;
+        }
+        std::vector<double> gs;
+        eval(pts, gs);
+        int k = 0;
+        iter++;
+        while (true) {
+            GState &g = node[k];
+            const double fv = f_of(pts[k], gs[k]);
+            if (isA[k]) g.f2 = fv;
+            else g.f1 = fv;
+            S = g;
+            if (!cont(S, iter)) break;
+            const int child = (S.f1 > S.f2) ? 2 * k + 1 : 2 * k + 2;
+            if (child >= NP) break;
+            k = child;
+            iter++;
+        }
+    }
+    return (S.f1 < S.f2) ? S.x1 : S.x2;
+}
+
+// update_sigma!, algorithm.jl:729-990
+void update_sigma(hprsocp_handle *h, Restart &r, const hprsocp_residuals &res) {
+    if (r.restart_flag < 1) return;
+    hipStream_t st = h->st;
+    const int32_t m = h->m, n = h->n;
+    const bool skip_q = !h->has_quad, has_m = m > 0, qd = h->Q_is_diag;
+    const double sigma_old = h->sigma;
+    const double SMIN = 1e-12, SMAX = 1e12, THETA_MAX = 1.0, IMB = 1e6, FLOOR = 2.0, THR = 1e-8, EPS = 1e-16;
+    const double LOG100 = log(100.0);
+    if (!skip_q) {
+        launch_axpby(st, n, 1.0, h->w_bar, -1.0, h->last_w, h->dw);
+        qmap(h, h->dw, h->dqw);
+    }
+    launch_axpby(st, n, 1.0, h->x_bar, -1.0, h->last_x, h->dx);
+    if (has_m) {
+        launch_axpby(st, m, 1.0, h->y_bar, -1.0, h->last_y, h->dy);
+        if (!skip_q) {
+            launch_axpby(st, n, 1.0, h->aty_bar, -1.0, h->last_aty, h->atdy);
+            qmap(h, h->atdy, h->qatdy);
+        } else {
+            launch_spmv(st, h->plan_AT, h->AT, h->dy, h->atdy);
+        }
+    }
+    // n-side dots: b=|dx|^2, <dQw,ATdy>, |dQw|^2, <dw,dQw>, <ATdy,QATdy>
+    std::vector<RedTerm> tn{RedTerm{h->dx, h->dx, nullptr, 0}};
+    if (!skip_q) {
+        tn.push_back(has_m ? RedTerm{h->dqw, h->atdy, nullptr, 0} : RedTerm{h->dx, h->dx, nullptr, 0});
+        tn.push_back(RedTerm{h->dqw, h->dqw, nullptr, 0});
+        tn.push_back(RedTerm{h->dw, h->dqw, nullptr, 0});
+        tn.push_back(has_m ? RedTerm{h->atdy, h->qatdy, nullptr, 0} : RedTerm{h->dx, h->dx, nullptr, 0});
+    }
+    launch_reduce(st, n, tn.data(), (int)tn.size(), h->red_part, reduce_blocks(n), slot(h, SL_SIG));
+    if (has_m) {
+        RedTerm ty{h->dy, h->dy, nullptr, 0};
+        launch_reduce(st, m, &ty, 1, h->red_part + (int64_t)h->red_blocks * kMaxRed * 2, reduce_blocks(m),
+                      slot(h, SL_SIG + 7));
+    }
+    const double *d = read_slots(h, SL_SIG, 8);
+    const double dd_x = d[0], dd_y = has_m ? d[7] : 0.0;
+    double a = 0.0, b = dd_x, c = 0.0, dl = 0.0;
+    if (has_m) {
+        if (!skip_q) a = h->lambda_A * dd_y - 2.0 * d[1];
+        else a = h->lambda_A * dd_y;
+    }
+    if (!skip_q) {
+        if (qd) {
+            const double nq = sqrt(d[2]);
+            a += nq * nq;
+        } else {
+            a += h->lambda_Q * d[3];
+            if (has_m) {
+                c = d[4];
+                dl = h->lambda_Q;
+            }
+        }
+    }
+    a = jmax(a, EPS);
+    b = jmax(b, EPS);
+    double sm;
+    if (skip_q) sm = sqrt(b / a);
+    else if (qd) sm = has_m ? golden_q_diag(h, a, b, 1e-12, 1e12, 1e-13) : sqrt(b / a);
+    else sm = has_m ? golden_scalar(a, b, c, dl, 1e-12, 1e12, 1e-13) : sqrt(b / a);
+
+    const double sref = r.best_sigma, ck = res.kkt, cg = r.current_gap, bk = r.best_kkt, bg = r.best_gap;
+    const bool ref_ok = (ck <= 10.0 * bk) && (cg <= 10.0 * bg);
+    double pr = sqrt((ck / bk) * (cg / bg));
+    pr = jclamp(pr, 1.0, 10.0);
+    double sbase = sm;
+    if (ref_ok) {
+        const double theta = jclamp(exp(-0.3 * (pr - 1.0)), 0.0, THETA_MAX);
+        sbase = exp(theta * log(sm) + (1.0 - theta) * log(sref));
+        sbase = jclamp(sbase, SMIN, SMAX);
+    }
+    const double ep = jmax(res.err_Rp, EPS), ed = jmax(res.err_Rd, EPS);
+    const double es = jmin(ep, ed), el = jmax(ep, ed), ir = el / es;
+    const bool both = el <= THR;
+    const bool one = (es <= 0.5 * THR) && (el >= 2.0 * THR);
+    const bool extreme = ir >= IMB;
+    const bool trig = !both && (one || extreme);
+    const bool keep = !both && ((es <= 0.8 * THR) || (ir >= 0.1 * IMB));
+    const int cdir = ed >= ep ? 1 : -1;
+    if (both) {
+        r.corr_active = false;
+        r.corr_hold = -1;
+        r.corr_dir = 0;
+    } else if (trig || (r.corr_active && keep)) {
+        r.corr_active = true;
+        r.corr_hold = 0;
+        r.corr_dir = cdir;
+    } else {
+        r.corr_active = false;
+        r.corr_hold = -1;
+        r.corr_dir = 0;
+    }
+    double beta = 0.0;
+    if (!both) {
+        if (one) beta = 0.5;
+        if (extreme) beta = 1.0;
+        else if (r.corr_active) beta = 0.5;
+    }
+    double kappa = 1.0;
+    if (beta > 0.0) {
+        double rk = ed / ep;
+        if (r.corr_active) {
+            if (r.corr_dir > 0) rk = jmax(rk, FLOOR);
+            else if (r.corr_dir < 0) rk = jmin(rk, 1.0 / FLOOR);
+        }
+        kappa = exp(beta * jclamp(log(rk), -LOG100, LOG100));
+    }
+    double sn = sbase * kappa;
+    if (r.corr_active) {
+        if (r.corr_dir > 0) sn = jmax(sn, sigma_old * 1.5);
+        else if (r.corr_dir < 0) sn = jmin(sn, sigma_old / 1.5);
+    }
+    sn = jclamp(sn, SMIN, SMAX);
+    h->sigma = sn;
+    r.best_iter += 1;
+    if (qd && !skip_q && fabs(sigma_old - sn) > 1e-15) update_q_factors(h);
+}
+
+// do_restart!, algorithm.jl:1130-1166
+void do_restart(hprsocp_handle *h, Restart &r) {
+    if (r.restart_flag <= 0) return;
+    hipStream_t st = h->st;
+    const int32_t m = h->m, n = h->n;
+    launch_copy(st, n, h->x_bar, h->x);
+    launch_copy(st, n, h->x_bar, h->last_x);
+    if (h->has_quad) {
+        launch_copy(st, n, h->w_bar, h->w);
+        launch_copy(st, n, h->w_bar, h->last_w);
+    }
+    if (m > 0) {
+        launch_copy(st, m, h->y_bar, h->y);
+        launch_copy(st, m, h->y_bar, h->last_y);
+        if (h->has_quad) {
+            launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->aty_bar);
+            launch_copy(st, n, h->aty_bar, h->last_aty);
+            launch_copy(st, n, h->aty_bar, h->aty);
+        }
+    }
+    r.last_gap = r.current_gap;
+    r.save_gap = INFINITY;
+    r.times += 1;
+    r.inner = 0;
+}
+
+double print_step(int64_t iter) {
+    const double e = floor(log10((double)iter));
+    return jmax(pow(10.0, e) / 10.0, 10.0);
+}
+bool rem_zero(int64_t iter, double step) { return fmod((double)iter, step) == 0.0; }
+
+bool should_print(const hprsocp_params &p, int64_t iter, double t_start) {
+    if (p.print_frequency == -1)
+        return rem_zero(iter, print_step(iter)) || iter == p.max_iter || (now_s() - t_start > p.time_limit);
+    return (iter % p.print_frequency == 0) || iter == p.max_iter || (now_s() - t_start > p.time_limit);
+}
+
+void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
+    const hprsocp_params &p = h->params;
+    const double t_start = now_s();
+    if (!h->lambda_set) hprsocp_estimate_lambda(h);
+    Restart r;
+    r.best_sigma = h->sigma;
+    if (h->Q_is_diag) update_q_factors(h);
+    if (p.verbose) {
+        printf("HPRSOCP SOLVER starts (MI355X HIP backend)...\n");
+        printf(" iter     errRp     errRd         p_obj           d_obj          gap        sigma       time\n");
+    }
+    hprsocp_residuals rs{};
+    int64_t it4 = 0, it6 = 0;
+    double t4 = 0, t6 = 0;
+    bool first4 = true, first6 = true;
+    const int64_t check_iter = p.check_iter;
+    // deferred M-norms: slot 0 -> last_gap candidate, slot 1 -> current_gap
+    MnormCtx mn[2];
+    h->to_check = 1;
+    for (int64_t iter = 0; iter <= p.max_iter; ++iter) {
+        const bool print_yes = should_print(p, iter, t_start);
+        if (iter % check_iter == 0 || print_yes) residuals_impl(h, iter, &rs);  // synchronises the stream
+        if (iter % check_iter == 0 && (mn[0].pending || mn[1].pending)) {
+            // deferred M-norms (perform_iteration_step!, algorithm.jl:2731-2741); the gaps are
+            // only consumed by check_restart at multiples of check_iter
+            read_slots(h, SL_MN0, 16);
+            for (int k = 0; k < 2; ++k)
+                if (mn[k].pending) {
+                    const double g = mnorm_finish(h, h->hscal + (k ? SL_MN1 : SL_MN0), mn[k]);
+                    if (k == 0) r.last_gap = jmin(g, r.last_gap);
+                    else r.current_gap = g;
+                    mn[k].pending = false;
+                }
+        }
+        int status = HPRSOCP_STATUS_CONTINUE;
+        if (rs.kkt < p.stoptol) status = HPRSOCP_STATUS_OPTIMAL;
+        else if (iter == p.max_iter) status = HPRSOCP_STATUS_MAX_ITER;
+        else if (now_s() - t_start > p.time_limit) status = HPRSOCP_STATUS_TIME_LIMIT;
+        check_restart(r, iter, check_iter, h->sigma, rs.kkt);
+        update_sigma(h, r, rs);
+        do_restart(h, r);
+        if ((print_yes || status != HPRSOCP_STATUS_CONTINUE) && p.verbose)
+            printf("%5lld    %3.2e    %3.2e    %+7.6e    %+7.6e    %3.2e    %3.2e    %6.2f\n", (long long)iter,
+                   rs.err_Rp, rs.err_Rd, rs.primal_obj, rs.dual_obj, rs.rel_gap, h->sigma, now_s() - t_start);
+        if (rs.kkt < 1e-4 && first4) {
+            t4 = now_s() - t_start;
+            it4 = iter;
+            first4 = false;
+        }
+        if (rs.kkt < 1e-6 && first6) {
+            t6 = now_s() - t_start;
+            it6 = iter;
+            first6 = false;
+        }
+        if (status != HPRSOCP_STATUS_CONTINUE) {
+            h->sync();
+            const double tt = now_s() - t_start;
+            memset(res, 0, sizeof(*res));
+            res->iter = iter;
+            res->time = tt;
+            res->power_time = h->power_time;
+            res->setup_time = h->setup_time;
+            res->residuals = rs.kkt;
+            res->primal_obj = rs.primal_obj;
+            res->dual_obj = rs.dual_obj;
+            res->gap = rs.rel_gap;
+            res->err_Rp = rs.err_Rp;
+            res->err_Rp_linear = rs.err_Rp_linear;
+            res->err_Rp_soc = rs.err_Rp_soc;
+            res->err_Rd = rs.err_Rd;
+            res->status = status;
+            res->time_4 = t4 == 0.0 ? tt : t4;
+            res->iter_4 = it4 == 0 ? iter : it4;
+            res->time_6 = t6 == 0.0 ? tt : t6;
+            res->iter_6 = it6 == 0 ? iter : it6;
+            res->sigma = h->sigma;
+            res->lambda_max_A = h->lambda_A;
+            res->lambda_max_Q = h->lambda_Q;
+            res->restarts = (int32_t)r.times;
+            if (p.verbose) {
+                printf("SOLUTION SUMMARY: status %d, iterations %lld, time %.3f s, primal %.12e, KKT %.3e\n",
+                       status, (long long)iter, tt, rs.primal_obj, rs.kkt);
+                fflush(stdout);
+            }
+            return;
+        }
+        const int64_t nx = iter + 1;
+        bool tc = (nx % check_iter == 0) || (r.restart_flag > 0);
+        if (p.print_frequency == -1) tc = tc || rem_zero(nx, print_step(nx));
+        else tc = tc || (nx % p.print_frequency == 0);
+        // perform_iteration_step!, algorithm.jl:2722-2744
+        do_update(h, r.inner, tc ? 1 : 0);
+        const bool need_last = r.restart_flag > 0, need_cur = ((iter + 1) % check_iter) == 0;
+        if (need_last) {
+            mnorm_enqueue(h, SL_MN0);
+            mn[0] = MnormCtx{h->sigma, h->lambda_A, h->lambda_Q, true};
+        }
+        if (need_cur) {
+            if (need_last) {  // same value: copy the slot block
+                launch_copy(h->st, 8, slot(h, SL_MN0), slot(h, SL_MN1));
+            } else {
+                mnorm_enqueue(h, SL_MN1);
+            }
+            mn[1] = MnormCtx{h->sigma, h->lambda_A, h->lambda_Q, true};
+        }
+        r.inner += 1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int hprsocp_mnorm(hprsocp_handle *h, double *out) {
+    ABI_TRY(h, {
+        mnorm_enqueue(h, SL_MN0);
+        read_slots(h, SL_MN0, 8);
+        *out = mnorm_finish(h, h->hscal + SL_MN0, MnormCtx{h->sigma, h->lambda_A, h->lambda_Q, true});
+    });
+    return 0;
+}
+
+int hprsocp_solve(hprsocp_handle *h, hprsocp_result *res) {
+    ABI_TRY(h, { solve_impl(h, res); });
+    return 0;
+}
+
+int hprsocp_get_solution(hprsocp_handle *h, double *x, double *y, double *z, double *w) {
+    ABI_TRY(h, {
+        hipStream_t st = h->st;
+        const int32_t m = h->m, n = h->n;
+        if (x) {
+            launch_unscale(st, n, h->x_bar, h->col_norm, h->b_scale, 0, h->tmp_n);
+            HIPCHK(hipMemcpyAsync(x, h->tmp_n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            h->sync();
+        }
+        if (w) {
+            launch_unscale(st, n, h->has_quad ? h->w_bar : h->x_bar, h->col_norm, h->b_scale, 0, h->tmp_n);
+            HIPCHK(hipMemcpyAsync(w, h->tmp_n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            h->sync();
+        }
+        if (z) {
+            launch_unscale(st, n, h->z_bar, h->col_norm, h->c_scale, 1, h->tmp_n);
+            HIPCHK(hipMemcpyAsync(z, h->tmp_n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            h->sync();
+        }
+        if (y && m > 0) {
+            launch_unscale(st, m, h->y_bar, h->row_norm, h->c_scale, 0, h->tmp_m);
+            HIPCHK(hipMemcpyAsync(y, h->tmp_m, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+            h->sync();
+        }
+    });
+    return 0;
+}
+
+int hprsocp_optimize(const hprsocp_problem *prob, const hprsocp_params *params, hprsocp_result *res, double *x,
+                     double *y, double *z, double *w) {
+    hprsocp_params p;
+    if (params) p = *params;
+    else hprsocp_default_params(&p);
+    if (p.warm_up) {  // optimize(): a 200-iteration warm-up solve first (algorithm.jl:2540-2568)
+        hprsocp_params q = p;
+        q.max_iter = 200;
+        q.verbose = 0;
+        q.warm_up = 0;
+        hprsocp_handle *hw = nullptr;
+        int rc = hprsocp_create(prob, &q, &hw);
+        if (rc) return rc;
+        hprsocp_result tmp;
+        rc = hprsocp_estimate_lambda(hw);
+        if (!rc && (q.initial_x || q.initial_y)) rc = hprsocp_set_initial(hw, q.initial_x, q.initial_y);
+        if (!rc) rc = hprsocp_solve(hw, &tmp);
+        if (rc) g_create_error = hw->err;
+        hprsocp_destroy(hw);
+        if (rc) return rc;
+    }
+    hprsocp_handle *h = nullptr;
+    int rc = hprsocp_create(prob, &p, &h);
+    if (rc) return rc;
+    rc = hprsocp_estimate_lambda(h);
+    if (!rc && (p.initial_x || p.initial_y)) rc = hprsocp_set_initial(h, p.initial_x, p.initial_y);
+    if (!rc) rc = hprsocp_solve(h, res);
+    if (!rc) rc = hprsocp_get_solution(h, x, y, z, w);
+    if (rc) g_create_error = h->err;
+    hprsocp_destroy(h);
+    return rc;
+}
+
+static double *vec_ptr(hprsocp_handle *h, int32_t id, int64_t *len) {
+    const int64_t n = h->n, m = h->m;
+    switch (id) {
+    case HPRSOCP_V_X: *len = n; return h->x;
+    case HPRSOCP_V_X_BAR: *len = n; return h->x_bar;
+    case HPRSOCP_V_X_HAT: *len = n; return h->x_hat;
+    case HPRSOCP_V_Z_BAR: *len = n; return h->z_bar;
+    case HPRSOCP_V_DX: *len = n; return h->dx;
+    case HPRSOCP_V_LAST_X: *len = n; return h->last_x;
+    case HPRSOCP_V_Y: *len = m; return h->y;
+    case HPRSOCP_V_Y_BAR: *len = m; return h->y_bar;
+    case HPRSOCP_V_DY: *len = m; return h->dy;
+    case HPRSOCP_V_LAST_Y: *len = m; return h->last_y;
+    case HPRSOCP_V_S: *len = m; return h->s;
+    case HPRSOCP_V_W: *len = n; return h->w;
+    case HPRSOCP_V_W_BAR: *len = n; return h->w_bar;
+    case HPRSOCP_V_LAST_W: *len = n; return h->last_w;
+    case HPRSOCP_V_DW: *len = n; return h->dw;
+    case HPRSOCP_V_ATY: *len = n; return h->aty;
+    case HPRSOCP_V_ATY_BAR: *len = n; return h->aty_bar;
+    case HPRSOCP_V_LAST_ATY: *len = n; return h->last_aty;
+    case HPRSOCP_V_C: *len = n; return h->c;
+    case HPRSOCP_V_L: *len = n; return h->l;
+    case HPRSOCP_V_U: *len = n; return h->u;
+    case HPRSOCP_V_AL: *len = m; return h->AL;
+    case HPRSOCP_V_AU: *len = m; return h->AU;
+    case HPRSOCP_V_SOC_RHS_FULL: *len = m; return h->rhs;
+    case HPRSOCP_V_ROW_NORM: *len = m; return h->row_norm;
+    case HPRSOCP_V_COL_NORM: *len = n; return h->col_norm;
+    case HPRSOCP_V_DIAG_Q: *len = n; return h->diag_Q;
+    case HPRSOCP_V_RP: *len = m; return h->rp;
+    case HPRSOCP_V_AX: *len = m; return h->ax;
+    case HPRSOCP_V_RD: *len = n; return h->rd;
+    default: *len = 0; return nullptr;
+    }
+}
+
+int hprsocp_get_vector(hprsocp_handle *h, int32_t id, double *out) {
+    int64_t len;
+    double *p = vec_ptr(h, id, &len);
+    if (!p) {
+        h->err = "unknown vector id";
+        return HPRSOCP_ERR_INVALID;
+    }
+    ABI_TRY(h, {
+        if (len > 0) HIPCHK(hipMemcpyAsync(out, p, sizeof(double) * len, hipMemcpyDeviceToHost, h->st));
+        h->sync();
+    });
+    return 0;
+}
+
+int hprsocp_set_vector(hprsocp_handle *h, int32_t id, const double *in) {
+    int64_t len;
+    double *p = vec_ptr(h, id, &len);
+    if (!p) {
+        h->err = "unknown vector id";
+        return HPRSOCP_ERR_INVALID;
+    }
+    ABI_TRY(h, {
+        if (len > 0) HIPCHK(hipMemcpyAsync(p, in, sizeof(double) * len, hipMemcpyHostToDevice, h->st));
+        h->sync();
+    });
+    return 0;
+}
+
+int hprsocp_get_scalars(hprsocp_handle *h, hprsocp_scalars *o) {
+    o->sigma = h->sigma;
+    o->lambda_max_A = h->lambda_A;
+    o->lambda_max_Q = h->lambda_Q;
+    o->b_scale = h->b_scale;
+    o->c_scale = h->c_scale;
+    o->norm_b = h->norm_b;
+    o->norm_c = h->norm_c;
+    o->norm_b_org = h->norm_b_org;
+    o->norm_c_org = h->norm_c_org;
+    o->Q_is_diag = h->Q_is_diag;
+    o->has_quadratic = h->has_quad;
+    return 0;
+}
+
+int hprsocp_set_scalars(hprsocp_handle *h, double sigma, double la, double lq) {
+    ABI_TRY(h, {
+        if (sigma >= 0) h->sigma = sigma;
+        if (la >= 0) h->lambda_A = la;
+        if (lq >= 0) h->lambda_Q = lq;
+        h->lambda_set = true;
+        if (h->Q_is_diag) update_q_factors(h);
+        h->sync();
+    });
+    return 0;
+}
+
+int hprsocp_get_scaled_A(hprsocp_handle *h, double *val) {
+    ABI_TRY(h, {
+        if (h->nnzA > 0) HIPCHK(hipMemcpyAsync(val, h->A_v, sizeof(double) * h->nnzA, hipMemcpyDeviceToHost, h->st));
+        h->sync();
+    });
+    return 0;
+}
+
+int hprsocp_get_dims(hprsocp_handle *h, int32_t *m, int32_t *n, int64_t *nnz) {
+    *m = h->m;
+    *n = h->n;
+    *nnz = h->nnzA;
+    return 0;
+}
+
+int hprsocp_spmv(hprsocp_handle *h, int32_t trans, const double *in, double *out) {
+    ABI_TRY(h, {
+        const int64_t lin = trans ? h->m : h->n, lout = trans ? h->n : h->m;
+        double *din = trans ? h->tmp_m : h->tmp_n;
+        double *dout = trans ? h->tmp_n2 : h->ax;
+        if (lin > 0) HIPCHK(hipMemcpyAsync(din, in, sizeof(double) * lin, hipMemcpyHostToDevice, h->st));
+        if (trans) launch_spmv(h->st, h->plan_AT, h->AT, din, dout);
+        else launch_spmv(h->st, h->plan_A, h->A, din, dout);
+        if (lout > 0) HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * lout, hipMemcpyDeviceToHost, h->st));
+        h->sync();
+    });
+    return 0;
+}
+
+// Runs `iters` partial HPR iterations (inner = 0, 1, ...) and times them with HIP
+// events on the handle's stream; ms_kernel receives the mean time of the
+// dominant fused SpMV+projection kernel(s) from a second, event-bracketed pass.
+int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_iter, double *ms_kernel) {
+    if (!h->lambda_set) {
+        h->err = "estimate_lambda must run before timing";
+        return HPRSOCP_ERR_STATE;
+    }
+    ABI_TRY(h, {
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, h->st));
+        for (int k = 0; k < iters; ++k) do_update(h, k, 0);
+        HIPCHK(hipEventRecord(e1, h->st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_iter = iters > 0 ? ms / iters : 0.0;
+        if (ms_kernel) {
+            if (h->ev.empty()) {
+                h->ev.resize(6);
+                for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
+            }
+            h->profile = true;
+            double acc[6] = {0, 0, 0, 0, 0, 0};
+            for (int k = 0; k < iters; ++k) {
+                do_update(h, k, 0);
+                HIPCHK(hipEventSynchronize(h->ev[h->has_quad ? 5 : 2]));
+                for (int j = 0; j + 1 < (h->has_quad ? 6 : 3); ++j) {
+                    float t = 0;
+                    HIPCHK(hipEventElapsedTime(&t, h->ev[j], h->ev[j + 1]));
+                    acc[j] += t;
+                }
+            }
+            h->profile = false;
+            for (int j = 0; j < 6; ++j) ms_kernel[j] = iters > 0 ? acc[j] / iters : 0.0;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+    return 0;
+}
+
+}  // extern "C"

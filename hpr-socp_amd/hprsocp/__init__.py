@@ -1,0 +1,420 @@
+"""hprsocp — MI355X (gfx950) backend for HPR-SOCP's dual Halpern Peaceman–Rachford
+iteration, with a host API that mirrors the reference package's:
+
+* ``HPRSOCP_parameters``   — reference src/structs.jl:208-278 (same names, defaults)
+* ``build_from_SOCP_data`` — reference src/utils/builders.jl:24-113 (same checks)
+* ``optimize``             — reference src/algorithm.jl:2538-2581
+* ``HPRSOCP_results``      — reference src/structs.jl:297-314
+
+Every solve runs through the C ABI of ``libhprsocp.so`` (include/hprsocp.h);
+there is no CPU fallback: if the library or a GPU is missing the call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _abi
+
+__all__ = [
+    "HPRSOCP_parameters", "HPRSOCP_results", "QP_info", "build_from_SOCP_data", "build_qp",
+    "optimize", "Solver", "load_library", "HPRSOCPError",
+]
+
+_LIB = None
+
+
+class HPRSOCPError(RuntimeError):
+    """Raised for any non-zero status returned through the C ABI."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[hprsocp error {code}] {msg}")
+        self.code = code
+
+
+def load_library(path: Optional[str] = None) -> C.CDLL:
+    """Load libhprsocp.so (built in-tree by __graft_entry__.build())."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or _abi.library_path("libhprsocp.so")
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"HIP backend library not found at {p}; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` (make -C hpr-socp_amd/csrc)")
+    lib = _abi.bind(C.CDLL(p), "hprsocp_")
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+# --------------------------------------------------------------------- params
+@dataclass
+class HPRSOCP_parameters:
+    """Solver parameters; defaults equal HPRSOCP_parameters() of the reference."""
+    stoptol: float = 1e-6
+    sigma: float = -1.0
+    max_iter: int = 2147483647
+    time_limit: float = 3600.0
+    check_iter: int = 150
+    warm_up: bool = False
+    print_frequency: int = -1
+    device_number: int = 0
+    use_Ruiz_scaling: bool = True
+    ruiz_iterations: int = 10
+    use_bc_scaling: bool = True
+    bc_scaling_norm_type: str = "l2"
+    use_l2_scaling: bool = False
+    use_Pock_Chambolle_scaling: bool = True
+    soc_block_scaling_strategy: str = "phase_taper"
+    initial_x: Optional[np.ndarray] = None
+    initial_y: Optional[np.ndarray] = None
+    auto_save: bool = False
+    save_filename: str = "HPRSOCP_autosave.h5"
+    verbose: bool = True
+    use_gpu: bool = True
+    # extensions of this backend
+    precision: str = "fp64"          # "fp64" | "fp32_refine"
+    refine_every: int = 1
+    use_graph: bool = True
+
+    def to_c(self, keep: list) -> _abi.Params:
+        p = _abi.Params()
+        p.stoptol = float(self.stoptol)
+        p.sigma = float(self.sigma)
+        p.max_iter = int(self.max_iter)
+        p.time_limit = float(self.time_limit)
+        p.check_iter = int(self.check_iter)
+        p.warm_up = int(bool(self.warm_up))
+        p.print_frequency = int(self.print_frequency)
+        p.device_number = int(self.device_number)
+        p.use_Ruiz_scaling = int(bool(self.use_Ruiz_scaling))
+        p.ruiz_iterations = int(self.ruiz_iterations)
+        p.use_bc_scaling = int(bool(self.use_bc_scaling))
+        key = str(self.bc_scaling_norm_type).lstrip(":")
+        if key not in _abi.BC_NORMS:
+            raise ValueError(f"Unsupported bc scaling norm_type: {self.bc_scaling_norm_type}. "
+                             "Use :l2, :rms, or :linf.")
+        p.bc_scaling_norm_type = _abi.BC_NORMS[key]
+        p.use_l2_scaling = int(bool(self.use_l2_scaling))
+        p.use_Pock_Chambolle_scaling = int(bool(self.use_Pock_Chambolle_scaling))
+        skey = str(self.soc_block_scaling_strategy).lstrip(":")
+        if skey not in _abi.SOC_STRATEGIES:
+            raise ValueError(f"Unsupported soc_block_scaling_strategy={self.soc_block_scaling_strategy}")
+        p.soc_block_scaling_strategy = _abi.SOC_STRATEGIES[skey]
+        if self.initial_x is not None:
+            a = _abi.f64(self.initial_x)
+            keep.append(a)
+            p.initial_x = _abi.ptr_f64(a)
+        if self.initial_y is not None:
+            a = _abi.f64(self.initial_y)
+            keep.append(a)
+            p.initial_y = _abi.ptr_f64(a)
+        p.auto_save = int(bool(self.auto_save))
+        p.verbose = int(bool(self.verbose))
+        p.use_gpu = int(bool(self.use_gpu))
+        p.precision = {"fp64": _abi.PREC_FP64, "fp32_refine": _abi.PREC_FP32_REFINE}[self.precision]
+        p.refine_every = int(self.refine_every)
+        p.use_graph = int(bool(self.use_graph))
+        return p
+
+
+@dataclass
+class HPRSOCP_results:
+    """Mirrors HPRSOCP_results (structs.jl:297-314) plus a few diagnostics."""
+    iter: int = 0
+    iter_4: int = 0
+    iter_6: int = 0
+    time: float = 0.0
+    time_4: float = 0.0
+    time_6: float = 0.0
+    power_time: float = 0.0
+    primal_obj: float = 0.0
+    residuals: float = 0.0
+    gap: float = 0.0
+    status: str = ""
+    x: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    y: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    z: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    w: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    dual_obj: float = 0.0
+    err_Rp: float = 0.0
+    err_Rd: float = 0.0
+    setup_time: float = 0.0
+    sigma: float = 0.0
+    lambda_max_A: float = 0.0
+    lambda_max_Q: float = 0.0
+    restarts: int = 0
+
+    @classmethod
+    def from_c(cls, r: _abi.Result, x, y, z, w) -> "HPRSOCP_results":
+        return cls(iter=r.iter, iter_4=r.iter_4, iter_6=r.iter_6, time=r.time, time_4=r.time_4,
+                   time_6=r.time_6, power_time=r.power_time, primal_obj=r.primal_obj,
+                   residuals=r.residuals, gap=r.gap, status=_abi.STATUS_NAMES[r.status],
+                   x=x, y=y, z=z, w=w, dual_obj=r.dual_obj, err_Rp=r.err_Rp, err_Rd=r.err_Rd,
+                   setup_time=r.setup_time, sigma=r.sigma, lambda_max_A=r.lambda_max_A,
+                   lambda_max_Q=r.lambda_max_Q, restarts=r.restarts)
+
+
+# --------------------------------------------------------------------- model
+@dataclass
+class QP_info:
+    """Canonical problem, QP_info_cpu (structs.jl:97-147).
+
+    ``SOC_con_idx`` / ``SOC_var_idx`` keep the reference's 1-based convention
+    (start of each cone, sentinel m+1 / n+1).  ``Q`` is a scipy sparse matrix or
+    ``("factored", F, diag)`` for the operator form Q = F'F + diag.
+    """
+    Q: object
+    c: np.ndarray
+    A: sp.csr_matrix
+    soc_rhs: np.ndarray
+    soc_rhs_full: np.ndarray
+    AL: np.ndarray
+    AU: np.ndarray
+    SOC_con_idx: np.ndarray
+    number_linear_con: int
+    l: np.ndarray
+    u: np.ndarray
+    SOC_var_idx: np.ndarray
+    number_lu_x: int
+    obj_constant: float = 0.0
+
+    @property
+    def AT(self):
+        return self.A.T.tocsr()
+
+    @property
+    def shape(self):
+        return self.A.shape
+
+    def buffers(self) -> _abi.ProblemBuffers:
+        F = Qd = None
+        Q = self.Q
+        if isinstance(Q, tuple) and Q and Q[0] == "factored":
+            F, Qd = Q[1], Q[2]
+            Q = None
+        return _abi.ProblemBuffers(
+            self.A, self.c, self.AL, self.AU, self.l, self.u, self.soc_rhs, self.number_linear_con,
+            self.number_lu_x, np.asarray(self.SOC_con_idx) - 1, np.asarray(self.SOC_var_idx) - 1,
+            Q=Q, F=F, Q_diag=Qd, obj_constant=self.obj_constant)
+
+
+def build_from_SOCP_data(Q, c, A, rhs, SOC_con_idx, number_eq, number_ineq, l, u, SOC_var_idx,
+                         obj_constant: float = 0.0, verbose: bool = True) -> QP_info:
+    """Canonical mixed linear/SOC model; same contract and errors as the
+    reference builder (builders.jl:24-113): rows = [eq | ineq (>= rhs) | SOC],
+    columns = [linear | SOC blocks]; SOC index vectors are 1-based."""
+    Q = sp.csr_matrix(Q, dtype=np.float64)
+    A = sp.csr_matrix(A, dtype=np.float64)
+    c = np.array(c, dtype=np.float64)
+    rhs = np.array(rhs, dtype=np.float64)
+    l = np.array(l, dtype=np.float64)
+    u = np.array(u, dtype=np.float64)
+    soc_con_idx = np.array(SOC_con_idx, dtype=np.int64)
+    soc_var_idx = np.array(SOC_var_idx, dtype=np.int64)
+    m, n = A.shape
+    if len(c) != n:
+        raise ValueError("Dimension mismatch: length(c) must equal number of columns of A.")
+    if len(rhs) != m:
+        raise ValueError("Dimension mismatch: length(rhs) must equal number of rows of A.")
+    if len(l) != n:
+        raise ValueError("Dimension mismatch: length(l) must equal number of variables.")
+    if len(u) != n:
+        raise ValueError("Dimension mismatch: length(u) must equal number of variables.")
+    if number_eq < 0:
+        raise ValueError("number_eq must be nonnegative.")
+    if number_ineq < 0:
+        raise ValueError("number_ineq must be nonnegative.")
+    if number_eq + number_ineq > m:
+        raise ValueError("number_eq + number_ineq cannot exceed the number of rows of A.")
+    if len(soc_con_idx) == 0:
+        raise ValueError("SOC_con_idx must contain at least one sentinel entry.")
+    if len(soc_var_idx) == 0:
+        raise ValueError("SOC_var_idx must contain at least one sentinel entry.")
+    if soc_con_idx[0] != number_eq + number_ineq + 1:
+        raise ValueError("SOC_con_idx must start at number_eq + number_ineq + 1.")
+    if soc_con_idx[-1] != m + 1:
+        raise ValueError("SOC_con_idx must end at m + 1.")
+    nl = number_eq + number_ineq
+    soc_rhs = rhs[nl:].copy()
+    soc_rhs_full = np.zeros(m)
+    soc_rhs_full[nl:] = soc_rhs
+    number_lu_x = int(soc_var_idx[0] - 1)
+    if soc_var_idx[-1] != n + 1:
+        raise ValueError("SOC_var_idx must end at n + 1.")
+    if not (0 <= number_lu_x <= n):
+        raise ValueError("Invalid SOC_var_idx start.")
+    AL = np.full(m, -np.inf)
+    AU = np.full(m, np.inf)
+    AL[:number_eq] = rhs[:number_eq]
+    AU[:number_eq] = rhs[:number_eq]
+    AL[number_eq:nl] = rhs[number_eq:nl]
+    return QP_info(Q=Q, c=c, A=A, soc_rhs=soc_rhs, soc_rhs_full=soc_rhs_full, AL=AL, AU=AU,
+                   SOC_con_idx=soc_con_idx, number_linear_con=nl, l=l, u=u,
+                   SOC_var_idx=soc_var_idx, number_lu_x=number_lu_x,
+                   obj_constant=float(obj_constant))
+
+
+def build_qp(A, c, AL, AU, l, u, soc_rhs, number_linear_con, number_lu_x, soc_con_idx0,
+             soc_var_idx0, Q=None, obj_constant: float = 0.0) -> QP_info:
+    """QP_info with two-sided linear rows [AL, AU] (the struct form the
+    reference's builder fills).  Cone offsets here are 0-based with sentinel."""
+    A = sp.csr_matrix(A, dtype=np.float64)
+    m, n = A.shape
+    soc_rhs = np.asarray(soc_rhs, dtype=np.float64)
+    full = np.zeros(m)
+    full[number_linear_con:] = soc_rhs
+    if Q is None:
+        Q = sp.csr_matrix((n, n))
+    return QP_info(Q=Q, c=np.asarray(c, float), A=A, soc_rhs=soc_rhs, soc_rhs_full=full,
+                   AL=np.asarray(AL, float), AU=np.asarray(AU, float),
+                   SOC_con_idx=np.asarray(soc_con_idx0, np.int64) + 1,
+                   number_linear_con=int(number_linear_con), l=np.asarray(l, float),
+                   u=np.asarray(u, float), SOC_var_idx=np.asarray(soc_var_idx0, np.int64) + 1,
+                   number_lu_x=int(number_lu_x), obj_constant=float(obj_constant))
+
+
+# --------------------------------------------------------------------- solver
+class Solver:
+    """One backend handle (prepared, scaled problem resident on the GPU).
+
+    Exposes the reference's per-step seams (estimate_eigenvalues, the update,
+    compute_residuals!, compute_M_norm!) for tests and tooling.  ``lib`` and
+    ``prefix`` default to the HIP backend.
+    """
+
+    def __init__(self, model: QP_info, params: Optional[HPRSOCP_parameters] = None,
+                 lib: Optional[C.CDLL] = None, prefix: str = "hprsocp_"):
+        self.lib = lib if lib is not None else load_library()
+        self.prefix = prefix
+        self.model = model
+        self.params = params or HPRSOCP_parameters(verbose=False)
+        self._keep = []
+        self._buf = model.buffers()
+        cp = self.params.to_c(self._keep)
+        h = C.c_void_p()
+        rc = self._fn("create")(C.byref(self._buf.problem), C.byref(cp), C.byref(h))
+        if rc != 0:
+            msg = self._fn("last_error")(None)
+            raise HPRSOCPError(rc, msg.decode() if msg else "create failed")
+        self.h = h
+        self.m, self.n = model.A.shape
+
+    def _fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._fn("last_error")(self.h)
+            raise HPRSOCPError(rc, msg.decode() if msg else "error")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._fn("destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # seams
+    def estimate_lambda(self):
+        self._check(self._fn("estimate_lambda")(self.h))
+        return self.scalars()
+
+    def set_initial(self, x0=None, y0=None):
+        xa, ya = _abi.f64(x0), _abi.f64(y0)
+        self._check(self._fn("set_initial")(self.h, _abi.ptr_f64(xa), _abi.ptr_f64(ya)))
+
+    def update(self, inner: int, to_check: bool = True):
+        self._check(self._fn("update")(self.h, int(inner), int(bool(to_check))))
+
+    def residuals(self, it: int) -> dict:
+        r = _abi.Residuals()
+        self._check(self._fn("compute_residuals")(self.h, int(it), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in _abi.Residuals._fields_}
+
+    def mnorm(self) -> float:
+        v = C.c_double()
+        self._check(self._fn("mnorm")(self.h, C.byref(v)))
+        return v.value
+
+    def get(self, name: str) -> np.ndarray:
+        ln = self.m if name in _abi.VEC_IS_M else self.n
+        out = np.zeros(ln)
+        self._check(self._fn("get_vector")(self.h, _abi.VEC[name], _abi.ptr_f64(out)))
+        return out
+
+    def set(self, name: str, v):
+        ln = self.m if name in _abi.VEC_IS_M else self.n
+        a = _abi.f64(v)
+        if a.shape != (ln,):
+            raise ValueError(f"{name}: expected length {ln}")
+        self._check(self._fn("set_vector")(self.h, _abi.VEC[name], _abi.ptr_f64(a)))
+
+    def scalars(self) -> dict:
+        s = _abi.Scalars()
+        self._check(self._fn("get_scalars")(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _abi.Scalars._fields_}
+
+    def set_scalars(self, sigma=-1.0, lambda_A=-1.0, lambda_Q=-1.0):
+        self._check(self._fn("set_scalars")(self.h, float(sigma), float(lambda_A), float(lambda_Q)))
+
+    def scaled_A(self) -> np.ndarray:
+        out = np.zeros(self._buf.problem.A_nnz)
+        self._check(self._fn("get_scaled_A")(self.h, _abi.ptr_f64(out)))
+        return out
+
+    def spmv(self, v, trans=False) -> np.ndarray:
+        a = _abi.f64(v)
+        out = np.zeros(self.n if trans else self.m)
+        self._check(self._fn("spmv")(self.h, int(trans), _abi.ptr_f64(a), _abi.ptr_f64(out)))
+        return out
+
+    def solve(self) -> HPRSOCP_results:
+        r = _abi.Result()
+        self._check(self._fn("solve")(self.h, C.byref(r)))
+        x, y, z, w = np.zeros(self.n), np.zeros(self.m), np.zeros(self.n), np.zeros(self.n)
+        self._check(self._fn("get_solution")(self.h, _abi.ptr_f64(x), _abi.ptr_f64(y),
+                                              _abi.ptr_f64(z), _abi.ptr_f64(w)))
+        return HPRSOCP_results.from_c(r, x, y, z, w)
+
+    def time_iterations(self, iters: int, per_kernel: bool = False):
+        ms = C.c_double()
+        kern = (C.c_double * 6)()
+        self._check(self.lib.hprsocp_time_iterations(self.h, int(iters), C.byref(ms),
+                                                      kern if per_kernel else None))
+        return ms.value, list(kern) if per_kernel else None
+
+
+def optimize(model: QP_info, params: Optional[HPRSOCP_parameters] = None) -> HPRSOCP_results:
+    """optimize(model, params) of the reference (algorithm.jl:2538): optional
+    200-iteration warm-up, then setup + solve on the GPU; returns unscaled x, y, z, w."""
+    lib = load_library()
+    params = params or HPRSOCP_parameters()
+    keep = []
+    buf = model.buffers()
+    cp = params.to_c(keep)
+    m, n = model.A.shape
+    r = _abi.Result()
+    x, y, z, w = np.zeros(n), np.zeros(m), np.zeros(n), np.zeros(n)
+    rc = lib.hprsocp_optimize(C.byref(buf.problem), C.byref(cp), C.byref(r), _abi.ptr_f64(x),
+                              _abi.ptr_f64(y), _abi.ptr_f64(z), _abi.ptr_f64(w))
+    if rc != 0:
+        msg = lib.hprsocp_last_error(None)
+        raise HPRSOCPError(rc, msg.decode() if msg else "optimize failed")
+    return HPRSOCP_results.from_c(r, x, y, z, w)

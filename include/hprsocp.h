@@ -1,0 +1,278 @@
+/*
+ * hprsocp.h — C ABI of the MI355X (gfx950) backend for HPR-SOCP's dual
+ * Halpern Peaceman–Rachford (HPR) iteration.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns an hprsocp_errc (0 = success).  No exception crosses the boundary;
+ * the text of the last failure is available from hprsocp_last_error().
+ *
+ * Indexing is 0-based throughout (the reference is 1-based; the Julia shim
+ * subtracts one from rowPtr/colVal/SOC_*_idx before calling in).
+ *
+ * Canonical layout (reference src/utils/builders.jl:24-113):
+ *   rows    = [ equalities | linear inequalities | SOC row blocks ]
+ *   columns = [ boxed variables | SOC variable blocks ]
+ * soc_con_idx / soc_var_idx hold the start of every cone and end with the
+ * sentinel m (resp. n), exactly like SOC_con_idx / SOC_var_idx minus one.
+ *
+ * Threading: one handle per host thread; a handle is not re-entrant.  Each
+ * handle owns one HIP stream and all device memory it allocates.
+ */
+#ifndef HPRSOCP_H
+#define HPRSOCP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPRSOCP_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- errors */
+typedef enum {
+    HPRSOCP_OK = 0,
+    HPRSOCP_ERR_INVALID = 1,     /* bad dimensions / indices / parameters  */
+    HPRSOCP_ERR_DEVICE = 2,      /* HIP runtime or kernel launch failure   */
+    HPRSOCP_ERR_ALLOC = 3,       /* device or host allocation failed       */
+    HPRSOCP_ERR_UNSUPPORTED = 4, /* option this backend does not provide   */
+    HPRSOCP_ERR_STATE = 5        /* call out of order (e.g. no lambda yet) */
+} hprsocp_errc;
+
+/* Termination status, reference algorithm.jl:1169-1187. */
+typedef enum {
+    HPRSOCP_STATUS_OPTIMAL = 0,
+    HPRSOCP_STATUS_MAX_ITER = 1,
+    HPRSOCP_STATUS_TIME_LIMIT = 2,
+    HPRSOCP_STATUS_CONTINUE = 3
+} hprsocp_status;
+
+/* Kind of quadratic term. */
+typedef enum {
+    HPRSOCP_Q_CSR = 0,      /* explicit sparse Q (reference SparseMatrixCSC Q)      */
+    HPRSOCP_Q_FACTORED = 1  /* operator Q = F^T F + diag(Q_diag) (reference
+                               AbstractQOperator seam, Q_operator_interface.jl:33) */
+} hprsocp_q_kind;
+
+/* bc_scaling_norm_type, reference scaling.jl:623-633. */
+typedef enum { HPRSOCP_BC_L2 = 0, HPRSOCP_BC_RMS = 1, HPRSOCP_BC_LINF = 2 } hprsocp_bc_norm;
+
+/* soc_block_scaling_strategy codes, reference scaling.jl:152-171. */
+typedef enum {
+    HPRSOCP_SOC_MAX = 1, HPRSOCP_SOC_RMS = 2, HPRSOCP_SOC_GEOM = 3,
+    HPRSOCP_SOC_ADAPTIVE = 4, HPRSOCP_SOC_HYBRID = 5, HPRSOCP_SOC_RUIZ_GEOM = 6,
+    HPRSOCP_SOC_ROLE_SPLIT = 7, HPRSOCP_SOC_POCK_SAFE_ROLE_SPLIT = 8,
+    HPRSOCP_SOC_SIZE_SPLIT_ROLE_SPLIT = 9, HPRSOCP_SOC_PRE_POCK_SIZE_SPLIT = 10,
+    HPRSOCP_SOC_PRE_POCK_HUGE_VAR_TAPER = 11,
+    HPRSOCP_SOC_PRE_POCK_HUGE_VAR_TAPER_POCK_GEOM = 12,
+    HPRSOCP_SOC_PRE_POCK_BAND_SPLIT = 13, HPRSOCP_SOC_PRE_POCK_CONSTRAINT_RMS = 14,
+    HPRSOCP_SOC_PRE_POCK_CONSTRAINT_MAX = 15, HPRSOCP_SOC_PRE_POCK_CONSTRAINT_TAPER = 16,
+    HPRSOCP_SOC_SMALL_CONE_CONSTRAINT_MAX = 17,
+    HPRSOCP_SOC_COUNT_AWARE_SMALL_CONE_BIAS = 18, HPRSOCP_SOC_PHASE_BLEND = 19,
+    HPRSOCP_SOC_PHASE_TAPER = 20
+} hprsocp_soc_strategy;
+
+/* Arithmetic of the iteration (north-star precision forms). */
+typedef enum {
+    HPRSOCP_PREC_FP64 = 0,         /* plain FP64 method (reference numerics)          */
+    HPRSOCP_PREC_FP32_REFINE = 1   /* FP32 bulk (SpMV, projections, averaging) with
+                                      FP64 residuals, scale computation, near-boundary
+                                      cone test and periodic FP64 refinement          */
+} hprsocp_precision;
+
+/* ------------------------------------------------------------ problem */
+/* Mirrors QP_info_cpu (reference src/structs.jl:97-147).  All arrays are
+ * caller-owned and only read during hprsocp_create(). */
+typedef struct {
+    int32_t m;                 /* rows of A                                   */
+    int32_t n;                 /* columns of A (variables)                    */
+    /* A, m x n CSR, 0-based int32 indices, FP64 values.  A^T is built and
+       stored by the backend (reference keeps both, builders.jl:104-105).   */
+    int64_t A_nnz;
+    const int32_t *A_rowptr;   /* m+1 */
+    const int32_t *A_colidx;   /* A_nnz */
+    const double *A_val;       /* A_nnz */
+    /* Q */
+    int32_t Q_kind;            /* hprsocp_q_kind                              */
+    int64_t Q_nnz;             /* CSR Q: n x n; 0 => no quadratic term        */
+    const int32_t *Q_rowptr;   /* n+1 (may be NULL when Q_nnz == 0)          */
+    const int32_t *Q_colidx;
+    const double *Q_val;
+    int32_t F_rows;            /* factored Q: F is F_rows x n CSR             */
+    int64_t F_nnz;
+    const int32_t *F_rowptr;
+    const int32_t *F_colidx;
+    const double *F_val;
+    const double *Q_diag;      /* factored Q: n entries (NULL => zero)        */
+    /* vectors */
+    const double *c;           /* n */
+    const double *AL;          /* m  (SOC rows: -inf)                        */
+    const double *AU;          /* m  (SOC rows: +inf)                        */
+    const double *l;           /* n */
+    const double *u;           /* n */
+    const double *soc_rhs;     /* m - number_linear_con (SOC rows only)       */
+    int32_t number_linear_con; /* rows before the first SOC row               */
+    int32_t number_lu_x;       /* boxed variables before the first SOC block  */
+    int32_t n_soc_con;         /* constraint cones                            */
+    const int32_t *soc_con_idx;/* n_soc_con+1, starts, last entry == m        */
+    int32_t n_soc_var;         /* variable cones                              */
+    const int32_t *soc_var_idx;/* n_soc_var+1, starts, last entry == n        */
+    double obj_constant;
+} hprsocp_problem;
+
+/* ------------------------------------------------------------ params */
+/* Mirrors HPRSOCP_parameters (reference src/structs.jl:208-278) with the same
+ * defaults (hprsocp_default_params).  Fields after `use_gpu` are extensions. */
+typedef struct {
+    double stoptol;            /* 1e-6                                        */
+    double sigma;              /* -1 => automatic (algorithm.jl:1538-1551)    */
+    int64_t max_iter;          /* typemax(Int32)                              */
+    double time_limit;         /* 3600 s                                      */
+    int32_t check_iter;        /* 150                                         */
+    int32_t warm_up;           /* 0                                           */
+    int32_t print_frequency;   /* -1 => adaptive                              */
+    int32_t device_number;     /* 0                                           */
+    int32_t use_Ruiz_scaling;  /* 1                                           */
+    int32_t ruiz_iterations;   /* 10                                          */
+    int32_t use_bc_scaling;    /* 1                                           */
+    int32_t bc_scaling_norm_type;       /* HPRSOCP_BC_L2                      */
+    int32_t use_l2_scaling;    /* 0 (accepted, unused, as in the reference)   */
+    int32_t use_Pock_Chambolle_scaling; /* 1                                  */
+    int32_t soc_block_scaling_strategy; /* HPRSOCP_SOC_PHASE_TAPER            */
+    const double *initial_x;   /* NULL or n                                   */
+    const double *initial_y;   /* NULL or m                                   */
+    int32_t auto_save;         /* 0 (HDF5 auto-save is not provided)          */
+    int32_t verbose;           /* 1                                           */
+    int32_t use_gpu;           /* 1 (0 is rejected: this backend is the GPU)  */
+    /* extensions */
+    int32_t precision;         /* hprsocp_precision, default FP64             */
+    int32_t refine_every;      /* FP32 mode: check windows between FP64
+                                  refinement passes (default 1)               */
+    int32_t use_graph;         /* capture the non-check iterations of a
+                                  check window in one hipGraph (default 1)    */
+    int32_t reserved[8];
+} hprsocp_params;
+
+/* ------------------------------------------------------------ results */
+/* Mirrors HPRSOCP_results (reference src/structs.jl:297-314). */
+typedef struct {
+    int64_t iter, iter_4, iter_6;
+    double time, time_4, time_6, power_time, setup_time;
+    double primal_obj, dual_obj;
+    double residuals;          /* KKT = max(err_Rp, err_Rd, rel_gap)          */
+    double gap;                /* rel_gap                                     */
+    double err_Rp, err_Rp_linear, err_Rp_soc, err_Rd;
+    double sigma, lambda_max_A, lambda_max_Q;
+    int32_t status;            /* hprsocp_status                              */
+    int32_t restarts;
+} hprsocp_result;
+
+/* Residual block, HPRSOCP_residuals (reference src/structs.jl:281-294). */
+typedef struct {
+    double err_Rp, err_Rp_linear, err_Rp_soc, err_Rd;
+    double kkt, primal_obj, rel_gap, dual_obj;
+} hprsocp_residuals;
+
+/* Solver state vectors addressable through get/set (fine-grained seams). */
+typedef enum {
+    HPRSOCP_V_X = 0, HPRSOCP_V_X_BAR, HPRSOCP_V_X_HAT, HPRSOCP_V_Z_BAR,
+    HPRSOCP_V_DX, HPRSOCP_V_LAST_X,                                   /* n */
+    HPRSOCP_V_Y, HPRSOCP_V_Y_BAR, HPRSOCP_V_DY, HPRSOCP_V_LAST_Y,
+    HPRSOCP_V_S,                                                      /* m */
+    HPRSOCP_V_W, HPRSOCP_V_W_BAR, HPRSOCP_V_LAST_W, HPRSOCP_V_DW,
+    HPRSOCP_V_ATY, HPRSOCP_V_ATY_BAR, HPRSOCP_V_LAST_ATY,             /* n */
+    HPRSOCP_V_C, HPRSOCP_V_L, HPRSOCP_V_U,                            /* n scaled */
+    HPRSOCP_V_AL, HPRSOCP_V_AU, HPRSOCP_V_SOC_RHS_FULL,               /* m scaled */
+    HPRSOCP_V_ROW_NORM,                                               /* m */
+    HPRSOCP_V_COL_NORM, HPRSOCP_V_DIAG_Q,                             /* n */
+    HPRSOCP_V_RP, HPRSOCP_V_AX,         /* m, original units after compute_residuals */
+    HPRSOCP_V_RD,                       /* n, original units after compute_residuals */
+    HPRSOCP_V_COUNT
+} hprsocp_vector_id;
+
+/* Scalar block of a handle. */
+typedef struct {
+    double sigma, lambda_max_A, lambda_max_Q;
+    double b_scale, c_scale, norm_b, norm_c, norm_b_org, norm_c_org;
+    int32_t Q_is_diag, has_quadratic;
+} hprsocp_scalars;
+
+typedef struct hprsocp_handle hprsocp_handle;
+
+/* ------------------------------------------------------------ lifecycle */
+/* Fills the reference defaults, HPRSOCP_parameters() (structs.jl:277). */
+void hprsocp_default_params(hprsocp_params *params);
+
+/* Copies the problem to the device and performs solve()'s setup up to the
+ * workspace: scaling! (scaling.jl:785), check_Q_diagonal (scaling.jl:1312),
+ * allocate_workspace (algorithm.jl:1443) incl. sigma_0.
+ * Replaces prepare_model/transfer_to_gpu (algorithm.jl:2314-2364). */
+int hprsocp_create(const hprsocp_problem *prob, const hprsocp_params *params,
+                   hprsocp_handle **out);
+void hprsocp_destroy(hprsocp_handle *h);
+/* Text of the last error on this handle (or of the last failed create when
+ * h is NULL).  Valid until the next call on the same handle. */
+const char *hprsocp_last_error(const hprsocp_handle *h);
+
+/* ------------------------------------------------------------ setup seams */
+/* estimate_eigenvalues (algorithm.jl:2434-2467): lambda_max(A A^T)*1.05 and
+ * lambda_max(Q) (power iteration, eigenvalues.jl:68-196). */
+int hprsocp_estimate_lambda(hprsocp_handle *h);
+/* process_initial_points! (algorithm.jl:2612-2677) with params.initial_x/y. */
+int hprsocp_set_initial(hprsocp_handle *h, const double *x0, const double *y0);
+
+/* ------------------------------------------------------------ hot path */
+/* One HPR step, main_update_gpu! (algorithm.jl:2284-2309) with
+ * Halpern_fact1 = 1/(inner+2); to_check selects full (1) or partial (0)
+ * outputs.  No host synchronisation. */
+int hprsocp_update(hprsocp_handle *h, int64_t inner, int32_t to_check);
+/* compute_residuals! (algorithm.jl:113-196) at iteration `iter`. */
+int hprsocp_compute_residuals(hprsocp_handle *h, int64_t iter, hprsocp_residuals *out);
+/* compute_M_norm! (algorithm.jl:1302-1359) from the current dx, dy, dw. */
+int hprsocp_mnorm(hprsocp_handle *h, double *out);
+/* The whole solve loop (algorithm.jl:2765-2905) from the current state;
+ * requires hprsocp_estimate_lambda first. */
+int hprsocp_solve(hprsocp_handle *h, hprsocp_result *res);
+/* Unscaled solution, collect_results! (algorithm.jl:1222-1256).
+ * Any pointer may be NULL.  Sizes n, m, n, n. */
+int hprsocp_get_solution(hprsocp_handle *h, double *x, double *y, double *z, double *w);
+
+/* One-shot optimize(model, params) (algorithm.jl:2538-2581). */
+int hprsocp_optimize(const hprsocp_problem *prob, const hprsocp_params *params,
+                     hprsocp_result *res, double *x, double *y, double *z, double *w);
+
+/* ------------------------------------------------------------ inspection */
+int hprsocp_get_vector(hprsocp_handle *h, int32_t id, double *out);
+int hprsocp_set_vector(hprsocp_handle *h, int32_t id, const double *in);
+int hprsocp_get_scalars(hprsocp_handle *h, hprsocp_scalars *out);
+/* Overrides sigma / lambda_max_A / lambda_max_Q (negative = keep) and
+ * refreshes the diagonal-Q factors (unified_update_Q_factors!,
+ * unified_operations.jl:468-482). */
+int hprsocp_set_scalars(hprsocp_handle *h, double sigma, double lambda_A, double lambda_Q);
+/* Scaled A values in CSR order (A_nnz entries). */
+int hprsocp_get_scaled_A(hprsocp_handle *h, double *val);
+/* Dimensions: m, n, nnz(A). */
+int hprsocp_get_dims(hprsocp_handle *h, int32_t *m, int32_t *n, int64_t *nnz);
+
+/* ------------------------------------------------------------ kernels */
+/* Stand-alone SpMV through the backend's tile plan: y = A x (trans=0) or
+ * x = A^T y (trans=1) on the handle's scaled A; host buffers in and out.
+ * (Replaces unified_mul!, unified_operations.jl:140-166.) */
+int hprsocp_spmv(hprsocp_handle *h, int32_t trans, const double *in, double *out);
+/* Time the hot loop: runs `iters` HPR iterations (partial outputs, no
+ * residual checks, Halpern counter 0..iters-1) from the current state and
+ * returns the device time per iteration measured with HIP events around the
+ * batch.  When ms_kernel is not NULL a second, event-bracketed batch fills
+ * ms_kernel[0..5] with the mean time per iteration of each launch group:
+ *   noQ: [0] A^T-SpMV + box/SOC-variable projection + Halpern,
+ *        [1] A-SpMV + linear/SOC-row projection + Halpern;
+ *   QP : [0] Q w, [1] primal projection (zxw1), [2] Q w_bar -> tempv,
+ *        [3] A-SpMV dual update, [4] A^T-SpMV w2 update.
+ * (No counterpart in the reference; it is the measurement hook.) */
+int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_iter,
+                            double *ms_kernel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HPRSOCP_H */

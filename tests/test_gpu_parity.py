@@ -1,0 +1,226 @@
+"""HIP backend vs the CPU oracle, through the C ABI, on the same seeded inputs.
+
+Tolerances (FP64 everywhere; only summation order differs):
+  * scaled data, scale factors, sigma_0:         rtol 1e-12
+  * lambda_max(AA^T), lambda_max(Q):             rtol 1e-8
+  * iterates after 30 HPR steps from one state:   |a-b|_inf <= 1e-9 (1 + |b|_inf)
+  * residual block, M-norm:                       rtol 1e-9
+  * end-to-end: GPU reaches KKT < stoptol = 1e-6, its independent certificate is
+    < 1e-5, and its primal objective agrees with an FP64 oracle solve at 1e-9 to
+    1e-6 relative (the north-star objective tolerance).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import hprsocp
+from hprsocp import synthetic
+from kat_models import kats, model_from, params, trust_region_optimum
+from kkt import certificate
+
+pytestmark = pytest.mark.gpu
+
+MODELS = {
+    "c0": lambda: synthetic.make_c0(seed=0),
+    "noq_small": lambda: synthetic.make_small_qp(seed=3, q="none"),
+    "qp_diag": lambda: synthetic.make_small_qp(seed=7, q="diag"),
+    "qp_csr": lambda: synthetic.make_small_qp(seed=8, q="csr"),
+    "qp_factored": lambda: synthetic.make_small_qp(seed=9, q="factored"),
+    # cones longer than one 256-row tile on both sides (two-phase projection) and
+    # rows long enough for the whole-workgroup SpMV path
+    "huge_cones": lambda: synthetic.make_small_qp(seed=10, n1=300, var_sizes=(700, 3, 1, 300),
+                                                  m1=40, con_sizes=(900, 5, 2), density=0.02,
+                                                  q="diag"),
+    "huge_noq": lambda: synthetic.make_small_qp(seed=11, n1=50, var_sizes=(1500,), m1=5,
+                                                con_sizes=(600,), density=0.6, q="none"),
+    "c1_tiny": lambda: synthetic.make_c1(scale=0.002),
+    "c2_tiny": lambda: synthetic.make_c2(scale=0.004),
+}
+
+_cache = {}
+
+
+def model(name):
+    if name not in _cache:
+        _cache[name] = MODELS[name]()
+    return _cache[name]
+
+
+def pair(name, oracle_lib, gpu_lib, **kw):
+    m = model(name)
+    p = params(**kw)
+    print(f"[case] {name} m={m.A.shape[0]} n={m.A.shape[1]} nnz={m.A.nnz}", flush=True)
+    g = hprsocp.Solver(m, p)
+    o = hprsocp.Solver(m, p, lib=oracle_lib, prefix="oracle_")
+    return m, g, o
+
+
+def close(a, b, tol):
+    a, b = np.asarray(a), np.asarray(b)
+    fa, fb = np.isfinite(a), np.isfinite(b)
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(a[~fa], b[~fb])
+    if fa.any():
+        err = np.max(np.abs(a[fa] - b[fb]))
+        assert err <= tol * (1 + np.max(np.abs(b[fb]))), f"max abs err {err}"
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_scaling_parity(name, oracle_lib, gpu_lib):
+    m, g, o = pair(name, oracle_lib, gpu_lib)
+    with g, o:
+        sg, so = g.scalars(), o.scalars()
+        for k in ("b_scale", "c_scale", "norm_b", "norm_c", "norm_b_org", "norm_c_org", "sigma"):
+            assert sg[k] == pytest.approx(so[k], rel=1e-12, abs=1e-300), k
+        assert sg["Q_is_diag"] == so["Q_is_diag"] and sg["has_quadratic"] == so["has_quadratic"]
+        for v in ("c", "l", "u", "AL", "AU", "soc_rhs_full", "row_norm", "col_norm", "diag_Q"):
+            close(g.get(v), o.get(v), 1e-12)
+        close(g.scaled_A(), o.scaled_A(), 1e-12)
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_lambda_parity(name, oracle_lib, gpu_lib):
+    m, g, o = pair(name, oracle_lib, gpu_lib)
+    with g, o:
+        sg, so = g.estimate_lambda(), o.estimate_lambda()
+        assert sg["lambda_max_A"] == pytest.approx(so["lambda_max_A"], rel=1e-8)
+        assert sg["lambda_max_Q"] == pytest.approx(so["lambda_max_Q"], rel=1e-8)
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_spmv_vs_scipy(name, oracle_lib, gpu_lib):
+    m, g, o = pair(name, oracle_lib, gpu_lib)
+    with g, o:
+        A = m.A.tocsr()
+        As = sp.csr_matrix((g.scaled_A(), A.indices, A.indptr), shape=A.shape)
+        rng = np.random.default_rng(0)
+        v, w = rng.standard_normal(A.shape[1]), rng.standard_normal(A.shape[0])
+        close(g.spmv(v), As @ v, 1e-13)
+        close(g.spmv(w, trans=True), As.T @ w, 1e-13)
+
+
+STATE = {"noq": ["x", "y"], "qp": ["x", "y", "w", "ATy"]}
+FULL = ["x_bar", "z_bar", "x_hat", "dx", "y_bar", "s", "dy"]
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_iteration_parity(name, oracle_lib, gpu_lib):
+    """30 HPR steps (mixed full/partial) from the same state and scalars."""
+    m, g, o = pair(name, oracle_lib, gpu_lib)
+    with g, o:
+        so = o.estimate_lambda()
+        for s in (g, o):
+            s.set_scalars(sigma=so["sigma"], lambda_A=so["lambda_max_A"], lambda_Q=so["lambda_max_Q"])
+        kind = "qp" if so["has_quadratic"] else "noq"
+        for k in range(30):
+            full = (k % 7 == 6) or k == 29
+            g.update(k, full)
+            o.update(k, full)
+        for v in STATE[kind] + FULL + (["w_bar", "dw"] if kind == "qp" else []):
+            close(g.get(v), o.get(v), 1e-9)
+        # residual block and M-norm on the same state
+        for it in (0, 1):
+            rg, ro = g.residuals(it), o.residuals(it)
+            for k in ro:
+                assert rg[k] == pytest.approx(ro[k], rel=1e-9, abs=1e-13), (it, k)
+        assert g.mnorm() == pytest.approx(o.mnorm(), rel=1e-9, abs=1e-14)
+
+
+@pytest.mark.parametrize("name", ["c0", "noq_small", "qp_diag", "qp_csr", "qp_factored",
+                                  "huge_cones", "c1_tiny", "c2_tiny"])
+def test_solve_parity(name, oracle_lib, gpu_lib):
+    m = model(name)
+    print(f"[case] solve {name}", flush=True)
+    pg = params(stoptol=1e-6, max_iter=100000, time_limit=100.0)
+    with hprsocp.Solver(m, pg) as g:
+        g.estimate_lambda()
+        rg = g.solve()
+    assert rg.status == "OPTIMAL", (rg.status, rg.residuals, rg.iter)
+    assert rg.residuals < 1e-6
+    cert = certificate(m, rg.x, rg.y, rg.z)
+    assert cert["err_Rp"] < 1e-5 and cert["err_Rd"] < 1e-5, cert
+    po = params(stoptol=1e-9, max_iter=400000, time_limit=300.0)
+    with hprsocp.Solver(m, po, lib=oracle_lib, prefix="oracle_") as o:
+        o.estimate_lambda()
+        ro = o.solve()
+    rel = abs(rg.primal_obj - ro.primal_obj) / (1 + abs(ro.primal_obj))
+    print(f"[result] {name}: gpu iter {rg.iter} obj {rg.primal_obj:.10e} oracle(1e-9) iter {ro.iter} "
+          f"obj {ro.primal_obj:.10e} rel {rel:.2e}", flush=True)
+    assert rel <= 1e-6
+
+
+# ------------------------------------------------------------ reference KATs on the GPU
+K = kats()
+
+
+def gsolver(spec, **kw):
+    return hprsocp.Solver(model_from(spec), params(**kw))
+
+
+def test_gpu_kat_soc_projections(gpu_lib):
+    s = K["soc_var_projection"]
+    with gsolver(s, sigma=1.0) as g:
+        g.set_scalars(sigma=1.0, lambda_A=0.0, lambda_Q=0.0)
+        g.set("x", s["x"])
+        g.update(0, True)
+        np.testing.assert_allclose(g.get("x_bar"), s["expect_x_bar"], atol=1e-14)
+        np.testing.assert_allclose(g.get("z_bar"), s["expect_z_bar"], atol=1e-14)
+    s = K["soc_con_projection"]
+    with gsolver(s, sigma=1.0) as g:
+        g.set_scalars(sigma=1.0, lambda_A=1.0, lambda_Q=0.0)
+        g.set("x", s["x_hat"])
+        g.update(0, True)
+        np.testing.assert_allclose(g.get("s"), s["expect_s"], atol=1e-14)
+        np.testing.assert_allclose(g.get("y_bar"), s["expect_y_bar"], atol=1e-14)
+
+
+def test_gpu_kat_residuals(gpu_lib):
+    s = K["bounds_violation"]
+    with gsolver(dict(n=3, soc_var_idx0=s["soc_var_idx0"]), sigma=1.0) as g:
+        g.set_scalars(sigma=1.0, lambda_A=0.0, lambda_Q=0.0)
+        g.set("x_bar", s["x_bar"])
+        assert g.residuals(0)["err_Rp"] == pytest.approx(0.5, abs=1e-14)
+    s = K["soc_rp"]
+    with gsolver(s, sigma=1.0) as g:
+        g.set_scalars(sigma=1.0, lambda_A=1.0, lambda_Q=0.0)
+        g.set("x_bar", s["x_bar"])
+        g.residuals(1)
+        np.testing.assert_allclose(g.get("Rp"), s["expect_Rp"], atol=1e-14)
+    s = K["dual_objective_rhs_shift"]
+    with gsolver(s) as g:
+        g.set_scalars(lambda_A=1.0, lambda_Q=0.0)
+        g.set("y_bar", s["y_bar"])
+        assert g.residuals(0)["dual_obj"] == pytest.approx(1.0, abs=1e-10)
+    s = K["mnorm_nonnegative"]
+    with gsolver(s, sigma=1.0) as g:
+        g.set_scalars(sigma=1.0, lambda_A=0.0, lambda_Q=1.0)
+        g.set("dw", s["dw"])
+        v = g.mnorm()
+        assert np.isfinite(v) and v >= 0.0
+
+
+@pytest.mark.parametrize("name", ["solve_soc_variables_only", "solve_pure_soc_constraints",
+                                  "solve_mixed_linear_soc", "clarabel_var_cone",
+                                  "clarabel_con_cone", "jump_soc", "trust_region", "model_cbf"])
+def test_gpu_kat_solves(name, gpu_lib):
+    s = K[name]
+    tol = s.get("stoptol", 1e-8)
+    r = hprsocp.optimize(model_from(s), params(stoptol=tol, max_iter=s.get("max_iter", 100000)))
+    assert r.status == "OPTIMAL"
+    if "expect_x" in s:
+        np.testing.assert_allclose(r.x, s["expect_x"], atol=s.get("atol", 1e-5))
+    if name == "trust_region":
+        np.testing.assert_allclose(r.x, trust_region_optimum(s["Q_diag"], s["c"]), atol=2e-3)
+        assert np.linalg.norm(np.array(s["A"]) @ r.x - np.array(s["rhs"])) <= 1e-5
+    if name.startswith("solve_"):
+        assert np.linalg.norm(r.x) <= 1e-3
+    if name == "model_cbf":
+        assert r.primal_obj == pytest.approx(-8.5, abs=1e-6)
+
+
+def test_gpu_rejects_bad_input(gpu_lib):
+    m = hprsocp.build_qp(sp.eye(2), np.zeros(2), [0, 0], [1, 1], [-1, -1], [1, 1], [], 2, 2, [1], [2])
+    with pytest.raises(hprsocp.HPRSOCPError):
+        hprsocp.Solver(m, params())
+    with pytest.raises(hprsocp.HPRSOCPError):
+        hprsocp.Solver(model("c0"), params(use_gpu=False))
