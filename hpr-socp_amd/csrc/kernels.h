@@ -8,6 +8,7 @@ namespace hpr {
 
 constexpr int kBlock = 256;       // threads per workgroup (4 wavefronts of 64)
 constexpr int kTileRows = 256;    // max rows of one tile (one row per thread in epilogues)
+constexpr int kStreamNnz = 2048;  // LDS product buffer of the CSR-stream SpMV path
 
 enum TileKind : int32_t { TILE_LINEAR = 0, TILE_CONES = 1, TILE_HUGE = 2 };
 
@@ -15,7 +16,9 @@ enum TileKind : int32_t { TILE_LINEAR = 0, TILE_CONES = 1, TILE_HUGE = 2 };
 //   LINEAR : independent rows (box coordinates / linear constraint rows)
 //   CONES  : whole cones c0..c1-1 (each with at most kTileRows rows)
 //   HUGE   : a row segment of a long cone; c0 = huge-cone id, c1 = partial slot
-// g  : lanes per row of the SpMV phase (1..64, or 256 = whole workgroup)
+// g  : SpMV phase: 0 = CSR-stream (the tile's <= kStreamNnz products are
+//      gathered by the whole workgroup into LDS, then summed per row in CSR
+//      order); otherwise lanes per row (1..64, or 256 = whole workgroup)
 // g2 : lanes per cone of the projection phase (CONES tiles)
 struct Tile {
     int32_t rb, re, kind, g, c0, c1, g2, pad;

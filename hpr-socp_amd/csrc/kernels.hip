@@ -51,6 +51,35 @@ __device__ double block_max(double v, double *sh) {
 }
 
 // ---------------------------------------------------------------- SpMV phase
+// CSR-stream: coalesced loads of the tile's nonzeros, independent gathers, one
+// product per LDS slot, then a sequential per-row sum (CSR order, products
+// rounded before summation).
+__device__ void tile_spmv_stream(const CsrView &M, const double *__restrict__ v, const Tile &t,
+                                 double *acc, double *prod, int32_t *rp) {
+    const int tid = threadIdx.x;
+    const int nrows = t.re - t.rb;
+    for (int r = tid; r <= nrows; r += kBlock) rp[r] = M.rowptr[t.rb + r];
+    __syncthreads();
+    const int k0 = rp[0];
+    const int nnz = rp[nrows] - k0;
+    const int32_t *__restrict__ col = M.col + k0;
+    const double *__restrict__ val = M.val + k0;
+#pragma unroll 4
+    for (int k = tid; k < nnz; k += kBlock) {
+        const double p = val[k] * v[col[k]];
+        prod[k] = p;
+    }
+    __syncthreads();
+    if (tid < nrows) {
+        const int a = rp[tid] - k0, b = rp[tid + 1] - k0;
+        double s = 0.0;
+        for (int k = a; k < b; ++k) s += prod[k];
+        acc[tid] = s;
+    }
+}
+
+// Vector path: g lanes per row (shuffle reduction), or the whole workgroup for
+// very long rows.
 __device__ void tile_spmv(const CsrView &M, const double *__restrict__ v, const Tile &t,
                           double *acc, double *sh) {
     const int tid = threadIdx.x;
@@ -205,11 +234,14 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
                                                  double *__restrict__ partials) {
     __shared__ double acc[kTileRows];
     __shared__ double sh[4];
+    __shared__ double prod[SPMV ? kStreamNnz : 1];
+    __shared__ int32_t rps[SPMV ? kTileRows + 1 : 1];
     const Tile t = tiles[blockIdx.x];
     const int tid = threadIdx.x;
     const int nrows = t.re - t.rb;
     if constexpr (SPMV) {
-        tile_spmv(M, vin, t, acc, sh);
+        if (t.g == 0) tile_spmv_stream(M, vin, t, acc, prod, rps);
+        else tile_spmv(M, vin, t, acc, sh);
     } else {
         if (tid < nrows) acc[tid] = 0.0;
     }

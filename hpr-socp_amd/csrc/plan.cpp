@@ -15,6 +15,7 @@ static int32_t pow2ceil(int64_t v) {
 
 static int32_t choose_g(int64_t nnz, int64_t rows, const PlanConfig &cfg) {
     if (rows <= 0) return 1;
+    if (cfg.stream && nnz <= kStreamNnz) return 0;  // CSR-stream through LDS
     const int64_t avg = (nnz + rows - 1) / rows;
     if (avg >= 1024) return kBlock;  // whole workgroup per row
     int32_t g = pow2ceil((avg + cfg.g_divisor - 1) / cfg.g_divisor);

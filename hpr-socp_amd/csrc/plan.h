@@ -27,6 +27,7 @@ struct PlanConfig {
     int32_t nnz_target = 2048;     // nonzeros per tile (SpMV work balance)
     int32_t huge_nnz_factor = 16;  // cones above factor*nnz_target nonzeros are split
     int32_t g_divisor = 2;         // g = pow2ceil(mean_row_nnz / g_divisor), clamped to 1..64
+    bool stream = true;            // CSR-stream tiles when the tile fits kStreamNnz
 };
 
 // rowptr may be null (no SpMV: rows weigh one unit each).

@@ -22,6 +22,7 @@
 #include "../../include/hprsocp.h"
 #include "kernels.h"
 #include "plan.h"
+#include "golden.h"
 #include "scale_rules.h"
 
 using namespace hpr;
@@ -1123,90 +1124,20 @@ double golden_scalar(double ap, double bp, double cp, double dp, double lo, doub
     return 0.5 * (a + b);
 }
 
-// unified_golden_Q_diag (unified_operations.jl:411-449) replayed exactly with
-// speculative batches: from a known state, the next L steps can only visit the
-// 2^L - 1 points of a binary decision tree; all of them are evaluated in one
-// device pass (g(x) = sum_i ATdy_i * QATdy_i / (1 + x d_i)), then the host walks
-// the tree with the reference's comparisons.
-struct GState {
-    double lo, hi, x1, x2, f1, f2;
-};
-
+// unified_golden_Q_diag (unified_operations.jl:411-449) with every evaluation of
+// g(x) = sum_i ATdy_i * QATdy_i / (1 + x d_i) batched on the device.
 double golden_q_diag(hprsocp_handle *h, double a, double b, double lo, double hi, double tol) {
-    const double phi = (sqrt(5.0) - 1.0) / 2.0;
-    const int L = 5, NP = (1 << L) - 1;
     const int nb = std::min(reduce_blocks(h->n), 1024);
     auto eval = [&](const std::vector<double> &xs, std::vector<double> &gs) {
         memcpy(h->hscal + SL_GOLD, xs.data(), sizeof(double) * xs.size());
-        HIPCHK(hipMemcpyAsync(h->dscal + SL_GOLD, h->hscal + SL_GOLD, sizeof(double) * xs.size(), hipMemcpyHostToDevice, h->st));
-        launch_golden_eval(h->st, h->n, h->atdy, h->qatdy, h->diag_Q, h->dscal + SL_GOLD, (int)xs.size(), h->red_part,
-                           nb, h->dscal + SL_GOLD);
+        HIPCHK(hipMemcpyAsync(h->dscal + SL_GOLD, h->hscal + SL_GOLD, sizeof(double) * xs.size(),
+                              hipMemcpyHostToDevice, h->st));
+        launch_golden_eval(h->st, h->n, h->atdy, h->qatdy, h->diag_Q, h->dscal + SL_GOLD, (int)xs.size(),
+                           h->red_part, nb, h->dscal + SL_GOLD);
         const double *r = read_slots(h, SL_GOLD, (int)xs.size());
         gs.assign(r, r + xs.size());
     };
-    auto f_of = [&](double x, double g) { return a * x + b / x + x * x * g; };
-    GState S;
-    S.lo = lo;
-    S.hi = hi;
-    S.x1 = hi - phi * (hi - lo);
-    S.x2 = lo + phi * (hi - lo);
-    {
-        std::vector<double> xs{S.x1, S.x2}, gs;
-        eval(xs, gs);
-        S.f1 = f_of(S.x1, gs[0]);
-        S.f2 = f_of(S.x2, gs[1]);
-    }
-    int iter = 0;
-    auto cont = [&](const GState &g, int it) { return fabs(g.hi - g.lo) > tol * jmax(1.0, fabs(g.lo)) && it < 200; };
-    // step of the reference loop with the new point's f unknown (returned via nx)
-    auto step = [&](const GState &g, bool branchA, GState &o, double &nx) {
-        o = g;
-        if (branchA) {  // f1 > f2
-            o.lo = g.x1;
-            o.x1 = g.x2;
-            o.f1 = g.f2;
-            o.x2 = o.lo + phi * (o.hi - o.lo);
-            nx = o.x2;
-        } else {
-            o.hi = g.x2;
-            o.x2 = g.x1;
-            o.f2 = g.f1;
-            o.x1 = o.hi - phi * (o.hi - o.lo);
-            nx = o.x1;
-        }
-    };
-    while (cont(S, iter)) {
-        // build the decision tree: node k's state has one unknown f at point pts[k]
-        std::vector<GState> node(NP);
-        std::vector<double> pts(NP);
-        std::vector<char> isA(NP);
-        step(S, S.f1 > S.f2, node[0], pts[0]);
-        isA[0] = S.f1 > S.f2;
-        for (int k = 0; 2 * k + 2 < NP; ++k) {
-            // child 2k+1: decision "f1 > f2" true at node k; child 2k+2: false
-            step(node[k], true, node[2 * k + 1], pts[2 * k + 1]);
-            isA[2 * k + 1] = 1;
-            step(node[k], false, node[2 * k + 2], pts[2 * k + 2]);
-            isA[2 * k + 2] = 0;
-        }
-        std::vector<double> gs;
-        eval(pts, gs);
-        int k = 0;
-        iter++;
-        while (true) {
-            GState &g = node[k];
-            const double fv = f_of(pts[k], gs[k]);
-            if (isA[k]) g.f2 = fv;
-            else g.f1 = fv;
-            S = g;
-            if (!cont(S, iter)) break;
-            const int child = (S.f1 > S.f2) ? 2 * k + 1 : 2 * k + 2;
-            if (child >= NP) break;
-            k = child;
-            iter++;
-        }
-    }
-    return (S.f1 < S.f2) ? S.x1 : S.x2;
+    return golden_speculative(a, b, lo, hi, tol, 5, eval);
 }
 
 // update_sigma!, algorithm.jl:729-990
