@@ -29,11 +29,29 @@ struct HugeInfo {
     const int32_t *start, *end, *slot0, *nslot;  // per huge cone
 };
 
+// Rows handled by one workgroup of the XCD-sliced SpMV (all slices share them).
+struct SliceTile {
+    int32_t rb, re;
+};
+
 struct CsrView {
     const int32_t *rowptr, *col;
     const double *val;
     int32_t nrows;
+    // XCD column slicing (nslices > 1): the columns are cut into nslices equal
+    // ranges and the entries regrouped slice-major (per-slice CSR, concatenated).
+    // Workgroup b of the slice kernel works on slice b % nslices — the XCD it is
+    // dispatched to — so each XCD gathers from one L2-resident slice of the
+    // input vector; per-row partial sums land in s_part[slice * nrows + row].
+    int32_t nslices = 1;
+    const int32_t *s_rowptr = nullptr;  // nslices * (nrows + 1)
+    const int32_t *s_col = nullptr;
+    const double *s_val = nullptr;
+    const SliceTile *s_tiles = nullptr;
+    int32_t s_ntiles = 0;
+    double *s_part = nullptr;
 };
+constexpr int kSliceRows = 1024;        // rows of one slice tile
 
 struct CsrViewF {
     const int32_t *rowptr, *col;
@@ -221,6 +239,8 @@ namespace hpr {
 void launch_scale_csr_t(hipStream_t st, int32_t nrows, const int32_t *rowptr, const int32_t *col,
                         double *val, const double *r, const double *c);
 void launch_sym_q(hipStream_t st, int64_t nnz, const double *src, const int32_t *tpos, double *dst);
+// dst[k] = src[perm[k]] (fills the slice-major copy of a matrix' values)
+void launch_gather_perm(hipStream_t st, int64_t nnz, const double *src, const int32_t *perm, double *dst);
 constexpr int kMaxGolden = 64;
 // out = (v .* d) ./ s   (warm-start scaling, algorithm.jl:2628, 2646)
 void launch_scale_vd(hipStream_t st, int64_t n, const double *v, const double *d, double s, double *out);

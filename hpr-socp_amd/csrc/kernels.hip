@@ -225,8 +225,67 @@ struct OpTempv {
     }
 };
 
+// ------------------------------------------------------------ XCD-sliced SpMV
+// Block b: slice p = b % P (the XCD it is dispatched to), row tile b / P.  The
+// slice's nonzeros of the tile are streamed (non-temporal: read once) in chunks
+// of kStreamNnz products through LDS; every row sums its products in CSR order.
+__global__ __launch_bounds__(kBlock) void k_spmv_slices(CsrView M, const double *__restrict__ x) {
+    __shared__ double prod[kStreamNnz];
+    __shared__ int32_t rps[kSliceRows + 1];
+    const int P = M.nslices;
+    const int p = blockIdx.x % P;
+    const SliceTile T = M.s_tiles[blockIdx.x / P];
+    const int tid = threadIdx.x;
+    const int nr = T.re - T.rb;
+    const int32_t *rp = M.s_rowptr + (int64_t)p * (M.nrows + 1);
+    for (int r = tid; r <= nr; r += kBlock) rps[r] = rp[T.rb + r];
+    __syncthreads();
+    const int k0 = rps[0], k1 = rps[nr];
+    constexpr int RPT = kSliceRows / kBlock;  // rows per thread
+    double acc[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) acc[j] = 0.0;
+    for (int c0 = k0; c0 < k1; c0 += kStreamNnz) {
+        const int c1 = min(c0 + kStreamNnz, k1);
+#pragma unroll 4
+        for (int k = c0 + tid; k < c1; k += kBlock) {
+            const double v = __builtin_nontemporal_load(M.s_val + k);
+            const int c = __builtin_nontemporal_load(M.s_col + k);
+            prod[k - c0] = v * x[c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int r = tid + j * kBlock;
+            if (r < nr) {
+                const int a = max(rps[r], c0), e = min(rps[r + 1], c1);
+                double s = acc[j];
+                for (int k = a; k < e; ++k) s += prod[k - c0];
+                acc[j] = s;
+            }
+        }
+        __syncthreads();
+    }
+    double *out = M.s_part + (int64_t)p * M.nrows + T.rb;
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int r = tid + j * kBlock;
+        if (r < nr) out[r] = acc[j];
+    }
+}
+
+static void launch_slices(hipStream_t st, const CsrView &M, const double *x) {
+    if (M.s_ntiles > 0)
+        hipLaunchKernelGGL(k_spmv_slices, dim3(M.s_ntiles * M.nslices), dim3(kBlock), 0, st, M, x);
+}
+
+__global__ void k_gather_perm(int64_t nnz, const double *src, const int32_t *perm, double *dst);
+
 // ------------------------------------------------------------ tile kernel
-template <class Op, bool SPMV, bool CONES>
+// SRC: 0 = no SpMV, 1 = CSR SpMV inside the tile, 2 = sum of the XCD-slice partials
+enum : int { SRC_NONE = 0, SRC_CSR = 1, SRC_SLICES = 2 };
+
+template <class Op, int SRC, bool CONES>
 __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles, CsrView M,
                                                  const double *__restrict__ vin,
                                                  const int32_t *__restrict__ cone_idx, HugeInfo huge,
@@ -234,14 +293,21 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
                                                  double *__restrict__ partials) {
     __shared__ double acc[kTileRows];
     __shared__ double sh[4];
-    __shared__ double prod[SPMV ? kStreamNnz : 1];
-    __shared__ int32_t rps[SPMV ? kTileRows + 1 : 1];
+    __shared__ double prod[SRC == SRC_CSR ? kStreamNnz : 1];
+    __shared__ int32_t rps[SRC == SRC_CSR ? kTileRows + 1 : 1];
     const Tile t = tiles[blockIdx.x];
     const int tid = threadIdx.x;
     const int nrows = t.re - t.rb;
-    if constexpr (SPMV) {
+    if constexpr (SRC == SRC_CSR) {
         if (t.g == 0) tile_spmv_stream(M, vin, t, acc, prod, rps);
         else tile_spmv(M, vin, t, acc, sh);
+    } else if constexpr (SRC == SRC_SLICES) {
+        if (tid < nrows) {
+            const double *pp = M.s_part + t.rb + tid;
+            double a = 0.0;
+            for (int p = 0; p < M.nslices; ++p) a += pp[(int64_t)p * M.nrows];
+            acc[tid] = a;
+        }
     } else {
         if (tid < nrows) acc[tid] = 0.0;
     }
@@ -310,47 +376,51 @@ __global__ __launch_bounds__(kBlock) void k_huge_finish(const Tile *__restrict__
     }
 }
 
-template <class Op, bool SPMV>
+// Fused SpMV + epilogue over a plan.  With XCD slicing the SpMV runs first as
+// k_spmv_slices and the tile kernel sums the slice partials.
+template <class Op, int SRC, bool CONES>
 static void run_tiles(hipStream_t st, const DevPlan &plan, const CsrView &M, const double *v,
                       const int32_t *cone_idx, const Op &op, double *raw_buf, double *partials) {
     if (plan.ntiles == 0) return;
-    hipLaunchKernelGGL((k_tile<Op, SPMV, true>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles, M, v,
-                       cone_idx, plan.huge, op, raw_buf, partials);
-    const int nh = plan.ntiles - plan.first_huge;
-    if (nh > 0)
-        hipLaunchKernelGGL((k_huge_finish<Op>), dim3(nh), dim3(kBlock), 0, st, plan.tiles + plan.first_huge,
-                           plan.huge, op, raw_buf, partials);
+    if (SRC == SRC_CSR && M.nslices > 1) {
+        launch_slices(st, M, v);
+        hipLaunchKernelGGL((k_tile<Op, SRC_SLICES, CONES>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles,
+                           M, v, cone_idx, plan.huge, op, raw_buf, partials);
+    } else {
+        hipLaunchKernelGGL((k_tile<Op, SRC, CONES>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles, M, v,
+                           cone_idx, plan.huge, op, raw_buf, partials);
+    }
+    if (CONES) {
+        const int nh = plan.ntiles - plan.first_huge;
+        if (nh > 0)
+            hipLaunchKernelGGL((k_huge_finish<Op>), dim3(nh), dim3(kBlock), 0, st, plan.tiles + plan.first_huge,
+                               plan.huge, op, raw_buf, partials);
+    }
 }
 
 void launch_noq_primal(hipStream_t st, const DevPlan &plan, const CsrView &AT, const int32_t *var_idx,
                        const NoQPrimalArgs &a, double *raw_buf, double *partials) {
-    run_tiles<OpNoQPrimal, true>(st, plan, AT, a.y, var_idx, OpNoQPrimal{a}, raw_buf, partials);
+    run_tiles<OpNoQPrimal, SRC_CSR, true>(st, plan, AT, a.y, var_idx, OpNoQPrimal{a}, raw_buf, partials);
 }
 void launch_dual(hipStream_t st, const DevPlan &plan, const CsrView &A, const int32_t *con_idx,
                  const DualArgs &a, double *raw_buf, double *partials) {
-    run_tiles<OpDual, true>(st, plan, A, a.v, con_idx, OpDual{a}, raw_buf, partials);
+    run_tiles<OpDual, SRC_CSR, true>(st, plan, A, a.v, con_idx, OpDual{a}, raw_buf, partials);
 }
 void launch_qp_primal(hipStream_t st, const DevPlan &plan, const int32_t *var_idx,
                       const QPPrimalArgs &a, double *raw_buf, double *partials) {
     CsrView none{nullptr, nullptr, nullptr, 0};
-    run_tiles<OpQPPrimal, false>(st, plan, none, nullptr, var_idx, OpQPPrimal{a}, raw_buf, partials);
+    run_tiles<OpQPPrimal, SRC_NONE, true>(st, plan, none, nullptr, var_idx, OpQPPrimal{a}, raw_buf, partials);
 }
 void launch_w2(hipStream_t st, const DevPlan &plan, const CsrView &AT, const W2Args &a) {
-    if (plan.ntiles == 0) return;
-    hipLaunchKernelGGL((k_tile<OpW2, true, false>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles, AT,
-                       a.ybar, nullptr, plan.huge, OpW2{a}, nullptr, nullptr);
+    run_tiles<OpW2, SRC_CSR, false>(st, plan, AT, a.ybar, nullptr, OpW2{a}, nullptr, nullptr);
 }
 void launch_spmv(hipStream_t st, const DevPlan &plan, const CsrView &M, const double *v, double *out,
                  const double *diag, const double *v2) {
-    if (plan.ntiles == 0) return;
-    hipLaunchKernelGGL((k_tile<OpStore, true, false>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles, M,
-                       v, nullptr, plan.huge, OpStore{out, diag, v2}, nullptr, nullptr);
+    run_tiles<OpStore, SRC_CSR, false>(st, plan, M, v, nullptr, OpStore{out, diag, v2}, nullptr, nullptr);
 }
 void launch_spmv_tempv(hipStream_t st, const DevPlan &plan, const CsrView &M, const double *wbar,
                        const TempvArgs &a) {
-    if (plan.ntiles == 0) return;
-    hipLaunchKernelGGL((k_tile<OpTempv, true, false>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles, M,
-                       wbar, nullptr, plan.huge, OpTempv{a}, nullptr, nullptr);
+    run_tiles<OpTempv, SRC_CSR, false>(st, plan, M, wbar, nullptr, OpTempv{a}, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------- reductions
@@ -915,6 +985,13 @@ __global__ void k_unscale(int64_t n, const double *v, const double *d, double a,
 }
 void launch_unscale(hipStream_t st, int64_t n, const double *v, const double *d, double a, int mul, double *out) {
     if (n > 0) hipLaunchKernelGGL(k_unscale, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, v, d, a, mul, out);
+}
+
+__global__ void k_gather_perm(int64_t nnz, const double *src, const int32_t *perm, double *dst) {
+    GRID_STRIDE(k, nnz) dst[k] = src[perm[k]];
+}
+void launch_gather_perm(hipStream_t st, int64_t nnz, const double *src, const int32_t *perm, double *dst) {
+    if (nnz > 0) hipLaunchKernelGGL(k_gather_perm, dim3(ew_blocks(nnz)), dim3(kBlock), 0, st, nnz, src, perm, dst);
 }
 
 // symmetrize Q values in place through the transpose-partner map: v = (v + v[tpos]) / 2

@@ -11,6 +11,7 @@
 
 #include <math.h>
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -452,6 +453,82 @@ double power_iteration(hprsocp_handle *h, bool onA) {
     return ps.lambda;
 }
 
+// ------------------------------------------------------------- XCD slicing
+// Number of column slices for an operator whose gathered input has `ncols`
+// FP64 entries: one per XCD (8 on MI355X) once the vector outgrows ~2 MB,
+// so that each XCD's 4 MB L2 keeps its slice resident.
+int32_t choose_slices(int64_t ncols) {
+    const char *env = getenv("HPRSOCP_XCD_SLICES");
+    if (env) return std::max(1, atoi(env));
+    return ncols * (int64_t)sizeof(double) > (2ll << 20) ? 8 : 1;
+}
+
+// Regroups the entries of M slice-major (slice p = columns [p*C/P, (p+1)*C/P)),
+// fills the device copy of the (already scaled) values through a permutation,
+// and builds the slice row tiles.
+void attach_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double *dval) {
+    const int32_t P = choose_slices(hm.ncols);
+    if (P <= 1 || hm.ci.empty()) return;
+    const int32_t R = hm.nrows;
+    const int64_t nnz = (int64_t)hm.ci.size();
+    std::vector<int64_t> bound(P + 1);
+    for (int p = 0; p <= P; ++p) bound[p] = (int64_t)hm.ncols * p / P;
+    auto slice_of = [&](int32_t c) {
+        int p = (int)((int64_t)c * P / std::max<int64_t>(hm.ncols, 1));
+        while (p + 1 < P && c >= bound[p + 1]) ++p;
+        while (p > 0 && c < bound[p]) --p;
+        return p;
+    };
+    std::vector<int32_t> cnt((size_t)P * R, 0);
+    for (int32_t i = 0; i < R; ++i)
+        for (int32_t k = hm.rp[i]; k < hm.rp[i + 1]; ++k) cnt[(size_t)slice_of(hm.ci[k]) * R + i]++;
+    std::vector<int32_t> srp((size_t)P * (R + 1));
+    int64_t pos = 0;
+    for (int p = 0; p < P; ++p) {
+        int32_t *r = srp.data() + (size_t)p * (R + 1);
+        for (int32_t i = 0; i < R; ++i) {
+            r[i] = (int32_t)pos;
+            pos += cnt[(size_t)p * R + i];
+        }
+        r[R] = (int32_t)pos;
+    }
+    std::vector<int32_t> scol((size_t)nnz), perm((size_t)nnz);
+    std::vector<int32_t> next((size_t)P * R);
+    for (int p = 0; p < P; ++p)
+        for (int32_t i = 0; i < R; ++i) next[(size_t)p * R + i] = srp[(size_t)p * (R + 1) + i];
+    for (int32_t i = 0; i < R; ++i)
+        for (int32_t k = hm.rp[i]; k < hm.rp[i + 1]; ++k) {
+            const int p = slice_of(hm.ci[k]);
+            const int32_t d = next[(size_t)p * R + i]++;
+            scol[d] = hm.ci[k];
+            perm[d] = k;
+        }
+    // slice tiles: <= kSliceRows rows and about P * kStreamNnz nonzeros
+    std::vector<SliceTile> tiles;
+    int32_t r = 0;
+    while (r < R) {
+        int32_t e = r + 1;
+        int64_t nz = hm.rp[e] - hm.rp[r];
+        while (e < R && e - r < kSliceRows && nz + (hm.rp[e + 1] - hm.rp[e]) <= (int64_t)P * kStreamNnz) {
+            nz += hm.rp[e + 1] - hm.rp[e];
+            ++e;
+        }
+        tiles.push_back(SliceTile{r, e});
+        r = e;
+    }
+    M.nslices = P;
+    M.s_rowptr = h->upload<int32_t>(srp.data(), (int64_t)srp.size());
+    M.s_col = h->upload<int32_t>(scol.data(), nnz);
+    int32_t *dperm = h->upload<int32_t>(perm.data(), nnz);
+    double *sval = h->dalloc<double>(nnz);
+    launch_gather_perm(h->st, nnz, dval, dperm, sval);
+    M.s_val = sval;
+    M.s_tiles = h->upload<SliceTile>(tiles.data(), (int64_t)tiles.size());
+    M.s_ntiles = (int32_t)tiles.size();
+    M.s_part = h->dalloc<double>((int64_t)P * R);
+    h->sync();
+}
+
 // ------------------------------------------------------------- validation
 int validate_csr(std::string &err, const char *name, int32_t nr, int32_t nc, int64_t nnz, const int32_t *rp,
                  const int32_t *ci, const double *v) {
@@ -669,14 +746,14 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
         h->q_struct_diag = sd;
         h->plan_Q = make_plan(h, Qh.rp, n, n, {}, 0);
     }
+    HostCsr F, FT;
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
-        HostCsr F;
         F.nrows = p->F_rows;
         F.ncols = n;
         F.rp.assign(p->F_rowptr, p->F_rowptr + p->F_rows + 1);
         F.ci.assign(p->F_colidx, p->F_colidx + p->F_nnz);
         F.v.assign(p->F_val, p->F_val + p->F_nnz);
-        HostCsr FT = transpose(F);
+        FT = transpose(F);
         h->Frows = p->F_rows;
         int32_t *F_rp = h->upload<int32_t>(F.rp.data(), F.nrows + 1);
         int32_t *F_ci = h->upload<int32_t>(F.ci.data(), (int64_t)F.ci.size());
@@ -731,6 +808,22 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     h->pstate = (PowerState *)h->dalloc<double>(4);
 
     do_scaling(h);
+    // XCD column slices of every SpMV operator (values are final after scaling);
+    // the epilogue plans of sliced operators no longer carry SpMV work, so they
+    // are re-cut by rows only
+    attach_slices(h, h->A, A, h->A_v);
+    attach_slices(h, h->AT, AT, h->AT_v);
+    if (h->A.nslices > 1) h->plan_A = make_plan(h, {}, m, h->m1, h->h_con_idx, h->ncon);
+    if (h->AT.nslices > 1) h->plan_AT = make_plan(h, {}, n, h->n1, h->h_var_idx, h->nvar);
+    if (h->q_kind == HPRSOCP_Q_FACTORED) {
+        attach_slices(h, h->F, F, h->F.val);
+        attach_slices(h, h->FT, FT, h->FT.val);
+        if (h->F.nslices > 1) h->plan_F = make_plan(h, {}, F.nrows, F.nrows, {}, 0);
+        if (h->FT.nslices > 1) h->plan_FT = make_plan(h, {}, n, n, {}, 0);
+    } else if (h->nnzQ > 0 && !h->q_struct_diag) {
+        attach_slices(h, h->Q, Qh, h->Q_v);
+        if (h->Q.nslices > 1) h->plan_Q = make_plan(h, {}, n, n, {}, 0);
+    }
     // check_Q_diagonal (scaling.jl:1312-1353), after scaling as in solve()
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
         h->Q_is_diag = 0;

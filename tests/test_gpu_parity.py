@@ -234,3 +234,44 @@ def test_gpu_rejects_bad_input(gpu_lib):
         hprsocp.Solver(m, params())
     with pytest.raises(hprsocp.HPRSOCPError):
         hprsocp.Solver(model("c0"), params(use_gpu=False))
+
+
+# ------------------------------------------------------------ XCD-sliced SpMV path
+@pytest.fixture
+def forced_slices(request, monkeypatch):
+    monkeypatch.setenv("HPRSOCP_XCD_SLICES", str(request.param))
+    return request.param
+
+
+SLICED = ["c0", "qp_csr", "qp_factored", "huge_cones", "c1_tiny", "c2_tiny"]
+
+
+@pytest.mark.parametrize("forced_slices", [8, 3], indirect=True)
+@pytest.mark.parametrize("name", SLICED)
+def test_sliced_spmv_and_iterations(name, forced_slices, oracle_lib, gpu_lib):
+    """Column-sliced SpMV (forced on small models): SpMV vs scipy, then 30 HPR
+    steps, residuals and M-norm against the oracle."""
+    m, g, o = pair(name, oracle_lib, gpu_lib)
+    with g, o:
+        A = m.A.tocsr()
+        As = sp.csr_matrix((g.scaled_A(), A.indices, A.indptr), shape=A.shape)
+        rng = np.random.default_rng(1)
+        v, w = rng.standard_normal(A.shape[1]), rng.standard_normal(A.shape[0])
+        close(g.spmv(v), As @ v, 1e-13)
+        close(g.spmv(w, trans=True), As.T @ w, 1e-13)
+        so = o.estimate_lambda()
+        sg = g.estimate_lambda()
+        assert sg["lambda_max_A"] == pytest.approx(so["lambda_max_A"], rel=1e-8)
+        for s in (g, o):
+            s.set_scalars(sigma=so["sigma"], lambda_A=so["lambda_max_A"], lambda_Q=so["lambda_max_Q"])
+        kind = "qp" if so["has_quadratic"] else "noq"
+        for k in range(30):
+            full = (k % 7 == 6) or k == 29
+            g.update(k, full)
+            o.update(k, full)
+        for vname in STATE[kind] + FULL:
+            close(g.get(vname), o.get(vname), 1e-9)
+        rg, ro = g.residuals(1), o.residuals(1)
+        for k in ro:
+            assert rg[k] == pytest.approx(ro[k], rel=1e-9, abs=1e-13), k
+        assert g.mnorm() == pytest.approx(o.mnorm(), rel=1e-9, abs=1e-14)
