@@ -454,21 +454,14 @@ double power_iteration(hprsocp_handle *h, bool onA) {
 }
 
 // ------------------------------------------------------------- XCD slicing
-// Number of column slices for an operator whose gathered input has `ncols`
-// FP64 entries: one per XCD (8 on MI355X) once the vector outgrows ~2 MB,
-// so that each XCD's 4 MB L2 keeps its slice resident.
-int32_t choose_slices(int64_t ncols) {
-    const char *env = getenv("HPRSOCP_XCD_SLICES");
-    if (env) return std::max(1, atoi(env));
-    return ncols * (int64_t)sizeof(double) > (2ll << 20) ? 8 : 1;
-}
+// One column slice per XCD (8 on MI355X) lets each XCD's 4 MB L2 keep its
+// slice of the gathered vector resident.
 
 // Regroups the entries of M slice-major (slice p = columns [p*C/P, (p+1)*C/P)),
 // fills the device copy of the (already scaled) values through a permutation,
 // and builds the slice row tiles.
-void attach_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double *dval) {
-    const int32_t P = choose_slices(hm.ncols);
-    if (P <= 1 || hm.ci.empty()) return;
+bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double *dval, int32_t P) {
+    if (P <= 1 || hm.ci.empty()) return false;
     const int32_t R = hm.nrows;
     const int64_t nnz = (int64_t)hm.ci.size();
     std::vector<int64_t> bound(P + 1);
@@ -527,6 +520,53 @@ void attach_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const doubl
     M.s_ntiles = (int32_t)tiles.size();
     M.s_part = h->dalloc<double>((int64_t)P * R);
     h->sync();
+    return true;
+}
+
+// Autotuner of the XCD slicing: builds the sliced form (8 slices) of an
+// operator whose input vector outgrows ~2 MB, times one SpMV of each form with
+// HIP events (median of 5 after a warm-up) and keeps the faster one together
+// with its epilogue plan (nnz-balanced tiles for the plain form, row tiles for
+// the sliced form).  HPRSOCP_XCD_SLICES=<P> forces P slices (tests).
+void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm, const double *dval,
+                 int32_t lin_end, const std::vector<int32_t> &cidx, int32_t ncones, const char *name) {
+    const char *env = getenv("HPRSOCP_XCD_SLICES");
+    int32_t P = env ? std::max(1, atoi(env)) : (hm.ncols * (int64_t)sizeof(double) > (2ll << 20) ? 8 : 1);
+    if (P <= 1 || hm.ci.empty()) return;
+    CsrView S = M;
+    if (!build_slices(h, S, hm, dval, P)) return;
+    DevPlan splan = make_plan(h, {}, hm.nrows, lin_end, cidx, ncones);
+    if (!env) {
+        double *xin = h->dalloc<double>(hm.ncols);
+        double *yout = h->dalloc<double>(hm.nrows);
+        launch_fill(h->st, hm.ncols, 1.0, xin);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        auto time_one = [&](const CsrView &V, const DevPlan &pl) {
+            std::vector<float> t;
+            for (int rep = 0; rep < 6; ++rep) {
+                HIPCHK(hipEventRecord(e0, h->st));
+                launch_spmv(h->st, pl, V, xin, yout);
+                HIPCHK(hipEventRecord(e1, h->st));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (rep) t.push_back(ms);
+            }
+            std::sort(t.begin(), t.end());
+            return t[t.size() / 2];
+        };
+        const float t_plain = time_one(M, plan), t_sliced = time_one(S, splan);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (h->params.verbose)
+            printf("XCD slicing autotune %s: plain %.4f ms, %d slices %.4f ms -> %s\n", name, t_plain, P, t_sliced,
+                   t_sliced < t_plain ? "sliced" : "plain");
+        if (!(t_sliced < t_plain)) return;
+    }
+    M = S;
+    plan = splan;
 }
 
 // ------------------------------------------------------------- validation
@@ -808,21 +848,15 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     h->pstate = (PowerState *)h->dalloc<double>(4);
 
     do_scaling(h);
-    // XCD column slices of every SpMV operator (values are final after scaling);
-    // the epilogue plans of sliced operators no longer carry SpMV work, so they
-    // are re-cut by rows only
-    attach_slices(h, h->A, A, h->A_v);
-    attach_slices(h, h->AT, AT, h->AT_v);
-    if (h->A.nslices > 1) h->plan_A = make_plan(h, {}, m, h->m1, h->h_con_idx, h->ncon);
-    if (h->AT.nslices > 1) h->plan_AT = make_plan(h, {}, n, h->n1, h->h_var_idx, h->nvar);
+    // XCD column slices of every SpMV operator, kept where they measure faster
+    // (values are final after scaling)
+    tune_slices(h, h->A, h->plan_A, A, h->A_v, h->m1, h->h_con_idx, h->ncon, "A");
+    tune_slices(h, h->AT, h->plan_AT, AT, h->AT_v, h->n1, h->h_var_idx, h->nvar, "A^T");
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
-        attach_slices(h, h->F, F, h->F.val);
-        attach_slices(h, h->FT, FT, h->FT.val);
-        if (h->F.nslices > 1) h->plan_F = make_plan(h, {}, F.nrows, F.nrows, {}, 0);
-        if (h->FT.nslices > 1) h->plan_FT = make_plan(h, {}, n, n, {}, 0);
+        tune_slices(h, h->F, h->plan_F, F, h->F.val, F.nrows, {}, 0, "F");
+        tune_slices(h, h->FT, h->plan_FT, FT, h->FT.val, n, {}, 0, "F^T");
     } else if (h->nnzQ > 0 && !h->q_struct_diag) {
-        attach_slices(h, h->Q, Qh, h->Q_v);
-        if (h->Q.nslices > 1) h->plan_Q = make_plan(h, {}, n, n, {}, 0);
+        tune_slices(h, h->Q, h->plan_Q, Qh, h->Q_v, n, {}, 0, "Q");
     }
     // check_Q_diagonal (scaling.jl:1312-1353), after scaling as in solve()
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
