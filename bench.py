@@ -28,6 +28,15 @@ sys.path.insert(0, PKG)
 METRIC = "time-to-1e-6 KKT & HPR iters/s (FP64 vs FP32+refine); batched solves/s"
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the dominant launch group, from two rocprofv3 --pmc
+# passes (FETCH_SIZE, WRITE_SIZE) over this script, each scaled by what the same
+# counter reads for the 1 GiB plain copy of hprsocp_probe_copy (read factor 2.0,
+# write factor 1.0).  Source: profiles/c1_fp64_iteration_hbm_traffic.txt.
+# Keyed by (config, precision, group); only full-size runs use it.
+MEASURED_TRAFFIC = {
+    ("c1", "fp64", "dual"): 549.1e6 + 172.3e6,   # k_spmv_slices(A) + k_tile<OpDual, slices>
+}
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -119,6 +128,8 @@ def main():
     ap.add_argument("--solve-time", type=float, default=90.0, help="time cap of the time-to-1e-6 solve")
     ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--probe-copy-mb", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the HPR path here is "
@@ -136,6 +147,10 @@ def main():
     log(f"setup (transfer + scaling + plans) {time.time() - t0:.2f}s")
     sc = g.estimate_lambda()
     log(f"lambda_A={sc['lambda_max_A']:.6e} lambda_Q={sc['lambda_max_Q']:.6e} sigma={sc['sigma']:.4e}")
+    if args.probe_copy_mb > 0:
+        pms = g.probe_copy(args.probe_copy_mb << 20)
+        log(f"copy probe: {args.probe_copy_mb} MiB in {pms:.4f} ms "
+            f"({2 * (args.probe_copy_mb << 20) / (pms * 1e-3) / 1e9:.0f} GB/s read+write)")
     g.time_iterations(args.warmup)
     ms, _ = g.time_iterations(args.steps)
     _, kern = g.time_iterations(max(args.steps // 4, 10), per_kernel=True)
@@ -146,6 +161,7 @@ def main():
     dom_bytes = bytes_by[dom]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     it_bytes = sum(bytes_by.values())
+    traffic = MEASURED_TRAFFIC.get((args.config, "fp64", dom)) if args.scale == 1.0 else None
     g.close()
 
     solve = None
@@ -186,7 +202,7 @@ def main():
                    "iteration_GBps": round(it_bytes / (ms * 1e-3) / 1e9, 1)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": None, "algorithmic_bytes": int(dom_bytes), "ms": round(dom_ms, 5)},
+                     "traffic": traffic, "algorithmic_bytes": int(dom_bytes), "ms": round(dom_ms, 5)},
         "kernel_ms": {name: round(t, 5) for name, t in groups},
         "solve_to_1e-6": solve,
         "cpu_baseline": {"value": cpu_rate and round(cpu_rate, 4), "unit": "HPR iters/s", "cores": 1,

@@ -1,0 +1,144 @@
+// hotpath.h — data structures and launch wrappers of the fused HPR kernels,
+// templated on the value type T of matrices and iterates (double for the FP64
+// method, float for the FP32 bulk of the mixed-precision form).  Cone norms
+// (the t^2 - |xi|^2 test) are always accumulated in FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hpr {
+
+constexpr int kBlock = 256;       // threads per workgroup (4 wavefronts of 64)
+constexpr int kTileRows = 256;    // max rows of one tile (one row per thread in epilogues)
+constexpr int kStreamNnz = 2048;  // LDS product buffer of the CSR-stream SpMV paths
+constexpr int kSliceRows = 1024;  // rows of one slice tile
+constexpr int kSliceTileNnz = 60000;  // cap of one slice tile (per-slice offsets fit uint16)
+
+enum TileKind : int32_t { TILE_LINEAR = 0, TILE_CONES = 1, TILE_HUGE = 2 };
+
+// One workgroup's unit of work: rows [rb, re) of an operator.
+//   LINEAR : independent rows (box coordinates / linear constraint rows)
+//   CONES  : whole cones c0..c1-1 (each with at most kTileRows rows)
+//   HUGE   : a row segment of a long cone; c0 = huge-cone id, c1 = partial slot
+// g  : SpMV phase: 0 = CSR-stream (the tile's <= kStreamNnz products are
+//      gathered by the whole workgroup into LDS, then summed per row in CSR
+//      order); otherwise lanes per row (1..64, or 256 = whole workgroup)
+// g2 : lanes per cone of the projection phase (CONES tiles)
+struct Tile {
+    int32_t rb, re, kind, g, c0, c1, g2, pad;
+};
+
+// Long cones split over several tiles (two-phase projection).
+struct HugeInfo {
+    const int32_t *start, *end, *slot0, *nslot;  // per huge cone
+};
+
+// Rows handled by one workgroup of the XCD-sliced SpMV (all slices share them).
+struct SliceTile {
+    int32_t rb, re;
+};
+
+template <typename T>
+struct CsrT {
+    const int32_t *rowptr = nullptr, *col = nullptr;
+    const T *val = nullptr;
+    int32_t nrows = 0;
+    // XCD column slicing (nslices > 1): the columns are cut into nslices equal
+    // ranges and the entries regrouped slice-major.  Workgroup b of the slice
+    // kernel works on slice b % nslices — the XCD it is dispatched to — so each
+    // XCD gathers from one L2-resident slice of the input vector; per-row
+    // partial sums land in s_part[slice * nrows + row].  Row offsets are
+    // stored per slice as uint16 relative to the slice tile's base:
+    //   entries of (slice p, row r) start at s_base[p*(s_ntiles+1) + t] + s_off[p*nrows + r]
+    // with t the slice tile holding r; the tile's end is s_base[... + t + 1].
+    int32_t nslices = 1;
+    const int32_t *s_base = nullptr;
+    const uint16_t *s_off = nullptr;
+    const int32_t *s_col = nullptr;
+    const T *s_val = nullptr;
+    const SliceTile *s_tiles = nullptr;
+    int32_t s_ntiles = 0;
+    T *s_part = nullptr;
+};
+using CsrView = CsrT<double>;
+
+// Tile plan on the device for one operator (A or A^T).
+struct DevPlan {
+    Tile *tiles = nullptr;
+    int32_t ntiles = 0;       // all tiles
+    int32_t first_huge = 0;   // tiles[first_huge .. ntiles) are HUGE segments
+    int32_t nslots = 0;
+    int32_t nhuge = 0;
+    HugeInfo huge{};
+    int32_t *huge_buf = nullptr;  // backing store of huge.*
+};
+
+// ---- hot-path launches ------------------------------------------------------
+template <typename T>
+struct NoQPrimalArgsT {  // unified_update_zx (noQ), reference kernels.jl:5955-6063, 1775-3790
+    const T *y;              // SpMV input (A^T y)
+    T *x, *x_hat, *x_bar, *z_bar, *dx;
+    const T *last_x, *c, *l, *u;
+    T sigma, h1, h2;
+    int32_t full;
+};
+template <typename T>
+struct DualArgsT {  // unified_update_y (linear + SOC rows), kernels.jl:4354-5368
+    const T *v;              // SpMV input (A x_hat or A tempv)
+    T *y, *y_bar, *s, *dy;
+    const T *last_y, *AL, *AU, *rhs;
+    T f1, f2, h1, h2;
+    int32_t full, write_ybar;
+};
+template <typename T>
+struct QPPrimalArgsT {  // zxw1 + tempv, kernels.jl:379-1650, 4268-4299
+    T *x, *x_hat, *x_bar, *z_bar, *dx, *w_bar, *tempv;
+    const T *w, *last_x, *c, *l, *u, *aty, *qw, *dq, *fact1, *fact2;
+    T sigma, h1, h2, f1s, f2s;
+    int32_t full;
+    int32_t diag;                 // Q structurally diagonal: Qw = dq .* w, tempv written here
+    int32_t vec_fact;             // Q_is_diag: per-coordinate factors fact1/fact2
+};
+template <typename T>
+struct W2ArgsT {  // update_w2, kernels.jl:5384-5497
+    const T *ybar;           // SpMV input (A^T y_bar)
+    T *w, *w_bar, *aty, *dw, *atdy, *aty_bar;
+    const T *last_w, *last_aty, *fact;
+    T fs, h1, h2;
+    int32_t full, diag;
+};
+template <typename T>
+struct TempvArgsT {  // tempv = x_hat + sigma*(Qw - Q w_bar) for non-diagonal Q
+    const T *x_hat, *qw;
+    T *tempv;
+    const T *dq;             // factored Q diagonal (may be null)
+    const T *wbar;           // needed with dq
+    T sigma;
+};
+using NoQPrimalArgs = NoQPrimalArgsT<double>;
+using DualArgs = DualArgsT<double>;
+using QPPrimalArgs = QPPrimalArgsT<double>;
+using W2Args = W2ArgsT<double>;
+using TempvArgs = TempvArgsT<double>;
+
+// raw_buf: one T per row (HUGE cones); partials: one double per huge slot
+template <typename T>
+void launch_noq_primal(hipStream_t st, const DevPlan &plan, const CsrT<T> &AT, const int32_t *var_idx,
+                       const NoQPrimalArgsT<T> &a, T *raw_buf, double *partials);
+template <typename T>
+void launch_dual(hipStream_t st, const DevPlan &plan, const CsrT<T> &A, const int32_t *con_idx,
+                 const DualArgsT<T> &a, T *raw_buf, double *partials);
+template <typename T>
+void launch_qp_primal(hipStream_t st, const DevPlan &plan, const int32_t *var_idx,
+                      const QPPrimalArgsT<T> &a, T *raw_buf, double *partials);
+template <typename T>
+void launch_w2(hipStream_t st, const DevPlan &plan, const CsrT<T> &AT, const W2ArgsT<T> &a);
+// out = M v (+ diag .* v2 when diag != null); plan must be built for M.
+template <typename T>
+void launch_spmv(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, const T *v, T *out,
+                 const T *diag = nullptr, const T *v2 = nullptr);
+template <typename T>
+void launch_spmv_tempv(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, const T *wbar,
+                       const TempvArgsT<T> &a);
+
+}  // namespace hpr

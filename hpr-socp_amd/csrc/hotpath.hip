@@ -1,0 +1,414 @@
+// hotpath.hip — the fused HPR kernels (SpMV + projection + reflection + Halpern),
+// templated on the value type T (double: FP64 method; float: FP32 bulk).
+//
+// One workgroup (256 threads = 4 wave64) per Tile.  Phase A produces the tile's
+// row sums in LDS: CSR-stream (coalesced loads, independent gathers, products
+// in LDS, per-row sums in CSR order), lane groups of g lanes per row for long
+// rows, or — when the operator is XCD-sliced — the fixed-order sum of the slice
+// partials written by k_spmv_slices.  Phase B is the epilogue: box clamp and
+// Halpern averaging for LINEAR tiles; for CONES tiles a lane group of g2 lanes
+// owns one cone, reduces |xi|^2 in FP64 with shuffles and applies the SOC
+// projection in registers; HUGE tiles store raw values and a partial |xi|^2,
+// and k_huge_finish projects once all partials of the cone exist.
+#include "devutil.h"
+#include "hotpath.h"
+
+namespace hpr {
+
+// ---------------------------------------------------------------- SpMV phase
+template <typename T>
+__device__ void tile_spmv_stream(const CsrT<T> &M, const T *__restrict__ v, const Tile &t, T *acc, T *prod,
+                                 int32_t *rp) {
+    const int tid = threadIdx.x;
+    const int nrows = t.re - t.rb;
+    for (int r = tid; r <= nrows; r += kBlock) rp[r] = M.rowptr[t.rb + r];
+    __syncthreads();
+    const int k0 = rp[0];
+    const int nnz = rp[nrows] - k0;
+    const int32_t *__restrict__ col = M.col + k0;
+    const T *__restrict__ val = M.val + k0;
+#pragma unroll 4
+    for (int k = tid; k < nnz; k += kBlock) prod[k] = val[k] * v[col[k]];
+    __syncthreads();
+    if (tid < nrows) {
+        const int a = rp[tid] - k0, b = rp[tid + 1] - k0;
+        T s = T(0);
+        for (int k = a; k < b; ++k) s += prod[k];
+        acc[tid] = s;
+    }
+}
+
+template <typename T>
+__device__ void tile_spmv(const CsrT<T> &M, const T *__restrict__ v, const Tile &t, T *acc, double *sh) {
+    const int tid = threadIdx.x;
+    const int nrows = t.re - t.rb;
+    const int G = t.g;
+    if (G <= 64) {
+        const int lane = tid & (G - 1), grp = tid / G, ngrp = kBlock / G;
+        for (int r = grp; r < nrows; r += ngrp) {
+            const int row = t.rb + r;
+            const int s = M.rowptr[row], e = M.rowptr[row + 1];
+            T sum = T(0);
+            for (int k = s + lane; k < e; k += G) sum += M.val[k] * v[M.col[k]];
+            for (int o = G >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, G);
+            if (lane == 0) acc[r] = sum;
+        }
+    } else {
+        for (int r = 0; r < nrows; ++r) {
+            const int row = t.rb + r;
+            const int s = M.rowptr[row], e = M.rowptr[row + 1];
+            T sum = T(0);
+            for (int k = s + tid; k < e; k += kBlock) sum += M.val[k] * v[M.col[k]];
+            // workgroup sum in T (wave butterflies, fixed order)
+            for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+            __syncthreads();
+            if ((tid & 63) == 0) sh[tid >> 6] = (double)sum;
+            __syncthreads();
+            if (tid == 0) acc[r] = (T)sh[0] + (T)sh[1] + (T)sh[2] + (T)sh[3];
+        }
+    }
+}
+
+// Projection scalars of one cone from its head value t and |xi| (FP64).
+struct ConeCase {
+    int cas;  // 0 polar (-> 0), 1 inside (keep), 2 boundary projection
+    double pt, alpha;
+};
+__device__ __forceinline__ ConeCase cone_case(double t, double ns) {
+    ConeCase c;
+    c.pt = 0.0;
+    c.alpha = 0.0;
+    if (ns <= -t) c.cas = 0;
+    else if (ns <= t) c.cas = 1;
+    else {
+        c.cas = 2;
+        c.pt = (ns + t) / 2.0;
+        c.alpha = c.pt / ns;
+    }
+    return c;
+}
+template <typename T>
+__device__ __forceinline__ T cone_apply(const ConeCase &c, bool head, T raw) {
+    return c.cas == 0 ? T(0) : (c.cas == 1 ? raw : (head ? (T)c.pt : (T)(c.alpha * (double)raw)));
+}
+
+// ------------------------------------------------------------ epilogue ops
+template <typename T>
+struct OpNoQPrimal {
+    NoQPrimalArgsT<T> a;
+    __device__ T raw(int i, T acc, bool) const { return a.x[i] + a.sigma * (acc - a.c[i]); }
+    __device__ T lin_proj(int i, T r) const { return jclamp(r, a.l[i], a.u[i]); }
+    __device__ void finish(int i, T raw, T p) const {
+        const T xi = a.x[i];
+        const T xh = T(2) * p - xi;
+        if (a.full) {
+            a.x_bar[i] = p;
+            a.z_bar[i] = (p - raw) / a.sigma;
+            a.dx[i] = p - xi;
+        }
+        a.x[i] = a.h2 * xh + a.h1 * a.last_x[i];
+        a.x_hat[i] = xh;
+    }
+};
+
+template <typename T>
+struct OpDual {
+    DualArgsT<T> a;
+    __device__ T raw(int i, T acc, bool cone) const {
+        return cone ? (acc - a.rhs[i]) - a.f1 * a.y[i] : acc - a.f1 * a.y[i];
+    }
+    __device__ T lin_proj(int i, T r) const { return jclamp(r, a.AL[i], a.AU[i]); }
+    __device__ void finish(int i, T raw, T p) const {
+        const T yi = a.y[i];
+        const T yb = a.f2 * (p - raw);
+        if (a.full) {
+            a.s[i] = p;
+            a.dy[i] = yb - yi;
+        }
+        if (a.full || a.write_ybar) a.y_bar[i] = yb;
+        a.y[i] = a.h2 * (T(2) * yb - yi) + a.h1 * a.last_y[i];
+    }
+};
+
+template <typename T>
+struct OpQPPrimal {
+    QPPrimalArgsT<T> a;
+    __device__ T qw(int i) const { return a.diag ? a.dq[i] * a.w[i] : a.qw[i]; }
+    __device__ T raw(int i, T, bool) const { return a.x[i] + a.sigma * ((-qw(i) + a.aty[i]) - a.c[i]); }
+    __device__ T lin_proj(int i, T r) const { return jclamp(r, a.l[i], a.u[i]); }
+    __device__ void finish(int i, T raw, T p) const {
+        const T xi = a.x[i], wi = a.w[i];
+        const T xh = T(2) * p - xi;
+        const T wb = a.vec_fact ? a.fact1[i] * wi + a.fact2[i] * xh : a.f1s * wi + a.f2s * xh;
+        if (a.full) {
+            a.x_bar[i] = p;
+            a.z_bar[i] = (p - raw) / a.sigma;
+            a.dx[i] = p - xi;
+        }
+        a.x[i] = a.h2 * xh + a.h1 * a.last_x[i];
+        a.x_hat[i] = xh;
+        a.w_bar[i] = wb;
+        if (a.diag) a.tempv[i] = xh + a.sigma * (a.dq[i] * wi - a.dq[i] * wb);
+    }
+};
+
+template <typename T>
+struct OpW2 {
+    W2ArgsT<T> a;
+    __device__ T raw(int, T acc, bool) const { return acc; }
+    __device__ T lin_proj(int, T r) const { return r; }
+    __device__ void finish(int i, T rb, T) const {
+        const T wi = a.w[i], atyi = a.aty[i];
+        const T f = a.diag ? a.fact[i] : a.fs;
+        const T wbn = a.w_bar[i] + f * (rb - atyi);
+        a.w[i] = a.h1 * a.last_w[i] + a.h2 * (T(2) * wbn - wi);
+        a.aty[i] = a.h1 * a.last_aty[i] + a.h2 * (T(2) * rb - atyi);
+        if (a.full) {
+            a.w_bar[i] = wbn;
+            a.dw[i] = wbn - wi;
+            a.atdy[i] = rb - atyi;
+            a.aty_bar[i] = rb;
+        }
+    }
+};
+
+template <typename T>
+struct OpStore {
+    T *out;
+    const T *diag, *v2;
+    __device__ T raw(int, T acc, bool) const { return acc; }
+    __device__ T lin_proj(int, T r) const { return r; }
+    __device__ void finish(int i, T r, T) const { out[i] = diag ? r + diag[i] * v2[i] : r; }
+};
+
+template <typename T>
+struct OpTempv {
+    TempvArgsT<T> a;
+    __device__ T raw(int, T acc, bool) const { return acc; }
+    __device__ T lin_proj(int, T r) const { return r; }
+    __device__ void finish(int i, T r, T) const {
+        const T qwb = a.dq ? r + a.dq[i] * a.wbar[i] : r;
+        a.tempv[i] = a.x_hat[i] + a.sigma * (a.qw[i] - qwb);
+    }
+};
+
+// ------------------------------------------------------------ XCD-sliced SpMV
+// Block b: slice p = b % P (the XCD it is dispatched to), slice tile b / P.  The
+// slice's nonzeros of the tile are streamed (non-temporal: read once) in chunks
+// of kStreamNnz products through LDS; every row sums its products in CSR order.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv_slices(CsrT<T> M, const T *__restrict__ x) {
+    __shared__ T prod[kStreamNnz];
+    __shared__ int32_t rps[kSliceRows + 1];
+    const int P = M.nslices;
+    const int p = blockIdx.x % P;
+    const int tix = blockIdx.x / P;
+    const SliceTile S = M.s_tiles[tix];
+    const int tid = threadIdx.x;
+    const int nr = S.re - S.rb;
+    const int32_t *base = M.s_base + (int64_t)p * (M.s_ntiles + 1) + tix;
+    const int32_t k0 = base[0];
+    const int32_t tnnz = base[1] - k0;
+    const uint16_t *off = M.s_off + (int64_t)p * M.nrows + S.rb;
+    for (int r = tid; r < nr; r += kBlock) rps[r] = off[r];
+    if (tid == 0) rps[nr] = tnnz;
+    __syncthreads();
+    constexpr int RPT = kSliceRows / kBlock;  // rows per thread
+    T acc[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) acc[j] = T(0);
+    const T *__restrict__ val = M.s_val + k0;
+    const int32_t *__restrict__ col = M.s_col + k0;
+    for (int c0 = 0; c0 < tnnz; c0 += kStreamNnz) {
+        const int c1 = min(c0 + kStreamNnz, tnnz);
+#pragma unroll 4
+        for (int k = c0 + tid; k < c1; k += kBlock) {
+            const T v = __builtin_nontemporal_load(val + k);
+            const int c = __builtin_nontemporal_load(col + k);
+            prod[k - c0] = v * x[c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int r = tid + j * kBlock;
+            if (r < nr) {
+                const int a = max(rps[r], c0), e = min(rps[r + 1], c1);
+                T s = acc[j];
+                for (int k = a; k < e; ++k) s += prod[k - c0];
+                acc[j] = s;
+            }
+        }
+        __syncthreads();
+    }
+    T *out = M.s_part + (int64_t)p * M.nrows + S.rb;
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int r = tid + j * kBlock;
+        if (r < nr) out[r] = acc[j];
+    }
+}
+
+// ------------------------------------------------------------ tile kernel
+// SRC: 0 = no SpMV, 1 = CSR SpMV inside the tile, 2 = sum of the XCD-slice partials
+enum : int { SRC_NONE = 0, SRC_CSR = 1, SRC_SLICES = 2 };
+
+template <typename T, class Op, int SRC, bool CONES>
+__global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles, CsrT<T> M, const T *__restrict__ vin,
+                                                 const int32_t *__restrict__ cone_idx, HugeInfo huge, Op op,
+                                                 T *__restrict__ raw_buf, double *__restrict__ partials) {
+    __shared__ T acc[kTileRows];
+    __shared__ double sh[4];
+    __shared__ T prod[SRC == SRC_CSR ? kStreamNnz : 1];
+    __shared__ int32_t rps[SRC == SRC_CSR ? kTileRows + 1 : 1];
+    const Tile t = tiles[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int nrows = t.re - t.rb;
+    if constexpr (SRC == SRC_CSR) {
+        if (t.g == 0) tile_spmv_stream(M, vin, t, acc, prod, rps);
+        else tile_spmv(M, vin, t, acc, sh);
+    } else if constexpr (SRC == SRC_SLICES) {
+        if (tid < nrows) {
+            const T *pp = M.s_part + t.rb + tid;
+            T a = T(0);
+            for (int p = 0; p < M.nslices; ++p) a += pp[(int64_t)p * M.nrows];
+            acc[tid] = a;
+        }
+    } else {
+        if (tid < nrows) acc[tid] = T(0);
+    }
+    __syncthreads();
+    if (!CONES || t.kind == TILE_LINEAR) {
+        if (tid < nrows) {
+            const int i = t.rb + tid;
+            const T r = op.raw(i, acc[tid], false);
+            op.finish(i, r, op.lin_proj(i, r));
+        }
+        return;
+    }
+    if (t.kind == TILE_CONES) {
+        const int G2 = t.g2;
+        const int lane = tid & (G2 - 1), grp = tid / G2, ngrp = kBlock / G2;
+        for (int c = t.c0 + grp; c < t.c1; c += ngrp) {
+            const int s = cone_idx[c], e = cone_idx[c + 1];
+            double sq = 0.0, head = 0.0;
+            for (int i = s + lane; i < e; i += G2) {
+                const T r = op.raw(i, acc[i - t.rb], true);
+                acc[i - t.rb] = r;
+                if (i == s) head = (double)r;
+                else sq += (double)r * (double)r;
+            }
+            for (int o = G2 >> 1; o > 0; o >>= 1) sq += __shfl_xor(sq, o, G2);
+            head = __shfl(head, 0, G2);
+            const ConeCase cc = cone_case(head, sqrt(sq));
+            for (int i = s + lane; i < e; i += G2) {
+                const T r = acc[i - t.rb];
+                op.finish(i, r, cone_apply(cc, i == s, r));
+            }
+        }
+        return;
+    }
+    // HUGE segment, phase 1: raw values and a partial |xi|^2
+    double sq = 0.0;
+    if (tid < nrows) {
+        const int i = t.rb + tid;
+        const T r = op.raw(i, acc[tid], true);
+        raw_buf[i] = r;
+        if (i != huge.start[t.c0]) sq = (double)r * (double)r;
+    }
+    sq = block_sum(sq, sh);
+    if (tid == 0) partials[t.c1] = sq;
+}
+
+// HUGE segment, phase 2: the cone's |xi| from all partials, then the projection.
+template <typename T, class Op>
+__global__ __launch_bounds__(kBlock) void k_huge_finish(const Tile *__restrict__ tiles, HugeInfo huge, Op op,
+                                                        const T *__restrict__ raw_buf,
+                                                        const double *__restrict__ partials) {
+    __shared__ double sh[4];
+    const Tile t = tiles[blockIdx.x];
+    const int h = t.c0;
+    const int s = huge.start[h];
+    const int slot0 = huge.slot0[h], nslot = huge.nslot[h];
+    double sum = 0.0;
+    for (int k = threadIdx.x; k < nslot; k += kBlock) sum += partials[slot0 + k];
+    sum = block_sum(sum, sh);
+    const ConeCase cc = cone_case((double)raw_buf[s], sqrt(sum));
+    const int nrows = t.re - t.rb;
+    if ((int)threadIdx.x < nrows) {
+        const int i = t.rb + threadIdx.x;
+        const T r = raw_buf[i];
+        op.finish(i, r, cone_apply(cc, i == s, r));
+    }
+}
+
+// Fused SpMV + epilogue over a plan.  With XCD slicing the SpMV runs first as
+// k_spmv_slices and the tile kernel sums the slice partials.
+template <typename T, class Op, int SRC, bool CONES>
+static void run_tiles(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, const T *v, const int32_t *cone_idx,
+                      const Op &op, T *raw_buf, double *partials) {
+    if (plan.ntiles == 0) return;
+    if (SRC == SRC_CSR && M.nslices > 1) {
+        if (M.s_ntiles > 0)
+            hipLaunchKernelGGL((k_spmv_slices<T>), dim3(M.s_ntiles * M.nslices), dim3(kBlock), 0, st, M, v);
+        hipLaunchKernelGGL((k_tile<T, Op, SRC_SLICES, CONES>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles, M,
+                           v, cone_idx, plan.huge, op, raw_buf, partials);
+    } else {
+        hipLaunchKernelGGL((k_tile<T, Op, SRC, CONES>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles, M, v,
+                           cone_idx, plan.huge, op, raw_buf, partials);
+    }
+    if (CONES) {
+        const int nh = plan.ntiles - plan.first_huge;
+        if (nh > 0)
+            hipLaunchKernelGGL((k_huge_finish<T, Op>), dim3(nh), dim3(kBlock), 0, st, plan.tiles + plan.first_huge,
+                               plan.huge, op, raw_buf, partials);
+    }
+}
+
+template <typename T>
+void launch_noq_primal(hipStream_t st, const DevPlan &plan, const CsrT<T> &AT, const int32_t *var_idx,
+                       const NoQPrimalArgsT<T> &a, T *raw_buf, double *partials) {
+    run_tiles<T, OpNoQPrimal<T>, SRC_CSR, true>(st, plan, AT, a.y, var_idx, OpNoQPrimal<T>{a}, raw_buf, partials);
+}
+template <typename T>
+void launch_dual(hipStream_t st, const DevPlan &plan, const CsrT<T> &A, const int32_t *con_idx,
+                 const DualArgsT<T> &a, T *raw_buf, double *partials) {
+    run_tiles<T, OpDual<T>, SRC_CSR, true>(st, plan, A, a.v, con_idx, OpDual<T>{a}, raw_buf, partials);
+}
+template <typename T>
+void launch_qp_primal(hipStream_t st, const DevPlan &plan, const int32_t *var_idx, const QPPrimalArgsT<T> &a,
+                      T *raw_buf, double *partials) {
+    CsrT<T> none{};
+    run_tiles<T, OpQPPrimal<T>, SRC_NONE, true>(st, plan, none, nullptr, var_idx, OpQPPrimal<T>{a}, raw_buf,
+                                                partials);
+}
+template <typename T>
+void launch_w2(hipStream_t st, const DevPlan &plan, const CsrT<T> &AT, const W2ArgsT<T> &a) {
+    run_tiles<T, OpW2<T>, SRC_CSR, false>(st, plan, AT, a.ybar, nullptr, OpW2<T>{a}, nullptr, nullptr);
+}
+template <typename T>
+void launch_spmv(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, const T *v, T *out, const T *diag,
+                 const T *v2) {
+    run_tiles<T, OpStore<T>, SRC_CSR, false>(st, plan, M, v, nullptr, OpStore<T>{out, diag, v2}, nullptr, nullptr);
+}
+template <typename T>
+void launch_spmv_tempv(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, const T *wbar,
+                       const TempvArgsT<T> &a) {
+    run_tiles<T, OpTempv<T>, SRC_CSR, false>(st, plan, M, wbar, nullptr, OpTempv<T>{a}, nullptr, nullptr);
+}
+
+#define HPR_INSTANTIATE(T)                                                                                        \
+    template void launch_noq_primal<T>(hipStream_t, const DevPlan &, const CsrT<T> &, const int32_t *,          \
+                                       const NoQPrimalArgsT<T> &, T *, double *);                                \
+    template void launch_dual<T>(hipStream_t, const DevPlan &, const CsrT<T> &, const int32_t *,                \
+                                 const DualArgsT<T> &, T *, double *);                                           \
+    template void launch_qp_primal<T>(hipStream_t, const DevPlan &, const int32_t *, const QPPrimalArgsT<T> &,  \
+                                      T *, double *);                                                            \
+    template void launch_w2<T>(hipStream_t, const DevPlan &, const CsrT<T> &, const W2ArgsT<T> &);              \
+    template void launch_spmv<T>(hipStream_t, const DevPlan &, const CsrT<T> &, const T *, T *, const T *,      \
+                                 const T *);                                                                     \
+    template void launch_spmv_tempv<T>(hipStream_t, const DevPlan &, const CsrT<T> &, const T *,                \
+                                       const TempvArgsT<T> &);
+HPR_INSTANTIATE(double)
+HPR_INSTANTIATE(float)
+
+}  // namespace hpr

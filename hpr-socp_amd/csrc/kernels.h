@@ -6,120 +6,11 @@
 
 namespace hpr {
 
-constexpr int kBlock = 256;       // threads per workgroup (4 wavefronts of 64)
-constexpr int kTileRows = 256;    // max rows of one tile (one row per thread in epilogues)
-constexpr int kStreamNnz = 2048;  // LDS product buffer of the CSR-stream SpMV path
+}  // namespace hpr
 
-enum TileKind : int32_t { TILE_LINEAR = 0, TILE_CONES = 1, TILE_HUGE = 2 };
+#include "hotpath.h"
 
-// One workgroup's unit of work: rows [rb, re) of an operator.
-//   LINEAR : independent rows (box coordinates / linear constraint rows)
-//   CONES  : whole cones c0..c1-1 (each with at most kTileRows rows)
-//   HUGE   : a row segment of a long cone; c0 = huge-cone id, c1 = partial slot
-// g  : SpMV phase: 0 = CSR-stream (the tile's <= kStreamNnz products are
-//      gathered by the whole workgroup into LDS, then summed per row in CSR
-//      order); otherwise lanes per row (1..64, or 256 = whole workgroup)
-// g2 : lanes per cone of the projection phase (CONES tiles)
-struct Tile {
-    int32_t rb, re, kind, g, c0, c1, g2, pad;
-};
-
-// Long cones split over several tiles (two-phase projection).
-struct HugeInfo {
-    const int32_t *start, *end, *slot0, *nslot;  // per huge cone
-};
-
-// Rows handled by one workgroup of the XCD-sliced SpMV (all slices share them).
-struct SliceTile {
-    int32_t rb, re;
-};
-
-struct CsrView {
-    const int32_t *rowptr, *col;
-    const double *val;
-    int32_t nrows;
-    // XCD column slicing (nslices > 1): the columns are cut into nslices equal
-    // ranges and the entries regrouped slice-major (per-slice CSR, concatenated).
-    // Workgroup b of the slice kernel works on slice b % nslices — the XCD it is
-    // dispatched to — so each XCD gathers from one L2-resident slice of the
-    // input vector; per-row partial sums land in s_part[slice * nrows + row].
-    int32_t nslices = 1;
-    const int32_t *s_rowptr = nullptr;  // nslices * (nrows + 1)
-    const int32_t *s_col = nullptr;
-    const double *s_val = nullptr;
-    const SliceTile *s_tiles = nullptr;
-    int32_t s_ntiles = 0;
-    double *s_part = nullptr;
-};
-constexpr int kSliceRows = 1024;        // rows of one slice tile
-
-struct CsrViewF {
-    const int32_t *rowptr, *col;
-    const float *val;
-    int32_t nrows;
-};
-
-// Tile plan on the device for one operator (A or A^T).
-struct DevPlan {
-    Tile *tiles = nullptr;
-    int32_t ntiles = 0;       // all tiles
-    int32_t first_huge = 0;   // tiles[first_huge .. ntiles) are HUGE segments
-    int32_t nslots = 0;
-    int32_t nhuge = 0;
-    HugeInfo huge{};
-    int32_t *huge_buf = nullptr;  // backing store of huge.*
-};
-
-// ---- hot-path launches (FP64 iterates) -------------------------------------
-struct NoQPrimalArgs {  // unified_update_zx (noQ), reference kernels.jl:5955-6063, 1775-3790
-    const double *y;              // SpMV input (A^T y)
-    double *x, *x_hat, *x_bar, *z_bar, *dx;
-    const double *last_x, *c, *l, *u;
-    double sigma, h1, h2;
-    int32_t full;
-};
-struct DualArgs {  // unified_update_y (linear + SOC rows), kernels.jl:4354-5368
-    const double *v;              // SpMV input (A x_hat or A tempv)
-    double *y, *y_bar, *s, *dy;
-    const double *last_y, *AL, *AU, *rhs;
-    double f1, f2, h1, h2;
-    int32_t full, write_ybar;
-};
-struct QPPrimalArgs {  // zxw1 + tempv, kernels.jl:379-1650, 4268-4299
-    double *x, *x_hat, *x_bar, *z_bar, *dx, *w_bar, *tempv;
-    const double *w, *last_x, *c, *l, *u, *aty, *qw, *dq, *fact1, *fact2;
-    double sigma, h1, h2, f1s, f2s;
-    int32_t full;
-    int32_t diag;                 // Q structurally diagonal: Qw = dq .* w, tempv written here
-    int32_t vec_fact;             // Q_is_diag: per-coordinate factors fact1/fact2
-};
-struct W2Args {  // update_w2, kernels.jl:5384-5497
-    const double *ybar;           // SpMV input (A^T y_bar)
-    double *w, *w_bar, *aty, *dw, *atdy, *aty_bar;
-    const double *last_w, *last_aty, *fact;
-    double fs, h1, h2;
-    int32_t full, diag;
-};
-struct TempvArgs {  // tempv = x_hat + sigma*(Qw - Q w_bar) for non-diagonal Q
-    const double *x_hat, *qw;
-    double *tempv;
-    const double *dq;             // factored Q diagonal (may be null)
-    const double *wbar;           // needed with dq
-    double sigma;
-};
-
-void launch_noq_primal(hipStream_t st, const DevPlan &plan, const CsrView &AT, const int32_t *var_idx,
-                       const NoQPrimalArgs &a, double *raw_buf, double *partials);
-void launch_dual(hipStream_t st, const DevPlan &plan, const CsrView &A, const int32_t *con_idx,
-                 const DualArgs &a, double *raw_buf, double *partials);
-void launch_qp_primal(hipStream_t st, const DevPlan &plan, const int32_t *var_idx,
-                      const QPPrimalArgs &a, double *raw_buf, double *partials);
-void launch_w2(hipStream_t st, const DevPlan &plan, const CsrView &AT, const W2Args &a);
-// out = M v (+ diag .* v2 when diag != null); plan must be built for M.
-void launch_spmv(hipStream_t st, const DevPlan &plan, const CsrView &M, const double *v, double *out,
-                 const double *diag = nullptr, const double *v2 = nullptr);
-void launch_spmv_tempv(hipStream_t st, const DevPlan &plan, const CsrView &M, const double *wbar,
-                       const TempvArgs &a);
+namespace hpr {
 
 // ---- reductions ---------------------------------------------------------
 constexpr int kMaxRed = 8;

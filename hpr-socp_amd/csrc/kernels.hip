@@ -1,13 +1,8 @@
-// kernels.hip — gfx950 kernels of the HPR iteration (FP64 iterates).
-//
-// Hot path: one workgroup (256 threads = 4 wave64) per Tile.  Phase A runs the
-// CSR SpMV rows of the tile with `g` lanes per row (shuffle reduction inside
-// the lane group, or the whole workgroup for very long rows) into LDS.  Phase B
-// is the fused epilogue: box clamp + reflection + Halpern averaging for
-// LINEAR tiles; for CONES tiles a lane group of `g2` lanes owns one cone,
-// reduces ||xi||^2 with shuffles and applies the SOC projection in registers;
-// HUGE tiles write raw values and a partial ||xi||^2 and a second kernel
-// finishes the projection once all partials of the cone exist.
+// kernels.hip — FP64 kernels around the hot path: deterministic reductions,
+// residuals / gap / bounds violation, the batched golden-section evaluation,
+// power-iteration steps, scaling passes and small elementwise helpers.
+// The fused SpMV + projection + Halpern kernels are in hotpath.hip.
+#include "devutil.h"
 #include "kernels.h"
 #include "scale_rules.h"
 
@@ -15,413 +10,7 @@
 
 namespace hpr {
 
-// NaN-propagating helpers (Julia's max/min/clamp semantics on the CPU path).
-__device__ __forceinline__ double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
-__device__ __forceinline__ double jclamp(double z, double lo, double hi) {
-    return (z != z) ? z : fmin(fmax(z, lo), hi);
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-// Deterministic block reductions; result valid in every thread.
-__device__ double block_sum(double v, double *sh) {
-    v = wave_sum(v);
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[w] = v;
-    __syncthreads();
-    double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-    return r;
-}
-__device__ double block_max(double v, double *sh) {
-    v = wave_max(v);
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[w] = v;
-    __syncthreads();
-    return nmax(nmax(sh[0], sh[1]), nmax(sh[2], sh[3]));
-}
-
-// ---------------------------------------------------------------- SpMV phase
-// CSR-stream: coalesced loads of the tile's nonzeros, independent gathers, one
-// product per LDS slot, then a sequential per-row sum (CSR order, products
-// rounded before summation).
-__device__ void tile_spmv_stream(const CsrView &M, const double *__restrict__ v, const Tile &t,
-                                 double *acc, double *prod, int32_t *rp) {
-    const int tid = threadIdx.x;
-    const int nrows = t.re - t.rb;
-    for (int r = tid; r <= nrows; r += kBlock) rp[r] = M.rowptr[t.rb + r];
-    __syncthreads();
-    const int k0 = rp[0];
-    const int nnz = rp[nrows] - k0;
-    const int32_t *__restrict__ col = M.col + k0;
-    const double *__restrict__ val = M.val + k0;
-#pragma unroll 4
-    for (int k = tid; k < nnz; k += kBlock) {
-        const double p = val[k] * v[col[k]];
-        prod[k] = p;
-    }
-    __syncthreads();
-    if (tid < nrows) {
-        const int a = rp[tid] - k0, b = rp[tid + 1] - k0;
-        double s = 0.0;
-        for (int k = a; k < b; ++k) s += prod[k];
-        acc[tid] = s;
-    }
-}
-
-// Vector path: g lanes per row (shuffle reduction), or the whole workgroup for
-// very long rows.
-__device__ void tile_spmv(const CsrView &M, const double *__restrict__ v, const Tile &t,
-                          double *acc, double *sh) {
-    const int tid = threadIdx.x;
-    const int nrows = t.re - t.rb;
-    const int G = t.g;
-    if (G <= 64) {
-        const int lane = tid & (G - 1), grp = tid / G, ngrp = kBlock / G;
-        for (int r = grp; r < nrows; r += ngrp) {
-            const int row = t.rb + r;
-            const int s = M.rowptr[row], e = M.rowptr[row + 1];
-            double sum = 0.0;
-            for (int k = s + lane; k < e; k += G) sum += M.val[k] * v[M.col[k]];
-            for (int o = G >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, G);
-            if (lane == 0) acc[r] = sum;
-        }
-    } else {
-        for (int r = 0; r < nrows; ++r) {
-            const int row = t.rb + r;
-            const int s = M.rowptr[row], e = M.rowptr[row + 1];
-            double sum = 0.0;
-            for (int k = s + tid; k < e; k += kBlock) sum += M.val[k] * v[M.col[k]];
-            sum = block_sum(sum, sh);
-            if (tid == 0) acc[r] = sum;
-        }
-    }
-}
-
-// Projection scalars of one cone from its head value t and ||xi||.
-struct ConeCase {
-    int cas;        // 0 polar (-> 0), 1 inside (keep), 2 boundary projection
-    double pt, alpha;
-};
-__device__ __forceinline__ ConeCase cone_case(double t, double ns) {
-    ConeCase c;
-    c.pt = 0.0;
-    c.alpha = 0.0;
-    if (ns <= -t) c.cas = 0;
-    else if (ns <= t) c.cas = 1;
-    else {
-        c.cas = 2;
-        c.pt = (ns + t) / 2.0;
-        c.alpha = c.pt / ns;
-    }
-    return c;
-}
-__device__ __forceinline__ double cone_apply(const ConeCase &c, bool head, double raw) {
-    return c.cas == 0 ? 0.0 : (c.cas == 1 ? raw : (head ? c.pt : c.alpha * raw));
-}
-
-// ------------------------------------------------------------ epilogue ops
-struct OpNoQPrimal {
-    NoQPrimalArgs a;
-    __device__ double raw(int i, double acc, bool) const { return a.x[i] + a.sigma * (acc - a.c[i]); }
-    __device__ double lin_proj(int i, double r) const { return jclamp(r, a.l[i], a.u[i]); }
-    __device__ void finish(int i, double raw, double p) const {
-        const double xi = a.x[i];
-        const double xh = 2.0 * p - xi;
-        if (a.full) {
-            a.x_bar[i] = p;
-            a.z_bar[i] = (p - raw) / a.sigma;
-            a.dx[i] = p - xi;
-        }
-        a.x[i] = a.h2 * xh + a.h1 * a.last_x[i];
-        a.x_hat[i] = xh;
-    }
-};
-
-struct OpDual {
-    DualArgs a;
-    __device__ double raw(int i, double acc, bool cone) const {
-        return cone ? (acc - a.rhs[i]) - a.f1 * a.y[i] : acc - a.f1 * a.y[i];
-    }
-    __device__ double lin_proj(int i, double r) const { return jclamp(r, a.AL[i], a.AU[i]); }
-    __device__ void finish(int i, double raw, double p) const {
-        const double yi = a.y[i];
-        const double yb = a.f2 * (p - raw);
-        if (a.full) {
-            a.s[i] = p;
-            a.dy[i] = yb - yi;
-        }
-        if (a.full || a.write_ybar) a.y_bar[i] = yb;
-        a.y[i] = a.h2 * (2.0 * yb - yi) + a.h1 * a.last_y[i];
-    }
-};
-
-struct OpQPPrimal {
-    QPPrimalArgs a;
-    __device__ double qw(int i) const { return a.diag ? a.dq[i] * a.w[i] : a.qw[i]; }
-    __device__ double raw(int i, double, bool) const {
-        return a.x[i] + a.sigma * ((-qw(i) + a.aty[i]) - a.c[i]);
-    }
-    __device__ double lin_proj(int i, double r) const { return jclamp(r, a.l[i], a.u[i]); }
-    __device__ void finish(int i, double raw, double p) const {
-        const double xi = a.x[i], wi = a.w[i];
-        const double xh = 2.0 * p - xi;
-        const double wb = a.vec_fact ? a.fact1[i] * wi + a.fact2[i] * xh : a.f1s * wi + a.f2s * xh;
-        if (a.full) {
-            a.x_bar[i] = p;
-            a.z_bar[i] = (p - raw) / a.sigma;
-            a.dx[i] = p - xi;
-        }
-        a.x[i] = a.h2 * xh + a.h1 * a.last_x[i];
-        a.x_hat[i] = xh;
-        a.w_bar[i] = wb;
-        if (a.diag) a.tempv[i] = xh + a.sigma * (a.dq[i] * wi - a.dq[i] * wb);
-    }
-};
-
-struct OpW2 {
-    W2Args a;
-    __device__ double raw(int, double acc, bool) const { return acc; }
-    __device__ double lin_proj(int, double r) const { return r; }
-    __device__ void finish(int i, double rb, double) const {
-        const double wi = a.w[i], atyi = a.aty[i];
-        const double f = a.diag ? a.fact[i] : a.fs;
-        const double wbn = a.w_bar[i] + f * (rb - atyi);
-        a.w[i] = a.h1 * a.last_w[i] + a.h2 * (2.0 * wbn - wi);
-        a.aty[i] = a.h1 * a.last_aty[i] + a.h2 * (2.0 * rb - atyi);
-        if (a.full) {
-            a.w_bar[i] = wbn;
-            a.dw[i] = wbn - wi;
-            a.atdy[i] = rb - atyi;
-            a.aty_bar[i] = rb;
-        }
-    }
-};
-
-struct OpStore {
-    double *out;
-    const double *diag, *v2;
-    __device__ double raw(int, double acc, bool) const { return acc; }
-    __device__ double lin_proj(int, double r) const { return r; }
-    __device__ void finish(int i, double r, double) const { out[i] = diag ? r + diag[i] * v2[i] : r; }
-};
-
-struct OpTempv {
-    TempvArgs a;
-    __device__ double raw(int, double acc, bool) const { return acc; }
-    __device__ double lin_proj(int, double r) const { return r; }
-    __device__ void finish(int i, double r, double) const {
-        const double qwb = a.dq ? r + a.dq[i] * a.wbar[i] : r;
-        a.tempv[i] = a.x_hat[i] + a.sigma * (a.qw[i] - qwb);
-    }
-};
-
-// ------------------------------------------------------------ XCD-sliced SpMV
-// Block b: slice p = b % P (the XCD it is dispatched to), row tile b / P.  The
-// slice's nonzeros of the tile are streamed (non-temporal: read once) in chunks
-// of kStreamNnz products through LDS; every row sums its products in CSR order.
-__global__ __launch_bounds__(kBlock) void k_spmv_slices(CsrView M, const double *__restrict__ x) {
-    __shared__ double prod[kStreamNnz];
-    __shared__ int32_t rps[kSliceRows + 1];
-    const int P = M.nslices;
-    const int p = blockIdx.x % P;
-    const SliceTile T = M.s_tiles[blockIdx.x / P];
-    const int tid = threadIdx.x;
-    const int nr = T.re - T.rb;
-    const int32_t *rp = M.s_rowptr + (int64_t)p * (M.nrows + 1);
-    for (int r = tid; r <= nr; r += kBlock) rps[r] = rp[T.rb + r];
-    __syncthreads();
-    const int k0 = rps[0], k1 = rps[nr];
-    constexpr int RPT = kSliceRows / kBlock;  // rows per thread
-    double acc[RPT];
-#pragma unroll
-    for (int j = 0; j < RPT; ++j) acc[j] = 0.0;
-    for (int c0 = k0; c0 < k1; c0 += kStreamNnz) {
-        const int c1 = min(c0 + kStreamNnz, k1);
-#pragma unroll 4
-        for (int k = c0 + tid; k < c1; k += kBlock) {
-            const double v = __builtin_nontemporal_load(M.s_val + k);
-            const int c = __builtin_nontemporal_load(M.s_col + k);
-            prod[k - c0] = v * x[c];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            const int r = tid + j * kBlock;
-            if (r < nr) {
-                const int a = max(rps[r], c0), e = min(rps[r + 1], c1);
-                double s = acc[j];
-                for (int k = a; k < e; ++k) s += prod[k - c0];
-                acc[j] = s;
-            }
-        }
-        __syncthreads();
-    }
-    double *out = M.s_part + (int64_t)p * M.nrows + T.rb;
-#pragma unroll
-    for (int j = 0; j < RPT; ++j) {
-        const int r = tid + j * kBlock;
-        if (r < nr) out[r] = acc[j];
-    }
-}
-
-static void launch_slices(hipStream_t st, const CsrView &M, const double *x) {
-    if (M.s_ntiles > 0)
-        hipLaunchKernelGGL(k_spmv_slices, dim3(M.s_ntiles * M.nslices), dim3(kBlock), 0, st, M, x);
-}
-
-__global__ void k_gather_perm(int64_t nnz, const double *src, const int32_t *perm, double *dst);
-
-// ------------------------------------------------------------ tile kernel
-// SRC: 0 = no SpMV, 1 = CSR SpMV inside the tile, 2 = sum of the XCD-slice partials
-enum : int { SRC_NONE = 0, SRC_CSR = 1, SRC_SLICES = 2 };
-
-template <class Op, int SRC, bool CONES>
-__global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles, CsrView M,
-                                                 const double *__restrict__ vin,
-                                                 const int32_t *__restrict__ cone_idx, HugeInfo huge,
-                                                 Op op, double *__restrict__ raw_buf,
-                                                 double *__restrict__ partials) {
-    __shared__ double acc[kTileRows];
-    __shared__ double sh[4];
-    __shared__ double prod[SRC == SRC_CSR ? kStreamNnz : 1];
-    __shared__ int32_t rps[SRC == SRC_CSR ? kTileRows + 1 : 1];
-    const Tile t = tiles[blockIdx.x];
-    const int tid = threadIdx.x;
-    const int nrows = t.re - t.rb;
-    if constexpr (SRC == SRC_CSR) {
-        if (t.g == 0) tile_spmv_stream(M, vin, t, acc, prod, rps);
-        else tile_spmv(M, vin, t, acc, sh);
-    } else if constexpr (SRC == SRC_SLICES) {
-        if (tid < nrows) {
-            const double *pp = M.s_part + t.rb + tid;
-            double a = 0.0;
-            for (int p = 0; p < M.nslices; ++p) a += pp[(int64_t)p * M.nrows];
-            acc[tid] = a;
-        }
-    } else {
-        if (tid < nrows) acc[tid] = 0.0;
-    }
-    __syncthreads();
-    if (!CONES || t.kind == TILE_LINEAR) {
-        if (tid < nrows) {
-            const int i = t.rb + tid;
-            const double r = op.raw(i, acc[tid], false);
-            op.finish(i, r, op.lin_proj(i, r));
-        }
-        return;
-    }
-    if (t.kind == TILE_CONES) {
-        const int G2 = t.g2;
-        const int lane = tid & (G2 - 1), grp = tid / G2, ngrp = kBlock / G2;
-        for (int c = t.c0 + grp; c < t.c1; c += ngrp) {
-            const int s = cone_idx[c], e = cone_idx[c + 1];
-            double sq = 0.0, head = 0.0;
-            for (int i = s + lane; i < e; i += G2) {
-                const double r = op.raw(i, acc[i - t.rb], true);
-                acc[i - t.rb] = r;
-                if (i == s) head = r;
-                else sq += r * r;
-            }
-            for (int o = G2 >> 1; o > 0; o >>= 1) sq += __shfl_xor(sq, o, G2);
-            head = __shfl(head, 0, G2);
-            const ConeCase cc = cone_case(head, sqrt(sq));
-            for (int i = s + lane; i < e; i += G2) {
-                const double r = acc[i - t.rb];
-                op.finish(i, r, cone_apply(cc, i == s, r));
-            }
-        }
-        return;
-    }
-    // HUGE segment, phase 1: raw values and a partial ||xi||^2
-    double sq = 0.0;
-    if (tid < nrows) {
-        const int i = t.rb + tid;
-        const double r = op.raw(i, acc[tid], true);
-        raw_buf[i] = r;
-        if (i != huge.start[t.c0]) sq = r * r;
-    }
-    sq = block_sum(sq, sh);
-    if (tid == 0) partials[t.c1] = sq;
-}
-
-// HUGE segment, phase 2: the cone's ||xi|| from all partials, then the projection.
-template <class Op>
-__global__ __launch_bounds__(kBlock) void k_huge_finish(const Tile *__restrict__ tiles, HugeInfo huge,
-                                                        Op op, const double *__restrict__ raw_buf,
-                                                        const double *__restrict__ partials) {
-    __shared__ double sh[4];
-    const Tile t = tiles[blockIdx.x];
-    const int h = t.c0;
-    const int s = huge.start[h];
-    const int slot0 = huge.slot0[h], nslot = huge.nslot[h];
-    double sum = 0.0;
-    for (int k = threadIdx.x; k < nslot; k += kBlock) sum += partials[slot0 + k];
-    sum = block_sum(sum, sh);
-    const ConeCase cc = cone_case(raw_buf[s], sqrt(sum));
-    const int nrows = t.re - t.rb;
-    if ((int)threadIdx.x < nrows) {
-        const int i = t.rb + threadIdx.x;
-        const double r = raw_buf[i];
-        op.finish(i, r, cone_apply(cc, i == s, r));
-    }
-}
-
-// Fused SpMV + epilogue over a plan.  With XCD slicing the SpMV runs first as
-// k_spmv_slices and the tile kernel sums the slice partials.
-template <class Op, int SRC, bool CONES>
-static void run_tiles(hipStream_t st, const DevPlan &plan, const CsrView &M, const double *v,
-                      const int32_t *cone_idx, const Op &op, double *raw_buf, double *partials) {
-    if (plan.ntiles == 0) return;
-    if (SRC == SRC_CSR && M.nslices > 1) {
-        launch_slices(st, M, v);
-        hipLaunchKernelGGL((k_tile<Op, SRC_SLICES, CONES>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles,
-                           M, v, cone_idx, plan.huge, op, raw_buf, partials);
-    } else {
-        hipLaunchKernelGGL((k_tile<Op, SRC, CONES>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles, M, v,
-                           cone_idx, plan.huge, op, raw_buf, partials);
-    }
-    if (CONES) {
-        const int nh = plan.ntiles - plan.first_huge;
-        if (nh > 0)
-            hipLaunchKernelGGL((k_huge_finish<Op>), dim3(nh), dim3(kBlock), 0, st, plan.tiles + plan.first_huge,
-                               plan.huge, op, raw_buf, partials);
-    }
-}
-
-void launch_noq_primal(hipStream_t st, const DevPlan &plan, const CsrView &AT, const int32_t *var_idx,
-                       const NoQPrimalArgs &a, double *raw_buf, double *partials) {
-    run_tiles<OpNoQPrimal, SRC_CSR, true>(st, plan, AT, a.y, var_idx, OpNoQPrimal{a}, raw_buf, partials);
-}
-void launch_dual(hipStream_t st, const DevPlan &plan, const CsrView &A, const int32_t *con_idx,
-                 const DualArgs &a, double *raw_buf, double *partials) {
-    run_tiles<OpDual, SRC_CSR, true>(st, plan, A, a.v, con_idx, OpDual{a}, raw_buf, partials);
-}
-void launch_qp_primal(hipStream_t st, const DevPlan &plan, const int32_t *var_idx,
-                      const QPPrimalArgs &a, double *raw_buf, double *partials) {
-    CsrView none{nullptr, nullptr, nullptr, 0};
-    run_tiles<OpQPPrimal, SRC_NONE, true>(st, plan, none, nullptr, var_idx, OpQPPrimal{a}, raw_buf, partials);
-}
-void launch_w2(hipStream_t st, const DevPlan &plan, const CsrView &AT, const W2Args &a) {
-    run_tiles<OpW2, SRC_CSR, false>(st, plan, AT, a.ybar, nullptr, OpW2{a}, nullptr, nullptr);
-}
-void launch_spmv(hipStream_t st, const DevPlan &plan, const CsrView &M, const double *v, double *out,
-                 const double *diag, const double *v2) {
-    run_tiles<OpStore, SRC_CSR, false>(st, plan, M, v, nullptr, OpStore{out, diag, v2}, nullptr, nullptr);
-}
-void launch_spmv_tempv(hipStream_t st, const DevPlan &plan, const CsrView &M, const double *wbar,
-                       const TempvArgs &a) {
-    run_tiles<OpTempv, SRC_CSR, false>(st, plan, M, wbar, nullptr, OpTempv{a}, nullptr, nullptr);
-}
+// device helpers (nmax, jclamp, wave/block reductions) live in devutil.h
 
 // ---------------------------------------------------------------- reductions
 struct RedTerms {

@@ -464,6 +464,8 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
     if (P <= 1 || hm.ci.empty()) return false;
     const int32_t R = hm.nrows;
     const int64_t nnz = (int64_t)hm.ci.size();
+    for (int32_t i = 0; i < R; ++i)
+        if (hm.rp[i + 1] - hm.rp[i] > kSliceTileNnz) return false;  // per-slice offsets must fit uint16
     std::vector<int64_t> bound(P + 1);
     for (int p = 0; p <= P; ++p) bound[p] = (int64_t)hm.ncols * p / P;
     auto slice_of = [&](int32_t c) {
@@ -472,52 +474,62 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
         while (p > 0 && c < bound[p]) --p;
         return p;
     };
+    // slice tiles: <= kSliceRows rows, about P * kStreamNnz nonzeros, <= kSliceTileNnz
+    std::vector<SliceTile> tiles;
+    {
+        const int64_t cap = std::min<int64_t>((int64_t)P * kStreamNnz, kSliceTileNnz);
+        int32_t r = 0;
+        while (r < R) {
+            int32_t e = r + 1;
+            int64_t nz = hm.rp[e] - hm.rp[r];
+            while (e < R && e - r < kSliceRows && nz + (hm.rp[e + 1] - hm.rp[e]) <= cap) {
+                nz += hm.rp[e + 1] - hm.rp[e];
+                ++e;
+            }
+            tiles.push_back(SliceTile{r, e});
+            r = e;
+        }
+    }
+    const int32_t NT = (int32_t)tiles.size();
     std::vector<int32_t> cnt((size_t)P * R, 0);
     for (int32_t i = 0; i < R; ++i)
         for (int32_t k = hm.rp[i]; k < hm.rp[i + 1]; ++k) cnt[(size_t)slice_of(hm.ci[k]) * R + i]++;
-    std::vector<int32_t> srp((size_t)P * (R + 1));
+    // slice-major entry order: slice p, then tile, then row
+    std::vector<int32_t> base((size_t)P * (NT + 1));
+    std::vector<uint16_t> off((size_t)P * R);
+    std::vector<int64_t> rowstart((size_t)P * R);
     int64_t pos = 0;
     for (int p = 0; p < P; ++p) {
-        int32_t *r = srp.data() + (size_t)p * (R + 1);
-        for (int32_t i = 0; i < R; ++i) {
-            r[i] = (int32_t)pos;
-            pos += cnt[(size_t)p * R + i];
+        for (int32_t t = 0; t < NT; ++t) {
+            base[(size_t)p * (NT + 1) + t] = (int32_t)pos;
+            int64_t o = 0;
+            for (int32_t i = tiles[t].rb; i < tiles[t].re; ++i) {
+                off[(size_t)p * R + i] = (uint16_t)o;
+                rowstart[(size_t)p * R + i] = pos + o;
+                o += cnt[(size_t)p * R + i];
+            }
+            pos += o;
         }
-        r[R] = (int32_t)pos;
+        base[(size_t)p * (NT + 1) + NT] = (int32_t)pos;
     }
     std::vector<int32_t> scol((size_t)nnz), perm((size_t)nnz);
-    std::vector<int32_t> next((size_t)P * R);
-    for (int p = 0; p < P; ++p)
-        for (int32_t i = 0; i < R; ++i) next[(size_t)p * R + i] = srp[(size_t)p * (R + 1) + i];
     for (int32_t i = 0; i < R; ++i)
         for (int32_t k = hm.rp[i]; k < hm.rp[i + 1]; ++k) {
             const int p = slice_of(hm.ci[k]);
-            const int32_t d = next[(size_t)p * R + i]++;
+            const int64_t d = rowstart[(size_t)p * R + i]++;
             scol[d] = hm.ci[k];
             perm[d] = k;
         }
-    // slice tiles: <= kSliceRows rows and about P * kStreamNnz nonzeros
-    std::vector<SliceTile> tiles;
-    int32_t r = 0;
-    while (r < R) {
-        int32_t e = r + 1;
-        int64_t nz = hm.rp[e] - hm.rp[r];
-        while (e < R && e - r < kSliceRows && nz + (hm.rp[e + 1] - hm.rp[e]) <= (int64_t)P * kStreamNnz) {
-            nz += hm.rp[e + 1] - hm.rp[e];
-            ++e;
-        }
-        tiles.push_back(SliceTile{r, e});
-        r = e;
-    }
     M.nslices = P;
-    M.s_rowptr = h->upload<int32_t>(srp.data(), (int64_t)srp.size());
+    M.s_base = h->upload<int32_t>(base.data(), (int64_t)base.size());
+    M.s_off = h->upload<uint16_t>(off.data(), (int64_t)off.size());
     M.s_col = h->upload<int32_t>(scol.data(), nnz);
     int32_t *dperm = h->upload<int32_t>(perm.data(), nnz);
     double *sval = h->dalloc<double>(nnz);
     launch_gather_perm(h->st, nnz, dval, dperm, sval);
     M.s_val = sval;
-    M.s_tiles = h->upload<SliceTile>(tiles.data(), (int64_t)tiles.size());
-    M.s_ntiles = (int32_t)tiles.size();
+    M.s_tiles = h->upload<SliceTile>(tiles.data(), NT);
+    M.s_ntiles = NT;
     M.s_part = h->dalloc<double>((int64_t)P * R);
     h->sync();
     return true;
@@ -1096,6 +1108,35 @@ static void residuals_impl(hprsocp_handle *h, int64_t iter, hprsocp_residuals *r
 
 int hprsocp_compute_residuals(hprsocp_handle *h, int64_t iter, hprsocp_residuals *out) {
     ABI_TRY(h, { residuals_impl(h, iter, out); });
+    return 0;
+}
+
+int hprsocp_probe_copy(hprsocp_handle *h, int64_t bytes, double *ms) {
+    ABI_TRY(h, {
+        const int64_t n = std::max<int64_t>(bytes / (int64_t)sizeof(double), 1);
+        void *a = nullptr, *b = nullptr;
+        if (hipMalloc(&a, (size_t)n * sizeof(double)) != hipSuccess) throw Err{HPRSOCP_ERR_ALLOC, "probe alloc"};
+        if (hipMalloc(&b, (size_t)n * sizeof(double)) != hipSuccess) {
+            (void)hipFree(a);
+            throw Err{HPRSOCP_ERR_ALLOC, "probe alloc"};
+        }
+        launch_fill(h->st, n, 1.0, (double *)a);
+        h->sync();
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, h->st));
+        launch_copy(h->st, n, (const double *)a, (double *)b);
+        HIPCHK(hipEventRecord(e1, h->st));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        *ms = t;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(a);
+        (void)hipFree(b);
+    });
     return 0;
 }
 

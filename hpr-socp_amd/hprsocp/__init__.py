@@ -393,6 +393,15 @@ class Solver:
                                               _abi.ptr_f64(z), _abi.ptr_f64(w)))
         return HPRSOCP_results.from_c(r, x, y, z, w)
 
+    def probe_copy(self, nbytes: int) -> float:
+        """Plain device copy of `nbytes` (counter calibration); returns ms."""
+        ms = C.c_double()
+        fn = self.lib.hprsocp_probe_copy
+        fn.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_double)]
+        fn.restype = C.c_int
+        self._check(fn(self.h, int(nbytes), C.byref(ms)))
+        return ms.value
+
     def time_iterations(self, iters: int, per_kernel: bool = False):
         ms = C.c_double()
         kern = (C.c_double * 6)()

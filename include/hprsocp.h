@@ -272,6 +272,12 @@ int hprsocp_spmv(hprsocp_handle *h, int32_t trans, const double *in, double *out
 int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_iter,
                             double *ms_kernel);
 
+/* Copies `bytes` between two fresh device buffers with the backend's plain copy
+ * kernel (one launch on a cold source) and returns its HIP-event time.  Used to
+ * calibrate hardware byte counters (FETCH_SIZE / WRITE_SIZE) against a kernel
+ * of known traffic.  (No counterpart in the reference; measurement hook.) */
+int hprsocp_probe_copy(hprsocp_handle *h, int64_t bytes, double *ms);
+
 #ifdef __cplusplus
 }
 #endif
