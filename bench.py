@@ -128,6 +128,7 @@ def main():
     ap.add_argument("--solve-time", type=float, default=90.0, help="time cap of the time-to-1e-6 solve")
     ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_refine"])
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -141,7 +142,8 @@ def main():
     model, wl = make_workload(args.config, args.scale)
     m, n = model.A.shape
     log(f"workload {wl} generated in {time.time() - t0:.1f}s")
-    p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, time_limit=args.solve_time)
+    p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, time_limit=args.solve_time,
+                                   precision=args.precision)
     t0 = time.time()
     g = hprsocp.Solver(model, p)
     log(f"setup (transfer + scaling + plans) {time.time() - t0:.2f}s")
@@ -155,13 +157,14 @@ def main():
     ms, _ = g.time_iterations(args.steps)
     _, kern = g.time_iterations(max(args.steps // 4, 10), per_kernel=True)
     log(f"{ms:.4f} ms/iter -> {1000.0 / ms:.1f} iters/s; per-group ms {kern}")
-    bytes_by, order = algorithmic_bytes(model)
+    vbytes = 4 if args.precision == "fp32_refine" else 8
+    bytes_by, order = algorithmic_bytes(model, vbytes)
     groups = [(name, kern[i]) for i, name in enumerate(order)]
     dom, dom_ms = max(groups, key=lambda t: t[1])
     dom_bytes = bytes_by[dom]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     it_bytes = sum(bytes_by.values())
-    traffic = MEASURED_TRAFFIC.get((args.config, "fp64", dom)) if args.scale == 1.0 else None
+    traffic = MEASURED_TRAFFIC.get((args.config, args.precision, dom)) if args.scale == 1.0 else None
     g.close()
 
     solve = None
@@ -174,7 +177,7 @@ def main():
         solve = {"status": r.status, "iter": r.iter, "time_s": r.time, "kkt": r.residuals,
                  "iter_to_1e-6": r.iter_6 if r.residuals < 1e-6 or r.iter_6 < r.iter else None,
                  "time_to_1e-6_s": r.time_6 if r.residuals < 1e-6 or r.iter_6 < r.iter else None,
-                 "primal_obj": r.primal_obj, "restarts": r.restarts}
+                 "primal_obj": r.primal_obj, "restarts": r.restarts, "iter_fp32": r.iter_fp32}
         log(f"solve: {solve} (wall {time.time() - t0:.1f}s)")
 
     if args.cpu_iters > 0:
@@ -194,10 +197,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if args.precision == "fp32_refine" else "f64",
         "data": "synthetic (seeded feasibility construction)",
         "config": {"workload": wl, "config": args.config, "scale": args.scale, "m": m, "n": n,
-                   "nnz_A": int(model.A.nnz), "precision": "fp64",
+                   "nnz_A": int(model.A.nnz), "precision": args.precision,
                    "algorithmic_MB_per_iter": round(it_bytes / 1e6, 2),
                    "iteration_GBps": round(it_bytes / (ms * 1e-3) / 1e9, 1)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),

@@ -30,6 +30,9 @@ void launch_axpby(hipStream_t st, int64_t n, double a, const double *x, double b
 void launch_copy(hipStream_t st, int64_t n, const double *src, double *dst);
 void launch_fill(hipStream_t st, int64_t n, double v, double *dst);
 void launch_mul(hipStream_t st, int64_t n, const double *a, const double *b, double *out);  // out = a.*b
+// precision conversion of the FP32 mirrors (round to nearest / exact widening)
+void launch_demote(hipStream_t st, int64_t n, const double *src, float *dst);
+void launch_promote(hipStream_t st, int64_t n, const float *src, double *dst);
 
 // ---- residual kernels (compute_Rd!/compute_Rp!, kernels.jl:118-292) ---------
 struct ResidNArgs {

@@ -103,6 +103,15 @@ __global__ void k_copy(int64_t n, const double *s, double *d) { GRID_STRIDE(i, n
 __global__ void k_fill(int64_t n, double v, double *d) { GRID_STRIDE(i, n) d[i] = v; }
 __global__ void k_mul(int64_t n, const double *a, const double *b, double *o) { GRID_STRIDE(i, n) o[i] = a[i] * b[i]; }
 
+__global__ void k_demote(int64_t n, const double *s, float *d) { GRID_STRIDE(i, n) d[i] = (float)s[i]; }
+__global__ void k_promote(int64_t n, const float *s, double *d) { GRID_STRIDE(i, n) d[i] = (double)s[i]; }
+
+void launch_demote(hipStream_t st, int64_t n, const double *src, float *dst) {
+    if (n > 0) hipLaunchKernelGGL(k_demote, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, src, dst);
+}
+void launch_promote(hipStream_t st, int64_t n, const float *src, double *dst) {
+    if (n > 0) hipLaunchKernelGGL(k_promote, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, src, dst);
+}
 void launch_axpby(hipStream_t st, int64_t n, double a, const double *x, double b, const double *y, double *out) {
     if (n > 0) hipLaunchKernelGGL(k_axpby, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, a, x, b, y, out);
 }

@@ -130,6 +130,28 @@ void canonicalize(HostCsr &a) {
 }  // namespace
 
 // ------------------------------------------------------------------ handle
+// Vectors touched by the update kernels.  STATE vectors carry the iteration
+// (and its check outputs) and travel both ways between FP64 and the FP32
+// mirrors; DATA vectors (problem data and sigma-dependent factors) are only
+// demoted.
+#define HPR_STATE_N(X) X(x) X(x_hat) X(x_bar) X(z_bar) X(dx) X(last_x) X(w) X(w_bar) X(last_w) X(dw) \
+    X(aty) X(aty_bar) X(last_aty)
+#define HPR_STATE_M(X) X(y) X(y_bar) X(dy) X(last_y) X(s)
+#define HPR_DATA_N(X) X(c) X(l) X(u) X(diag_Q) X(fact1) X(fact2) X(fact)
+#define HPR_DATA_M(X) X(AL) X(AU) X(rhs)
+#define HPR_SCRATCH_N(X) X(atdy) X(qw) X(tempv) X(raw_n)
+#define HPR_SCRATCH_M(X) X(raw_m)
+
+// Matrices and vectors one precision of the update runs on.
+template <typename T>
+struct IterView {
+    CsrT<T> A{}, AT{}, Q{}, F{}, FT{};
+    T *Fdiag = nullptr, *Ftmp = nullptr;
+#define HPR_FIELD(f) T *f = nullptr;
+    HPR_STATE_N(HPR_FIELD) HPR_STATE_M(HPR_FIELD) HPR_DATA_N(HPR_FIELD) HPR_DATA_M(HPR_FIELD)
+    HPR_SCRATCH_N(HPR_FIELD) HPR_SCRATCH_M(HPR_FIELD)
+#undef HPR_FIELD
+};
 struct hprsocp_handle {
     std::string err;
     hprsocp_params params{};
@@ -172,6 +194,12 @@ struct hprsocp_handle {
     bool lambda_set = false;
     double power_time = 0, setup_time = 0;
     int to_check = 1;
+    // FP32 mirrors of the iteration (precision FP32_REFINE)
+    bool fp32 = false;         // mirrors allocated
+    bool fp32_active = false;  // updates run on the mirrors
+    bool fp32_dirty = true;    // FP64 state changed since the last demotion
+    int64_t nnzF = 0;
+    IterView<float> v32{};
     // timing (profiled iterations)
     bool profile = false;
     std::vector<hipEvent_t> ev;
@@ -613,6 +641,97 @@ int validate_csr(std::string &err, const char *name, int32_t nr, int32_t nc, int
     return 0;
 }
 
+// ------------------------------------------------------------- precision views
+IterView<double> view64(hprsocp_handle *h) {
+    IterView<double> v;
+    v.A = h->A;
+    v.AT = h->AT;
+    v.Q = h->Q;
+    v.F = h->F;
+    v.FT = h->FT;
+    v.Fdiag = h->Fdiag;
+    v.Ftmp = h->Ftmp;
+#define HPR_COPY(f) v.f = h->f;
+    HPR_STATE_N(HPR_COPY) HPR_STATE_M(HPR_COPY) HPR_DATA_N(HPR_COPY) HPR_DATA_M(HPR_COPY)
+    HPR_SCRATCH_N(HPR_COPY) HPR_SCRATCH_M(HPR_COPY)
+#undef HPR_COPY
+    return v;
+}
+
+// FP32 copy of a (scaled) operator: values only; index arrays and the slice
+// layout are shared with the FP64 operator.
+CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz) {
+    CsrT<float> f;
+    f.rowptr = M.rowptr;
+    f.col = M.col;
+    f.nrows = M.nrows;
+    float *v = h->dalloc<float>(nnz);
+    launch_demote(h->st, nnz, M.val, v);
+    f.val = v;
+    f.nslices = M.nslices;
+    if (M.nslices > 1) {
+        f.s_base = M.s_base;
+        f.s_off = M.s_off;
+        f.s_col = M.s_col;
+        f.s_tiles = M.s_tiles;
+        f.s_ntiles = M.s_ntiles;
+        float *sv = h->dalloc<float>(nnz);
+        launch_demote(h->st, nnz, M.s_val, sv);
+        f.s_val = sv;
+        f.s_part = h->dalloc<float>((int64_t)M.nslices * M.nrows);
+    }
+    return f;
+}
+
+// Allocates the FP32 mirrors; called once the FP64 operators are final.
+void setup_fp32(hprsocp_handle *h) {
+    IterView<float> &v = h->v32;
+    const int64_t n = h->n, m = h->m;
+    v.A = csr_to_f32(h, h->A, h->nnzA);
+    v.AT = csr_to_f32(h, h->AT, h->nnzA);
+    if (h->nnzQ > 0) v.Q = csr_to_f32(h, h->Q, h->nnzQ);
+    if (h->q_kind == HPRSOCP_Q_FACTORED) {
+        v.F = csr_to_f32(h, h->F, h->nnzF);
+        v.FT = csr_to_f32(h, h->FT, h->nnzF);
+        v.Ftmp = h->dalloc<float>(h->Frows);
+        if (h->Fdiag) v.Fdiag = h->dalloc<float>(n);
+    }
+#define HPR_ALLOC_N(f) v.f = h->dalloc<float>(n);
+#define HPR_ALLOC_M(f) v.f = h->dalloc<float>(m);
+    HPR_STATE_N(HPR_ALLOC_N) HPR_DATA_N(HPR_ALLOC_N) HPR_SCRATCH_N(HPR_ALLOC_N)
+    HPR_STATE_M(HPR_ALLOC_M) HPR_DATA_M(HPR_ALLOC_M) HPR_SCRATCH_M(HPR_ALLOC_M)
+#undef HPR_ALLOC_N
+#undef HPR_ALLOC_M
+    h->fp32 = true;
+    h->fp32_dirty = true;
+}
+
+// FP64 state and data -> FP32 mirrors (round to nearest).
+void demote_state(hprsocp_handle *h) {
+    IterView<float> &v = h->v32;
+    const int64_t n = h->n, m = h->m;
+    hipStream_t st = h->st;
+#define HPR_DEM_N(f) launch_demote(st, n, h->f, v.f);
+#define HPR_DEM_M(f) launch_demote(st, m, h->f, v.f);
+    HPR_STATE_N(HPR_DEM_N) HPR_DATA_N(HPR_DEM_N) HPR_STATE_M(HPR_DEM_M) HPR_DATA_M(HPR_DEM_M)
+#undef HPR_DEM_N
+#undef HPR_DEM_M
+    if (v.Fdiag) launch_demote(st, n, h->Fdiag, v.Fdiag);
+    h->fp32_dirty = false;
+}
+
+// FP32 state -> FP64 (exact widening), for the FP64 checks, restarts and reads.
+void promote_state(hprsocp_handle *h) {
+    IterView<float> &v = h->v32;
+    const int64_t n = h->n, m = h->m;
+    hipStream_t st = h->st;
+#define HPR_PRO_N(f) launch_promote(st, n, v.f, h->f);
+#define HPR_PRO_M(f) launch_promote(st, m, v.f, h->f);
+    HPR_STATE_N(HPR_PRO_N) HPR_STATE_M(HPR_PRO_M)
+#undef HPR_PRO_N
+#undef HPR_PRO_M
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -638,7 +757,8 @@ void hprsocp_default_params(hprsocp_params *p) {
     p->verbose = 1;
     p->use_gpu = 1;
     p->precision = HPRSOCP_PREC_FP64;
-    p->refine_every = 1;
+    p->refine_tol = 1e-6;
+    p->fp32_stall_checks = 40;
     p->use_graph = 1;
 }
 
@@ -653,7 +773,10 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     hprsocp_params &P = h->params;
     if (!P.use_gpu) throw Err{HPRSOCP_ERR_UNSUPPORTED, "use_gpu = false: this backend runs the HPR iteration on the GPU only"};
     if (P.auto_save) throw Err{HPRSOCP_ERR_UNSUPPORTED, "auto_save (HDF5) is not provided by this backend"};
-    if (P.precision != HPRSOCP_PREC_FP64) throw Err{HPRSOCP_ERR_UNSUPPORTED, "precision mode not available yet"};
+    if (P.precision != HPRSOCP_PREC_FP64 && P.precision != HPRSOCP_PREC_FP32_REFINE)
+        throw Err{HPRSOCP_ERR_INVALID, "unknown precision mode"};
+    if (P.precision == HPRSOCP_PREC_FP32_REFINE && (!(P.refine_tol >= 0.0) || P.fp32_stall_checks <= 0))
+        throw Err{HPRSOCP_ERR_INVALID, "FP32 mode needs refine_tol >= 0 and fp32_stall_checks > 0"};
     if (P.sigma != -1.0 && !(P.sigma > 0.0)) {
         char b[160];
         snprintf(b, sizeof b, "Invalid sigma value: %g. It should be a positive number or -1 for automatic.", P.sigma);
@@ -807,6 +930,7 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
         F.v.assign(p->F_val, p->F_val + p->F_nnz);
         FT = transpose(F);
         h->Frows = p->F_rows;
+        h->nnzF = p->F_nnz;
         int32_t *F_rp = h->upload<int32_t>(F.rp.data(), F.nrows + 1);
         int32_t *F_ci = h->upload<int32_t>(F.ci.data(), (int64_t)F.ci.size());
         double *F_v = h->upload<double>(F.v.data(), (int64_t)F.v.size());
@@ -888,6 +1012,10 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
         h->sigma = P.sigma;
     }
     if (h->Q_is_diag) update_q_factors(h);
+    if (P.precision == HPRSOCP_PREC_FP32_REFINE) {
+        setup_fp32(h);
+        h->fp32_active = true;
+    }
     h->to_check = 1;
     h->sync();
     h->setup_time = now_s() - t0;
@@ -983,31 +1111,46 @@ int hprsocp_set_initial(hprsocp_handle *h, const double *x0, const double *y0) {
                 launch_init_s(st, m, h->ax, h->y, h->AL, h->AU, h->lambda_A * h->sigma, h->s);
             }
         }
+        h->fp32_dirty = true;
         h->sync();
     });
     return 0;
 }
 
 // main_update_gpu! (algorithm.jl:2284-2309)
-static void do_update(hprsocp_handle *h, int64_t inner, int full) {
+extern "C++" {
+// Q v on the update's precision (factored or general CSR Q).
+template <typename T>
+static void qmap_t(hprsocp_handle *h, IterView<T> &V, const T *v, T *out) {
+    if (h->q_kind == HPRSOCP_Q_FACTORED) {
+        launch_spmv<T>(h->st, h->plan_F, V.F, v, V.Ftmp);
+        launch_spmv<T>(h->st, h->plan_FT, V.FT, V.Ftmp, out, V.Fdiag, v);
+    } else {
+        launch_spmv<T>(h->st, h->plan_Q, V.Q, v, out);
+    }
+}
+
+// One HPR iteration (main_update_gpu!, algorithm.jl:2284-2309) on the
+// matrices and vectors of V; scalars are rounded to T.
+template <typename T>
+static void do_update_t(hprsocp_handle *h, IterView<T> &V, int64_t inner, int full) {
     hipStream_t st = h->st;
-    const double h1 = 1.0 / ((double)inner + 2.0), h2 = 1.0 - h1;
-    const int32_t m = h->m, n = h->n;
-    h->to_check = full;
+    const double h1d = 1.0 / ((double)inner + 2.0);
+    const T h1 = (T)h1d, h2 = (T)(1.0 - h1d), sig = (T)h->sigma;
+    const int32_t m = h->m;
     auto ev = [&](int k) {
         if (h->profile) HIPCHK(hipEventRecord(h->ev[k], st));
     };
     if (!h->has_quad) {
         ev(0);
-        NoQPrimalArgs a{h->y, h->x, h->x_hat, h->x_bar, h->z_bar, h->dx, h->last_x, h->c, h->l, h->u,
-                        h->sigma, h1, h2, full};
-        launch_noq_primal(st, h->plan_AT, h->AT, h->var_idx, a, h->raw_n, h->part_n);
+        NoQPrimalArgsT<T> a{V.y, V.x, V.x_hat, V.x_bar, V.z_bar, V.dx, V.last_x, V.c, V.l, V.u, sig, h1, h2, full};
+        launch_noq_primal<T>(st, h->plan_AT, V.AT, h->var_idx, a, V.raw_n, h->part_n);
         ev(1);
         if (m > 0) {
             const double f1 = h->lambda_A * h->sigma;
-            DualArgs d{h->x_hat, h->y, h->y_bar, h->s, h->dy, h->last_y, h->AL, h->AU, h->rhs,
-                       f1, 1.0 / f1, h1, h2, full, 0};
-            launch_dual(st, h->plan_A, h->A, h->con_idx, d, h->raw_m, h->part_m);
+            DualArgsT<T> d{V.x_hat, V.y, V.y_bar, V.s, V.dy, V.last_y, V.AL, V.AU, V.rhs,
+                           (T)f1, (T)(1.0 / f1), h1, h2, full, 0};
+            launch_dual<T>(st, h->plan_A, V.A, h->con_idx, d, V.raw_m, h->part_m);
         }
         ev(2);
         return;
@@ -1015,38 +1158,53 @@ static void do_update(hprsocp_handle *h, int64_t inner, int full) {
     const double f2s = 1.0 / (1.0 + h->sigma * h->lambda_Q), f1s = 1.0 - f2s;
     const int sdiag = h->q_struct_diag && h->q_kind == HPRSOCP_Q_CSR;
     ev(0);
-    if (!sdiag) qmap(h, h->w, h->qw);
+    if (!sdiag) qmap_t<T>(h, V, V.w, V.qw);
     ev(1);
-    QPPrimalArgs a{h->x, h->x_hat, h->x_bar, h->z_bar, h->dx, h->w_bar, h->tempv, h->w, h->last_x, h->c,
-                   h->l, h->u, h->aty, h->qw, h->diag_Q, h->fact1, h->fact2, h->sigma, h1, h2, f1s, f2s,
-                   full, sdiag, h->Q_is_diag};
-    launch_qp_primal(st, h->plan_AT, h->var_idx, a, h->raw_n, h->part_n);
+    QPPrimalArgsT<T> a{V.x, V.x_hat, V.x_bar, V.z_bar, V.dx, V.w_bar, V.tempv, V.w, V.last_x, V.c,
+                       V.l, V.u, V.aty, V.qw, V.diag_Q, V.fact1, V.fact2, sig, h1, h2, (T)f1s, (T)f2s,
+                       full, sdiag, h->Q_is_diag};
+    launch_qp_primal<T>(st, h->plan_AT, h->var_idx, a, V.raw_n, h->part_n);
     ev(2);
     if (m > 0) {
         if (!sdiag) {
             if (h->q_kind == HPRSOCP_Q_FACTORED) {
-                launch_spmv(st, h->plan_F, h->F, h->w_bar, h->Ftmp);
-                TempvArgs t{h->x_hat, h->qw, h->tempv, h->Fdiag, h->w_bar, h->sigma};
-                launch_spmv_tempv(st, h->plan_FT, h->FT, h->Ftmp, t);
+                launch_spmv<T>(st, h->plan_F, V.F, V.w_bar, V.Ftmp);
+                TempvArgsT<T> t{V.x_hat, V.qw, V.tempv, V.Fdiag, V.w_bar, sig};
+                launch_spmv_tempv<T>(st, h->plan_FT, V.FT, V.Ftmp, t);
             } else {
-                TempvArgs t{h->x_hat, h->qw, h->tempv, nullptr, nullptr, h->sigma};
-                launch_spmv_tempv(st, h->plan_Q, h->Q, h->w_bar, t);
+                TempvArgsT<T> t{V.x_hat, V.qw, V.tempv, nullptr, nullptr, sig};
+                launch_spmv_tempv<T>(st, h->plan_Q, V.Q, V.w_bar, t);
             }
         }
         ev(3);
         const double f1 = h->lambda_A * h->sigma;
-        DualArgs d{h->tempv, h->y, h->y_bar, h->s, h->dy, h->last_y, h->AL, h->AU, h->rhs,
-                   f1, 1.0 / f1, h1, h2, full, 1};
-        launch_dual(st, h->plan_A, h->A, h->con_idx, d, h->raw_m, h->part_m);
+        DualArgsT<T> d{V.tempv, V.y, V.y_bar, V.s, V.dy, V.last_y, V.AL, V.AU, V.rhs,
+                       (T)f1, (T)(1.0 / f1), h1, h2, full, 1};
+        launch_dual<T>(st, h->plan_A, V.A, h->con_idx, d, V.raw_m, h->part_m);
     } else {
         ev(3);
     }
     ev(4);
-    W2Args w2{h->y_bar, h->w, h->w_bar, h->aty, h->dw, h->atdy, h->aty_bar, h->last_w, h->last_aty, h->fact,
-              h->sigma / (1.0 + h->sigma * h->lambda_Q), h1, h2, full, h->Q_is_diag};
-    launch_w2(st, h->plan_AT, h->AT, w2);
+    W2ArgsT<T> w2{V.y_bar, V.w, V.w_bar, V.aty, V.dw, V.atdy, V.aty_bar, V.last_w, V.last_aty, V.fact,
+                  (T)(h->sigma / (1.0 + h->sigma * h->lambda_Q)), h1, h2, full, h->Q_is_diag};
+    launch_w2<T>(st, h->plan_AT, V.AT, w2);
     ev(5);
-    (void)n;
+}
+}  // extern "C++"
+
+// One iteration in the handle's current precision.  In FP32 the mirrors are
+// refreshed from FP64 first if the FP64 state changed, and the FP64 state is
+// refreshed from the mirrors after a full (check) iteration or when `sync64`.
+static void do_update(hprsocp_handle *h, int64_t inner, int full, bool sync64 = false) {
+    h->to_check = full;
+    if (h->fp32_active) {
+        if (h->fp32_dirty) demote_state(h);
+        do_update_t<float>(h, h->v32, inner, full);
+        if (full || sync64) promote_state(h);
+    } else {
+        IterView<double> V = view64(h);
+        do_update_t<double>(h, V, inner, full);
+    }
 }
 
 int hprsocp_update(hprsocp_handle *h, int64_t inner, int32_t to_check) {
@@ -1055,7 +1213,7 @@ int hprsocp_update(hprsocp_handle *h, int64_t inner, int32_t to_check) {
         return HPRSOCP_ERR_STATE;
     }
     ABI_TRY(h, {
-        do_update(h, inner, to_check ? 1 : 0);
+        do_update(h, inner, to_check ? 1 : 0, true);
         HIPCHK(hipGetLastError());
     });
     return 0;
@@ -1485,6 +1643,13 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
     // deferred M-norms: slot 0 -> last_gap candidate, slot 1 -> current_gap
     MnormCtx mn[2];
     h->to_check = 1;
+    // FP32 bulk: windows of FP32 iterations, FP64 checks; FP64 iterations take
+    // over once the KKT residual reaches refine_tol or stops improving
+    h->fp32_active = h->fp32;
+    h->fp32_dirty = true;
+    double best32 = INFINITY;
+    int stall32 = 0;
+    int64_t iter32 = 0, restarts32 = 0;
     for (int64_t iter = 0; iter <= p.max_iter; ++iter) {
         const bool print_yes = should_print(p, iter, t_start);
         if (iter % check_iter == 0 || print_yes) residuals_impl(h, iter, &rs);  // synchronises the stream
@@ -1504,9 +1669,30 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
         if (rs.kkt < p.stoptol) status = HPRSOCP_STATUS_OPTIMAL;
         else if (iter == p.max_iter) status = HPRSOCP_STATUS_MAX_ITER;
         else if (now_s() - t_start > p.time_limit) status = HPRSOCP_STATUS_TIME_LIMIT;
+        if (h->fp32_active && status == HPRSOCP_STATUS_CONTINUE && iter > 0 && iter % check_iter == 0) {
+            // the FP64 state holds this check's iterate (the last update was a full one)
+            // stalled: no 1% improvement of the best KKT over fp32_stall_checks
+            // checks that also saw two restarts (a long restart cycle alone is
+            // not a stall)
+            if (rs.kkt < 0.99 * best32) {
+                best32 = rs.kkt;
+                stall32 = 0;
+                restarts32 = r.times;
+            } else {
+                ++stall32;
+            }
+            const bool stalled = stall32 >= p.fp32_stall_checks && r.times - restarts32 >= 2;
+            if (rs.kkt < p.refine_tol || stalled) {
+                h->fp32_active = false;
+                if (p.verbose)
+                    printf("precision: FP64 iterations from iteration %lld (KKT %.3e, %s)\n", (long long)iter, rs.kkt,
+                           rs.kkt < p.refine_tol ? "refine_tol reached" : "FP32 stalled");
+            }
+        }
         check_restart(r, iter, check_iter, h->sigma, rs.kkt);
         update_sigma(h, r, rs);
         do_restart(h, r);
+        if (r.restart_flag > 0) h->fp32_dirty = true;
         if ((print_yes || status != HPRSOCP_STATUS_CONTINUE) && p.verbose)
             printf("%5lld    %3.2e    %3.2e    %+7.6e    %+7.6e    %3.2e    %3.2e    %6.2f\n", (long long)iter,
                    rs.err_Rp, rs.err_Rd, rs.primal_obj, rs.dual_obj, rs.rel_gap, h->sigma, now_s() - t_start);
@@ -1545,6 +1731,8 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
             res->lambda_max_A = h->lambda_A;
             res->lambda_max_Q = h->lambda_Q;
             res->restarts = (int32_t)r.times;
+            res->iter_fp32 = iter32;
+            h->fp32_active = h->fp32;
             if (p.verbose) {
                 printf("SOLUTION SUMMARY: status %d, iterations %lld, time %.3f s, primal %.12e, KKT %.3e\n",
                        status, (long long)iter, tt, rs.primal_obj, rs.kkt);
@@ -1557,6 +1745,7 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
         if (p.print_frequency == -1) tc = tc || rem_zero(nx, print_step(nx));
         else tc = tc || (nx % p.print_frequency == 0);
         // perform_iteration_step!, algorithm.jl:2722-2744
+        if (h->fp32_active) ++iter32;
         do_update(h, r.inner, tc ? 1 : 0);
         const bool need_last = r.restart_flag > 0, need_cur = ((iter + 1) % check_iter) == 0;
         if (need_last) {
@@ -1714,6 +1903,7 @@ int hprsocp_set_vector(hprsocp_handle *h, int32_t id, const double *in) {
     }
     ABI_TRY(h, {
         if (len > 0) HIPCHK(hipMemcpyAsync(p, in, sizeof(double) * len, hipMemcpyHostToDevice, h->st));
+        h->fp32_dirty = true;
         h->sync();
     });
     return 0;
@@ -1741,6 +1931,7 @@ int hprsocp_set_scalars(hprsocp_handle *h, double sigma, double la, double lq) {
         if (lq >= 0) h->lambda_Q = lq;
         h->lambda_set = true;
         if (h->Q_is_diag) update_q_factors(h);
+        h->fp32_dirty = true;
         h->sync();
     });
     return 0;
@@ -1813,6 +2004,8 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
             h->profile = false;
             for (int j = 0; j < 6; ++j) ms_kernel[j] = iters > 0 ? acc[j] / iters : 0.0;
         }
+        if (h->fp32_active) promote_state(h);
+        h->sync();
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     });

@@ -80,7 +80,8 @@ class HPRSOCP_parameters:
     use_gpu: bool = True
     # extensions of this backend
     precision: str = "fp64"          # "fp64" | "fp32_refine"
-    refine_every: int = 1
+    refine_tol: float = 1e-6         # FP32 mode: KKT at which FP64 iterations take over
+    fp32_stall_checks: int = 40      # FP32 mode: stalled checks before FP64 takes over
     use_graph: bool = True
 
     def to_c(self, keep: list) -> _abi.Params:
@@ -119,7 +120,8 @@ class HPRSOCP_parameters:
         p.verbose = int(bool(self.verbose))
         p.use_gpu = int(bool(self.use_gpu))
         p.precision = {"fp64": _abi.PREC_FP64, "fp32_refine": _abi.PREC_FP32_REFINE}[self.precision]
-        p.refine_every = int(self.refine_every)
+        p.refine_tol = float(self.refine_tol)
+        p.fp32_stall_checks = int(self.fp32_stall_checks)
         p.use_graph = int(bool(self.use_graph))
         return p
 
@@ -150,6 +152,7 @@ class HPRSOCP_results:
     lambda_max_A: float = 0.0
     lambda_max_Q: float = 0.0
     restarts: int = 0
+    iter_fp32: int = 0
 
     @classmethod
     def from_c(cls, r: _abi.Result, x, y, z, w) -> "HPRSOCP_results":
@@ -158,7 +161,7 @@ class HPRSOCP_results:
                    residuals=r.residuals, gap=r.gap, status=_abi.STATUS_NAMES[r.status],
                    x=x, y=y, z=z, w=w, dual_obj=r.dual_obj, err_Rp=r.err_Rp, err_Rd=r.err_Rd,
                    setup_time=r.setup_time, sigma=r.sigma, lambda_max_A=r.lambda_max_A,
-                   lambda_max_Q=r.lambda_max_Q, restarts=r.restarts)
+                   lambda_max_Q=r.lambda_max_Q, restarts=r.restarts, iter_fp32=r.iter_fp32)
 
 
 # --------------------------------------------------------------------- model

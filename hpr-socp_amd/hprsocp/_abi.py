@@ -75,8 +75,8 @@ class Params(C.Structure):
         ("soc_block_scaling_strategy", C.c_int32),
         ("initial_x", _f64p), ("initial_y", _f64p),
         ("auto_save", C.c_int32), ("verbose", C.c_int32), ("use_gpu", C.c_int32),
-        ("precision", C.c_int32), ("refine_every", C.c_int32), ("use_graph", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("precision", C.c_int32), ("use_graph", C.c_int32), ("refine_tol", C.c_double),
+        ("fp32_stall_checks", C.c_int32), ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -90,7 +90,7 @@ class Result(C.Structure):
         ("err_Rp", C.c_double), ("err_Rp_linear", C.c_double), ("err_Rp_soc", C.c_double),
         ("err_Rd", C.c_double),
         ("sigma", C.c_double), ("lambda_max_A", C.c_double), ("lambda_max_Q", C.c_double),
-        ("status", C.c_int32), ("restarts", C.c_int32),
+        ("status", C.c_int32), ("restarts", C.c_int32), ("iter_fp32", C.c_int64),
     ]
 
 

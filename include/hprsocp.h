@@ -72,12 +72,16 @@ typedef enum {
     HPRSOCP_SOC_PHASE_TAPER = 20
 } hprsocp_soc_strategy;
 
-/* Arithmetic of the iteration (north-star precision forms). */
+/* Arithmetic of the iteration. */
 typedef enum {
     HPRSOCP_PREC_FP64 = 0,         /* plain FP64 method (reference numerics)          */
-    HPRSOCP_PREC_FP32_REFINE = 1   /* FP32 bulk (SpMV, projections, averaging) with
-                                      FP64 residuals, scale computation, near-boundary
-                                      cone test and periodic FP64 refinement          */
+    HPRSOCP_PREC_FP32_REFINE = 1   /* FP32 bulk: SpMV, projections and Halpern
+                                      averaging in FP32 (cone norms t^2 - |xi|^2 in
+                                      FP64); every check window ends in FP64:
+                                      residuals, gap, M-norm, restart and sigma run
+                                      on the FP64 copy of the state; FP64 iterations
+                                      take over once the KKT residual reaches
+                                      refine_tol or stalls in FP32                    */
 } hprsocp_precision;
 
 /* ------------------------------------------------------------ problem */
@@ -146,11 +150,14 @@ typedef struct {
     int32_t use_gpu;           /* 1 (0 is rejected: this backend is the GPU)  */
     /* extensions */
     int32_t precision;         /* hprsocp_precision, default FP64             */
-    int32_t refine_every;      /* FP32 mode: check windows between FP64
-                                  refinement passes (default 1)               */
     int32_t use_graph;         /* capture the non-check iterations of a
                                   check window in one hipGraph (default 1)    */
-    int32_t reserved[8];
+    double refine_tol;         /* FP32 mode: KKT below which the iteration
+                                  continues in FP64 (default 1e-6)            */
+    int32_t fp32_stall_checks; /* FP32 mode: checks without a 1% improvement
+                                  of the best KKT (and with two restarts or
+                                  more) before FP64 takes over (default 40)   */
+    int32_t reserved[7];
 } hprsocp_params;
 
 /* ------------------------------------------------------------ results */
@@ -165,6 +172,7 @@ typedef struct {
     double sigma, lambda_max_A, lambda_max_Q;
     int32_t status;            /* hprsocp_status                              */
     int32_t restarts;
+    int64_t iter_fp32;         /* iterations run in FP32 (precision FP32_REFINE) */
 } hprsocp_result;
 
 /* Residual block, HPRSOCP_residuals (reference src/structs.jl:281-294). */

@@ -212,8 +212,9 @@ void oracle_default_params(hprsocp_params *p) {
     p->verbose = 1;
     p->use_gpu = 1;
     p->precision = HPRSOCP_PREC_FP64;
-    p->refine_every = 1;
     p->use_graph = 1;
+    p->refine_tol = 1e-6;
+    p->fp32_stall_checks = 40;
 }
 
 /* Q apply, Qmap! (sparse_matrix_operator.jl:47-49) or the factored operator

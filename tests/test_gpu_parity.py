@@ -7,7 +7,7 @@ Tolerances (FP64 everywhere; only summation order differs):
   * residual block, M-norm:                       rtol 1e-9
   * end-to-end: GPU reaches KKT < stoptol = 1e-6, its independent certificate is
     < 1e-5, and its primal objective agrees with an FP64 oracle solve at 1e-9 to
-    1e-6 relative (the north-star objective tolerance).
+    1e-6 relative.
 """
 import numpy as np
 import pytest
