@@ -68,46 +68,77 @@ double golden_sequential(double a, double b, double lo, double hi, double tol, G
     return (S.f1 < S.f2) ? S.x1 : S.x2;
 }
 
-// Eval: void(const std::vector<double>& xs, std::vector<double>& gs) -> gs[k] = g(xs[k])
-template <class Eval>
-double golden_speculative(double a, double b, double lo, double hi, double tol, int L, Eval eval) {
-    const double phi = (sqrt(5.0) - 1.0) / 2.0;
-    const int NP = (1 << L) - 1;
-    auto f_of = [&](double x, double gx) { return a * x + b / x + x * x * gx; };
-    GoldenState S{lo, hi, hi - phi * (hi - lo), lo + phi * (hi - lo), 0, 0};
-    {
-        std::vector<double> xs{S.x1, S.x2}, gs;
-        eval(xs, gs);
-        S.f1 = f_of(S.x1, gs[0]);
-        S.f2 = f_of(S.x2, gs[1]);
+// The speculative search as a resumable walk, so that many independent searches
+// (one per problem instance) can share each batched evaluation:
+//   while (!w.done()) { pts = w.next_points(); gs = g(pts); w.consume(gs); }
+class GoldenWalk {
+public:
+    GoldenWalk(double a, double b, double lo, double hi, double tol, int L)
+        : a_(a), b_(b), tol_(tol), np_((1 << L) - 1), node_((size_t)((1 << L) - 1)), pts_((size_t)((1 << L) - 1)) {
+        const double phi = (sqrt(5.0) - 1.0) / 2.0;
+        S_ = GoldenState{lo, hi, hi - phi * (hi - lo), lo + phi * (hi - lo), 0, 0};
     }
-    int it = 0;
-    std::vector<GoldenState> node(NP);
-    std::vector<double> pts(NP), gs;
-    while (golden_continue(S, it, tol)) {
-        golden_step(S, S.f1 > S.f2, node[0], pts[0]);
-        for (int k = 0; 2 * k + 2 < NP; ++k) {
-            golden_step(node[k], true, node[2 * k + 1], pts[2 * k + 1]);
-            golden_step(node[k], false, node[2 * k + 2], pts[2 * k + 2]);
+    bool done() const { return started_ && !golden_continue(S_, it_, tol_); }
+    // points whose g values the next consume() expects
+    const std::vector<double> &next_points() {
+        if (!started_) {
+            init_ = {S_.x1, S_.x2};
+            return init_;
         }
-        eval(pts, gs);
+        golden_step(S_, S_.f1 > S_.f2, node_[0], pts_[0]);
+        for (int k = 0; 2 * k + 2 < np_; ++k) {
+            golden_step(node_[k], true, node_[2 * k + 1], pts_[2 * k + 1]);
+            golden_step(node_[k], false, node_[2 * k + 2], pts_[2 * k + 2]);
+        }
+        return pts_;
+    }
+    void consume(const double *gs) {
+        if (!started_) {
+            S_.f1 = f_of(S_.x1, gs[0]);
+            S_.f2 = f_of(S_.x2, gs[1]);
+            started_ = true;
+            return;
+        }
         int k = -1;
-        while (golden_continue(S, it, tol)) {
-            const bool A = S.f1 > S.f2;
+        while (golden_continue(S_, it_, tol_)) {
+            const bool A = S_.f1 > S_.f2;
             const int child = k < 0 ? 0 : (A ? 2 * k + 1 : 2 * k + 2);
-            if (child >= NP) break;
+            if (child >= np_) break;
             GoldenState n;
             double p;
-            golden_step(S, A, n, p);
+            golden_step(S_, A, n, p);
             const double fv = f_of(p, gs[child]);
             if (A) n.f2 = fv;
             else n.f1 = fv;
-            S = n;
-            ++it;
+            S_ = n;
+            ++it_;
             k = child;
         }
     }
-    return (S.f1 < S.f2) ? S.x1 : S.x2;
+    double result() const { return (S_.f1 < S_.f2) ? S_.x1 : S_.x2; }
+
+private:
+    double f_of(double x, double gx) const { return a_ * x + b_ / x + x * x * gx; }
+    double a_, b_, tol_;
+    int np_;
+    GoldenState S_{};
+    int it_ = 0;
+    bool started_ = false;
+    std::vector<GoldenState> node_;
+    std::vector<double> pts_, init_;
+};
+
+// Eval: void(const std::vector<double>& xs, std::vector<double>& gs) -> gs[k] = g(xs[k])
+template <class Eval>
+double golden_speculative(double a, double b, double lo, double hi, double tol, int L, Eval eval) {
+    GoldenWalk w(a, b, lo, hi, tol, L);
+    std::vector<double> gs;
+    while (!w.done()) {
+        const std::vector<double> &pts = w.next_points();
+        eval(pts, gs);
+        w.consume(gs.data());
+    }
+    return w.result();
 }
 
 }  // namespace hpr

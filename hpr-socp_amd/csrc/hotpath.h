@@ -24,8 +24,17 @@ enum TileKind : int32_t { TILE_LINEAR = 0, TILE_CONES = 1, TILE_HUGE = 2 };
 //      gathered by the whole workgroup into LDS, then summed per row in CSR
 //      order); otherwise lanes per row (1..64, or 256 = whole workgroup)
 // g2 : lanes per cone of the projection phase (CONES tiles)
+// inst: the problem instance the tile's rows belong to (tiles never straddle
+//      instances; 0 for a single problem)
 struct Tile {
-    int32_t rb, re, kind, g, c0, c1, g2, pad;
+    int32_t rb, re, kind, g, c0, c1, g2, inst;
+};
+
+// Per-instance scalars of the iteration, read by every tile of the instance.
+// h1 = 1/(inner0 + kstep + 2) with kstep the iterations since inner0 was set.
+struct InstScal {
+    double sigma, lambda_A, lambda_Q, b_scale, c_scale, inner0;
+    int32_t active, pad;
 };
 
 // Long cones split over several tiles (two-phase projection).
@@ -74,12 +83,15 @@ struct DevPlan {
 };
 
 // ---- hot-path launches ------------------------------------------------------
+// Every argument block carries the per-instance scalars (isc, indexed by the
+// tile's instance) and the step count since InstScal::inner0 was set.
 template <typename T>
 struct NoQPrimalArgsT {  // unified_update_zx (noQ), reference kernels.jl:5955-6063, 1775-3790
     const T *y;              // SpMV input (A^T y)
     T *x, *x_hat, *x_bar, *z_bar, *dx;
     const T *last_x, *c, *l, *u;
-    T sigma, h1, h2;
+    const InstScal *isc;
+    int64_t kstep;
     int32_t full;
 };
 template <typename T>
@@ -87,14 +99,16 @@ struct DualArgsT {  // unified_update_y (linear + SOC rows), kernels.jl:4354-536
     const T *v;              // SpMV input (A x_hat or A tempv)
     T *y, *y_bar, *s, *dy;
     const T *last_y, *AL, *AU, *rhs;
-    T f1, f2, h1, h2;
+    const InstScal *isc;
+    int64_t kstep;
     int32_t full, write_ybar;
 };
 template <typename T>
 struct QPPrimalArgsT {  // zxw1 + tempv, kernels.jl:379-1650, 4268-4299
     T *x, *x_hat, *x_bar, *z_bar, *dx, *w_bar, *tempv;
     const T *w, *last_x, *c, *l, *u, *aty, *qw, *dq, *fact1, *fact2;
-    T sigma, h1, h2, f1s, f2s;
+    const InstScal *isc;
+    int64_t kstep;
     int32_t full;
     int32_t diag;                 // Q structurally diagonal: Qw = dq .* w, tempv written here
     int32_t vec_fact;             // Q_is_diag: per-coordinate factors fact1/fact2
@@ -104,7 +118,8 @@ struct W2ArgsT {  // update_w2, kernels.jl:5384-5497
     const T *ybar;           // SpMV input (A^T y_bar)
     T *w, *w_bar, *aty, *dw, *atdy, *aty_bar;
     const T *last_w, *last_aty, *fact;
-    T fs, h1, h2;
+    const InstScal *isc;
+    int64_t kstep;
     int32_t full, diag;
 };
 template <typename T>
@@ -113,7 +128,7 @@ struct TempvArgsT {  // tempv = x_hat + sigma*(Qw - Q w_bar) for non-diagonal Q
     T *tempv;
     const T *dq;             // factored Q diagonal (may be null)
     const T *wbar;           // needed with dq
-    T sigma;
+    const InstScal *isc;
 };
 using NoQPrimalArgs = NoQPrimalArgsT<double>;
 using DualArgs = DualArgsT<double>;

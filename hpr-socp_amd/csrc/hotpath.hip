@@ -93,20 +93,37 @@ __device__ __forceinline__ T cone_apply(const ConeCase &c, bool head, T raw) {
 }
 
 // ------------------------------------------------------------ epilogue ops
+// bind(inst) loads the instance's scalars (rounded to T); false = instance
+// retired (its tiles write nothing).
+__device__ __forceinline__ void halpern_weights(const InstScal &s, int64_t k, double &h1, double &h2) {
+    h1 = 1.0 / (s.inner0 + (double)k + 2.0);
+    h2 = 1.0 - h1;
+}
+
 template <typename T>
 struct OpNoQPrimal {
     NoQPrimalArgsT<T> a;
-    __device__ T raw(int i, T acc, bool) const { return a.x[i] + a.sigma * (acc - a.c[i]); }
+    T sigma, h1, h2;
+    __device__ bool bind(int inst) {
+        const InstScal s = a.isc[inst];
+        double d1, d2;
+        halpern_weights(s, a.kstep, d1, d2);
+        sigma = (T)s.sigma;
+        h1 = (T)d1;
+        h2 = (T)d2;
+        return s.active != 0;
+    }
+    __device__ T raw(int i, T acc, bool) const { return a.x[i] + sigma * (acc - a.c[i]); }
     __device__ T lin_proj(int i, T r) const { return jclamp(r, a.l[i], a.u[i]); }
     __device__ void finish(int i, T raw, T p) const {
         const T xi = a.x[i];
         const T xh = T(2) * p - xi;
         if (a.full) {
             a.x_bar[i] = p;
-            a.z_bar[i] = (p - raw) / a.sigma;
+            a.z_bar[i] = (p - raw) / sigma;
             a.dx[i] = p - xi;
         }
-        a.x[i] = a.h2 * xh + a.h1 * a.last_x[i];
+        a.x[i] = h2 * xh + h1 * a.last_x[i];
         a.x_hat[i] = xh;
     }
 };
@@ -114,55 +131,90 @@ struct OpNoQPrimal {
 template <typename T>
 struct OpDual {
     DualArgsT<T> a;
+    T f1, f2, h1, h2;
+    __device__ bool bind(int inst) {
+        const InstScal s = a.isc[inst];
+        double d1, d2;
+        halpern_weights(s, a.kstep, d1, d2);
+        const double g = s.lambda_A * s.sigma;
+        f1 = (T)g;
+        f2 = (T)(1.0 / g);
+        h1 = (T)d1;
+        h2 = (T)d2;
+        return s.active != 0;
+    }
     __device__ T raw(int i, T acc, bool cone) const {
-        return cone ? (acc - a.rhs[i]) - a.f1 * a.y[i] : acc - a.f1 * a.y[i];
+        return cone ? (acc - a.rhs[i]) - f1 * a.y[i] : acc - f1 * a.y[i];
     }
     __device__ T lin_proj(int i, T r) const { return jclamp(r, a.AL[i], a.AU[i]); }
     __device__ void finish(int i, T raw, T p) const {
         const T yi = a.y[i];
-        const T yb = a.f2 * (p - raw);
+        const T yb = f2 * (p - raw);
         if (a.full) {
             a.s[i] = p;
             a.dy[i] = yb - yi;
         }
         if (a.full || a.write_ybar) a.y_bar[i] = yb;
-        a.y[i] = a.h2 * (T(2) * yb - yi) + a.h1 * a.last_y[i];
+        a.y[i] = h2 * (T(2) * yb - yi) + h1 * a.last_y[i];
     }
 };
 
 template <typename T>
 struct OpQPPrimal {
     QPPrimalArgsT<T> a;
+    T sigma, h1, h2, f1s, f2s;
+    __device__ bool bind(int inst) {
+        const InstScal s = a.isc[inst];
+        double d1, d2;
+        halpern_weights(s, a.kstep, d1, d2);
+        const double q2 = 1.0 / (1.0 + s.sigma * s.lambda_Q);
+        sigma = (T)s.sigma;
+        h1 = (T)d1;
+        h2 = (T)d2;
+        f2s = (T)q2;
+        f1s = (T)(1.0 - q2);
+        return s.active != 0;
+    }
     __device__ T qw(int i) const { return a.diag ? a.dq[i] * a.w[i] : a.qw[i]; }
-    __device__ T raw(int i, T, bool) const { return a.x[i] + a.sigma * ((-qw(i) + a.aty[i]) - a.c[i]); }
+    __device__ T raw(int i, T, bool) const { return a.x[i] + sigma * ((-qw(i) + a.aty[i]) - a.c[i]); }
     __device__ T lin_proj(int i, T r) const { return jclamp(r, a.l[i], a.u[i]); }
     __device__ void finish(int i, T raw, T p) const {
         const T xi = a.x[i], wi = a.w[i];
         const T xh = T(2) * p - xi;
-        const T wb = a.vec_fact ? a.fact1[i] * wi + a.fact2[i] * xh : a.f1s * wi + a.f2s * xh;
+        const T wb = a.vec_fact ? a.fact1[i] * wi + a.fact2[i] * xh : f1s * wi + f2s * xh;
         if (a.full) {
             a.x_bar[i] = p;
-            a.z_bar[i] = (p - raw) / a.sigma;
+            a.z_bar[i] = (p - raw) / sigma;
             a.dx[i] = p - xi;
         }
-        a.x[i] = a.h2 * xh + a.h1 * a.last_x[i];
+        a.x[i] = h2 * xh + h1 * a.last_x[i];
         a.x_hat[i] = xh;
         a.w_bar[i] = wb;
-        if (a.diag) a.tempv[i] = xh + a.sigma * (a.dq[i] * wi - a.dq[i] * wb);
+        if (a.diag) a.tempv[i] = xh + sigma * (a.dq[i] * wi - a.dq[i] * wb);
     }
 };
 
 template <typename T>
 struct OpW2 {
     W2ArgsT<T> a;
+    T fs, h1, h2;
+    __device__ bool bind(int inst) {
+        const InstScal s = a.isc[inst];
+        double d1, d2;
+        halpern_weights(s, a.kstep, d1, d2);
+        fs = (T)(s.sigma / (1.0 + s.sigma * s.lambda_Q));
+        h1 = (T)d1;
+        h2 = (T)d2;
+        return s.active != 0;
+    }
     __device__ T raw(int, T acc, bool) const { return acc; }
     __device__ T lin_proj(int, T r) const { return r; }
     __device__ void finish(int i, T rb, T) const {
         const T wi = a.w[i], atyi = a.aty[i];
-        const T f = a.diag ? a.fact[i] : a.fs;
+        const T f = a.diag ? a.fact[i] : fs;
         const T wbn = a.w_bar[i] + f * (rb - atyi);
-        a.w[i] = a.h1 * a.last_w[i] + a.h2 * (T(2) * wbn - wi);
-        a.aty[i] = a.h1 * a.last_aty[i] + a.h2 * (T(2) * rb - atyi);
+        a.w[i] = h1 * a.last_w[i] + h2 * (T(2) * wbn - wi);
+        a.aty[i] = h1 * a.last_aty[i] + h2 * (T(2) * rb - atyi);
         if (a.full) {
             a.w_bar[i] = wbn;
             a.dw[i] = wbn - wi;
@@ -172,10 +224,12 @@ struct OpW2 {
     }
 };
 
+// plain SpMV output (all instances, no scalars)
 template <typename T>
 struct OpStore {
     T *out;
     const T *diag, *v2;
+    __device__ bool bind(int) { return true; }
     __device__ T raw(int, T acc, bool) const { return acc; }
     __device__ T lin_proj(int, T r) const { return r; }
     __device__ void finish(int i, T r, T) const { out[i] = diag ? r + diag[i] * v2[i] : r; }
@@ -184,11 +238,17 @@ struct OpStore {
 template <typename T>
 struct OpTempv {
     TempvArgsT<T> a;
+    T sigma;
+    __device__ bool bind(int inst) {
+        const InstScal s = a.isc[inst];
+        sigma = (T)s.sigma;
+        return s.active != 0;
+    }
     __device__ T raw(int, T acc, bool) const { return acc; }
     __device__ T lin_proj(int, T r) const { return r; }
     __device__ void finish(int i, T r, T) const {
         const T qwb = a.dq ? r + a.dq[i] * a.wbar[i] : r;
-        a.tempv[i] = a.x_hat[i] + a.sigma * (a.qw[i] - qwb);
+        a.tempv[i] = a.x_hat[i] + sigma * (a.qw[i] - qwb);
     }
 };
 
@@ -254,13 +314,15 @@ enum : int { SRC_NONE = 0, SRC_CSR = 1, SRC_SLICES = 2 };
 
 template <typename T, class Op, int SRC, bool CONES>
 __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles, CsrT<T> M, const T *__restrict__ vin,
-                                                 const int32_t *__restrict__ cone_idx, HugeInfo huge, Op op,
+                                                 const int32_t *__restrict__ cone_idx, HugeInfo huge, Op op_in,
                                                  T *__restrict__ raw_buf, double *__restrict__ partials) {
     __shared__ T acc[kTileRows];
     __shared__ double sh[4];
     __shared__ T prod[SRC == SRC_CSR ? kStreamNnz : 1];
     __shared__ int32_t rps[SRC == SRC_CSR ? kTileRows + 1 : 1];
     const Tile t = tiles[blockIdx.x];
+    Op op = op_in;
+    if (!op.bind(t.inst)) return;  // retired instance (uniform over the workgroup)
     const int tid = threadIdx.x;
     const int nrows = t.re - t.rb;
     if constexpr (SRC == SRC_CSR) {
@@ -321,11 +383,13 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
 
 // HUGE segment, phase 2: the cone's |xi| from all partials, then the projection.
 template <typename T, class Op>
-__global__ __launch_bounds__(kBlock) void k_huge_finish(const Tile *__restrict__ tiles, HugeInfo huge, Op op,
+__global__ __launch_bounds__(kBlock) void k_huge_finish(const Tile *__restrict__ tiles, HugeInfo huge, Op op_in,
                                                         const T *__restrict__ raw_buf,
                                                         const double *__restrict__ partials) {
     __shared__ double sh[4];
     const Tile t = tiles[blockIdx.x];
+    Op op = op_in;
+    if (!op.bind(t.inst)) return;
     const int h = t.c0;
     const int s = huge.start[h];
     const int slot0 = huge.slot0[h], nslot = huge.nslot[h];

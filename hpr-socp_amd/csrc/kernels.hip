@@ -26,12 +26,15 @@ __device__ __forceinline__ double red_term(const RedTerm &r, int64_t i) {
     return d * d;
 }
 
-__global__ __launch_bounds__(kBlock) void k_reduce1(int64_t n, RedTerms T, double *partials) {
+#define CHUNK_LOOP(i, ch) for (int64_t i = (ch).start + (int64_t)threadIdx.x; i < (ch).end; i += kBlock)
+
+__global__ __launch_bounds__(kBlock) void k_reduce1(SegPlan sp, RedTerms T, double *partials) {
     __shared__ double sh[4];
+    const SegChunk ch = sp.chunks[blockIdx.x];
     double acc[kMaxRed];
 #pragma unroll
     for (int k = 0; k < kMaxRed; ++k) acc[k] = (k < T.n && T.t[k].op == 3) ? -INFINITY : 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    CHUNK_LOOP(i, ch) {
 #pragma unroll
         for (int k = 0; k < kMaxRed; ++k) {
             if (k < T.n) {
@@ -47,32 +50,34 @@ __global__ __launch_bounds__(kBlock) void k_reduce1(int64_t n, RedTerms T, doubl
     }
 }
 
-// ops: per slot 0 = sum, 1 = max (packed bits in `maxmask`)
-__global__ __launch_bounds__(kBlock) void k_reduce2(int nblocks, int nterms, uint32_t maxmask,
-                                                    const double *partials, double *out) {
+// Second level: workgroup i reduces the partials of instance i's chunks in
+// chunk order.  maxmask bit k: slot k is a max; bit 16+k: signed max (-inf identity).
+__global__ __launch_bounds__(kBlock) void k_reduce2(SegPlan sp, int nterms, uint32_t maxmask, int pstride,
+                                                    const double *partials, double *out, int ostride) {
     __shared__ double sh[4];
+    const int inst = blockIdx.x;
+    const int c0 = sp.first[inst], c1 = sp.first[inst + 1];
     for (int k = 0; k < nterms; ++k) {
         const bool mx = (maxmask >> k) & 1u;
-        const bool neg = (maxmask >> (16 + k)) & 1u;  // signed max: -inf identity
+        const bool neg = (maxmask >> (16 + k)) & 1u;
         double acc = neg ? -INFINITY : 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += kBlock) {
-            const double v = partials[(int64_t)b * kMaxRed + k];
+        for (int b = c0 + threadIdx.x; b < c1; b += kBlock) {
+            const double v = partials[(int64_t)b * pstride + k];
             acc = mx ? nmax(acc, v) : acc + v;
         }
         const double r = mx ? block_max(acc, sh) : block_sum(acc, sh);
-        if (threadIdx.x == 0) out[k] = r;
+        if (threadIdx.x == 0) out[(int64_t)inst * ostride + k] = r;
     }
 }
 
-int reduce_blocks(int64_t n) {
-    int64_t b = (n + kBlock - 1) / kBlock;
-    if (b < 1) b = 1;
-    if (b > 1024) b = 1024;
-    return (int)b;
+static void finish_reduce(hipStream_t st, const SegPlan &sp, int nterms, uint32_t maxmask, int pstride,
+                          const double *partials, double *out, int ostride) {
+    hipLaunchKernelGGL(k_reduce2, dim3(sp.ninst), dim3(kBlock), 0, st, sp, nterms, maxmask, pstride, partials, out,
+                       ostride);
 }
 
-void launch_reduce(hipStream_t st, int64_t n, const RedTerm *terms, int nterms, double *partials,
-                   int nblocks, double *out) {
+void launch_reduce(hipStream_t st, const SegPlan &sp, const RedTerm *terms, int nterms, double *partials,
+                   double *out, int ostride) {
     RedTerms T{};
     uint32_t mask = 0;
     for (int k = 0; k < nterms; ++k) {
@@ -81,9 +86,11 @@ void launch_reduce(hipStream_t st, int64_t n, const RedTerm *terms, int nterms, 
         if (terms[k].op == 3) mask |= 1u << (16 + k);
     }
     T.n = nterms;
-    hipLaunchKernelGGL(k_reduce1, dim3(nblocks), dim3(kBlock), 0, st, n, T, partials);
-    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, st, nblocks, nterms, mask, partials, out);
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_reduce1, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, T, partials);
+    finish_reduce(st, sp, nterms, mask, kMaxRed, partials, out, ostride);
 }
+
+__device__ __forceinline__ double ival(InstVal v, int inst) { return v.p[(int64_t)inst * v.stride]; }
 
 // ---------------------------------------------------------------- elementwise
 #define GRID_STRIDE(i, n) \
@@ -126,11 +133,13 @@ void launch_mul(hipStream_t st, int64_t n, const double *a, const double *b, dou
 }
 
 // ---------------------------------------------------------------- residuals
-__global__ __launch_bounds__(kBlock) void k_resid_n(int32_t n, ResidNArgs a, double *partials) {
+__global__ __launch_bounds__(kBlock) void k_resid_n(SegPlan sp, ResidNArgs a, double *partials) {
     __shared__ double sh[4];
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const double cs = a.isc[ch.inst].c_scale;
     double s0 = 0, s1 = 0, s2 = 0, m3 = 0, m4 = 0, m5 = 0;
-    GRID_STRIDE(j, n) {
-        const double cn = a.col_norm[j], cs = a.c_scale;
+    CHUNK_LOOP(j, ch) {
+        const double cn = a.col_norm[j];
         const double xb = a.x_bar[j], zb = a.z_bar[j];
         const double c_org = a.c[j] * cn * cs, at = a.atdy[j] * cn * cs, z = zb * cn * cs;
         double rd;
@@ -157,29 +166,32 @@ __global__ __launch_bounds__(kBlock) void k_resid_n(int32_t n, ResidNArgs a, dou
         for (int k = 0; k < 6; ++k) partials[(int64_t)blockIdx.x * kMaxRed + k] = vals[k];
 }
 
-void launch_resid_n(hipStream_t st, int32_t n, const ResidNArgs &a, double *partials, int nblocks,
-                    double *out) {
-    hipLaunchKernelGGL(k_resid_n, dim3(nblocks), dim3(kBlock), 0, st, n, a, partials);
-    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, st, nblocks, 6, 0x38u, partials, out);
+void launch_resid_n(hipStream_t st, const SegPlan &sp, const ResidNArgs &a, double *partials, double *out,
+                    int ostride) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_resid_n, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, a, partials);
+    finish_reduce(st, sp, 6, 0x38u, kMaxRed, partials, out, ostride);
 }
 
-__global__ void k_ax_to_org(int32_t m, double *ax, const double *rn, double bs) {
-    GRID_STRIDE(i, m) ax[i] = ax[i] * rn[i] * bs;
+__global__ void k_ax_to_org(SegPlan sp, double *ax, const double *rn, const InstScal *isc) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const double bs = isc[ch.inst].b_scale;
+    CHUNK_LOOP(i, ch) ax[i] = ax[i] * rn[i] * bs;
 }
-void launch_ax_to_org(hipStream_t st, int32_t m, double *ax, const double *row_norm, double b_scale) {
-    if (m > 0) hipLaunchKernelGGL(k_ax_to_org, dim3(ew_blocks(m)), dim3(kBlock), 0, st, m, ax, row_norm, b_scale);
+void launch_ax_to_org(hipStream_t st, const SegPlan &sp, double *ax, const double *row_norm, const InstScal *isc) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_ax_to_org, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, ax, row_norm, isc);
 }
 
 // one wavefront per cone: t and ||xi|| of (A x)_org - rhs_org
-__global__ __launch_bounds__(kBlock) void k_cone_rp_stats(int32_t ncones, const int32_t *idx,
+__global__ __launch_bounds__(kBlock) void k_cone_rp_stats(int32_t ncones, const int32_t *idx, const int32_t *cinst,
                                                           const double *ax, const double *rhs,
-                                                          const double *rn, double bs, double *ct,
+                                                          const double *rn, const InstScal *isc, double *ct,
                                                           double *cns) {
     const int lane = threadIdx.x & 63;
     const int64_t wid = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
     for (int64_t c = wid; c < ncones; c += nw) {
         const int s = idx[c], e = idx[c + 1];
+        const double bs = isc[cinst ? cinst[c] : 0].b_scale;
         double sq = 0.0;
         for (int j = s + 1 + lane; j < e; j += 64) {
             const double v = ax[j] - rhs[j] * rn[j] * bs;
@@ -192,23 +204,24 @@ __global__ __launch_bounds__(kBlock) void k_cone_rp_stats(int32_t ncones, const 
         }
     }
 }
-void launch_cone_rp_stats(hipStream_t st, int32_t ncones, const int32_t *con_idx, const double *ax_org,
-                          const double *rhs, const double *row_norm, double b_scale, double *cone_t,
-                          double *cone_ns) {
+void launch_cone_rp_stats(hipStream_t st, int32_t ncones, const int32_t *con_idx, const int32_t *cone_inst,
+                          const double *ax_org, const double *rhs, const double *row_norm, const InstScal *isc,
+                          double *cone_t, double *cone_ns) {
     if (ncones <= 0) return;
     int blocks = (int)((ncones + 3) / 4);
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_cone_rp_stats, dim3(blocks), dim3(kBlock), 0, st, ncones, con_idx, ax_org, rhs,
-                       row_norm, b_scale, cone_t, cone_ns);
+    hipLaunchKernelGGL(k_cone_rp_stats, dim3(blocks), dim3(kBlock), 0, st, ncones, con_idx, cone_inst, ax_org, rhs,
+                       row_norm, isc, cone_t, cone_ns);
 }
 
-
-__global__ __launch_bounds__(kBlock) void k_resid_m(int32_t m, ResidMArgs a, double *partials) {
+__global__ __launch_bounds__(kBlock) void k_resid_m(SegPlan sp, ResidMArgs a, double *partials) {
     __shared__ double sh[4];
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const double bs = a.isc[ch.inst].b_scale;
     double s0 = 0, s1 = 0, m2 = 0, m3 = 0, m4 = 0;
-    GRID_STRIDE(i, m) {
+    CHUNK_LOOP(i, ch) {
         const double ax = a.ax[i];
-        const double rn = a.row_norm[i], bs = a.b_scale;
+        const double rn = a.row_norm[i];
         const double yb = a.y_bar[i];
         s0 += yb * a.s[i];
         s1 += yb * a.rhs[i];
@@ -237,25 +250,28 @@ __global__ __launch_bounds__(kBlock) void k_resid_m(int32_t m, ResidMArgs a, dou
         for (int k = 0; k < 5; ++k) partials[(int64_t)blockIdx.x * kMaxRed + k] = vals[k];
 }
 
-void launch_resid_m(hipStream_t st, int32_t m, const ResidMArgs &a, double *partials, int nblocks,
-                    double *out) {
-    hipLaunchKernelGGL(k_resid_m, dim3(nblocks), dim3(kBlock), 0, st, m, a, partials);
-    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, st, nblocks, 5, 0x1Cu, partials, out);
+void launch_resid_m(hipStream_t st, const SegPlan &sp, const ResidMArgs &a, double *partials, double *out,
+                    int ostride) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_resid_m, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, a, partials);
+    finish_reduce(st, sp, 5, 0x1Cu, kMaxRed, partials, out, ostride);
 }
 
 // compute_bounds_violation!: box part over all coordinates, cone part per cone
-__global__ void k_bv_box(int32_t n, const double *xb, const double *l, const double *u, const double *cn,
-                         double bs, double *dx) {
-    GRID_STRIDE(j, n) {
+__global__ void k_bv_box(SegPlan sp, const double *xb, const double *l, const double *u, const double *cn,
+                         const InstScal *isc, double *dx) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const double bs = isc[ch.inst].b_scale;
+    CHUNK_LOOP(j, ch) {
         const double lo = nmax(l[j] - xb[j], 0.0), up = nmax(xb[j] - u[j], 0.0);
         dx[j] = (lo + up) * (bs / cn[j]);
     }
 }
-__global__ void k_bv_cone(int32_t nvar, const int32_t *idx, const double *xb, const double *cn, double bs,
-                          double *dx) {
+__global__ void k_bv_cone(int32_t nvar, const int32_t *idx, const int32_t *cinst, const double *xb,
+                          const double *cn, const InstScal *isc, double *dx) {
     GRID_STRIDE(k, nvar) {
         const int s = idx[k], e = idx[k + 1];
         if (s >= e) continue;
+        const double bs = isc[cinst ? cinst[k] : 0].b_scale;
         const double t = xb[s];
         double ns = 0.0;
         dx[s] = t;
@@ -275,33 +291,33 @@ __global__ void k_bv_cone(int32_t nvar, const int32_t *idx, const double *xb, co
         for (int j = s; j < e; ++j) dx[j] *= bs / cn[j];
     }
 }
-void launch_bounds_violation(hipStream_t st, int32_t n, int32_t nvar, const int32_t *var_idx,
-                             const double *x_bar, const double *l, const double *u,
-                             const double *col_norm, double b_scale, double *dx, double *partials,
-                             int nblocks, double *out) {
-    if (n > 0)
-        hipLaunchKernelGGL(k_bv_box, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, x_bar, l, u, col_norm, b_scale, dx);
+void launch_bounds_violation(hipStream_t st, const SegPlan &sp, int32_t nvar, const int32_t *var_idx,
+                             const int32_t *cone_inst, const double *x_bar, const double *l, const double *u,
+                             const double *col_norm, const InstScal *isc, double *dx, double *partials,
+                             double *out, int ostride) {
+    if (sp.nchunks > 0)
+        hipLaunchKernelGGL(k_bv_box, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, x_bar, l, u, col_norm, isc, dx);
     if (nvar > 0)
-        hipLaunchKernelGGL(k_bv_cone, dim3(ew_blocks(nvar)), dim3(kBlock), 0, st, nvar, var_idx, x_bar, col_norm,
-                           b_scale, dx);
+        hipLaunchKernelGGL(k_bv_cone, dim3(ew_blocks(nvar)), dim3(kBlock), 0, st, nvar, var_idx, cone_inst, x_bar,
+                           col_norm, isc, dx);
     RedTerm t{dx, nullptr, nullptr, 1};
-    launch_reduce(st, n, &t, 1, partials, nblocks, out);
+    launch_reduce(st, sp, &t, 1, partials, out, ostride);
 }
 
 // ------------------------------------------------------------ golden section
-constexpr int kMaxGoldenPts = 64;
-__global__ __launch_bounds__(kBlock) void k_golden_eval(int32_t n, const double *atdy, const double *qatdy,
+__global__ __launch_bounds__(kBlock) void k_golden_eval(SegPlan sp, const double *atdy, const double *qatdy,
                                                         const double *dq, const double *xs, int npts,
                                                         double *partials) {
     __shared__ double sh[4];
-    __shared__ double xsh[kMaxGoldenPts];
-    if ((int)threadIdx.x < npts) xsh[threadIdx.x] = xs[threadIdx.x];
+    __shared__ double xsh[kMaxGolden];
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    if ((int)threadIdx.x < npts) xsh[threadIdx.x] = xs[(int64_t)ch.inst * kMaxGolden + threadIdx.x];
     __syncthreads();
     for (int k0 = 0; k0 < npts; k0 += kMaxRed) {
         double acc[kMaxRed];
 #pragma unroll
         for (int k = 0; k < kMaxRed; ++k) acc[k] = 0.0;
-        GRID_STRIDE(i, n) {
+        CHUNK_LOOP(i, ch) {
             const double c = atdy[i], d = qatdy[i], q = dq[i];
 #pragma unroll
             for (int k = 0; k < kMaxRed; ++k)
@@ -309,59 +325,55 @@ __global__ __launch_bounds__(kBlock) void k_golden_eval(int32_t n, const double 
         }
         for (int k = 0; k < kMaxRed && k0 + k < npts; ++k) {
             const double r = block_sum(acc[k], sh);
-            if (threadIdx.x == 0) partials[((int64_t)blockIdx.x * kMaxGoldenPts) + k0 + k] = r;
+            if (threadIdx.x == 0) partials[((int64_t)blockIdx.x * kMaxGolden) + k0 + k] = r;
         }
     }
 }
-__global__ __launch_bounds__(kBlock) void k_golden_sum(int nblocks, int npts, const double *partials,
-                                                       double *out) {
-    __shared__ double sh[4];
-    for (int k = 0; k < npts; ++k) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += kBlock) acc += partials[(int64_t)b * kMaxGoldenPts + k];
-        const double r = block_sum(acc, sh);
-        if (threadIdx.x == 0) out[k] = r;
-    }
-}
-void launch_golden_eval(hipStream_t st, int32_t n, const double *atdy, const double *qatdy,
-                        const double *dq, const double *xs, int npts, double *partials, int nblocks,
-                        double *out) {
-    hipLaunchKernelGGL(k_golden_eval, dim3(nblocks), dim3(kBlock), 0, st, n, atdy, qatdy, dq, xs, npts, partials);
-    hipLaunchKernelGGL(k_golden_sum, dim3(1), dim3(kBlock), 0, st, nblocks, npts, partials, out);
+void launch_golden_eval(hipStream_t st, const SegPlan &sp, const double *atdy, const double *qatdy,
+                        const double *dq, const double *xs, int npts, double *partials, double *out,
+                        int ostride) {
+    if (sp.nchunks > 0)
+        hipLaunchKernelGGL(k_golden_eval, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, atdy, qatdy, dq, xs, npts,
+                           partials);
+    finish_reduce(st, sp, npts, 0u, kMaxGolden, partials, out, ostride);
 }
 
 // ------------------------------------------------------------ power iteration
-__global__ void k_scale_by_rnorm(int64_t n, const double *z, const double *nrm2, double *q, const int32_t *done) {
-    if (*done) return;
-    const double nz = sqrt(*nrm2);
-    GRID_STRIDE(i, n) q[i] = z[i] / nz;
+__global__ void k_scale_by_rnorm(SegPlan sp, const double *z, InstVal nrm2, double *q, const PowerState *ps) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    if (ps[ch.inst].done) return;
+    const double nz = sqrt(ival(nrm2, ch.inst));
+    CHUNK_LOOP(i, ch) q[i] = z[i] / nz;
 }
-__global__ void k_power_residual(int64_t n, const double *z, const double *lam, double *q, const int32_t *done) {
-    if (*done) return;
-    const double l = *lam;
-    GRID_STRIDE(i, n) q[i] = z[i] - l * q[i];
+__global__ void k_power_residual(SegPlan sp, const double *z, InstVal lam, double *q, const PowerState *ps) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    if (ps[ch.inst].done) return;
+    const double l = ival(lam, ch.inst);
+    CHUNK_LOOP(i, ch) q[i] = z[i] - l * q[i];
 }
-__global__ void k_power_check(const double *lam, const double *qq, const double *zz, PowerState *ps, double tol,
+__global__ void k_power_check(int32_t ninst, InstVal lam, InstVal qq, InstVal zz, PowerState *ps, double tol,
                               int32_t max_iter) {
-    if (ps->done) return;
-    const double l = *lam;
-    const double err = sqrt(*qq) / (sqrt(*zz) + l);
-    ps->lambda = l;
-    ps->err = err;
-    ps->iters += 1;
-    if (err < tol || ps->iters >= max_iter) ps->done = 1;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ninst || ps[i].done) return;
+    const double l = ival(lam, i);
+    const double err = sqrt(ival(qq, i)) / (sqrt(ival(zz, i)) + l);
+    ps[i].lambda = l;
+    ps[i].err = err;
+    ps[i].iters += 1;
+    if (err < tol || ps[i].iters >= max_iter) ps[i].done = 1;
 }
-void launch_scale_by_rnorm(hipStream_t st, int64_t n, const double *z, const double *nrm2, double *q,
-                           const int32_t *done) {
-    hipLaunchKernelGGL(k_scale_by_rnorm, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, z, nrm2, q, done);
+void launch_scale_by_rnorm(hipStream_t st, const SegPlan &sp, const double *z, InstVal nrm2, double *q,
+                           const PowerState *ps) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_scale_by_rnorm, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, z, nrm2, q, ps);
 }
-void launch_power_residual(hipStream_t st, int64_t n, const double *z, const double *lam, double *q,
-                           const int32_t *done) {
-    hipLaunchKernelGGL(k_power_residual, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, z, lam, q, done);
+void launch_power_residual(hipStream_t st, const SegPlan &sp, const double *z, InstVal lam, double *q,
+                           const PowerState *ps) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_power_residual, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, z, lam, q, ps);
 }
-void launch_power_check(hipStream_t st, const double *lam, const double *qq, const double *zz,
-                        PowerState *ps, double tol, int32_t max_iter) {
-    hipLaunchKernelGGL(k_power_check, dim3(1), dim3(1), 0, st, lam, qq, zz, ps, tol, max_iter);
+void launch_power_check(hipStream_t st, int32_t ninst, InstVal lam, InstVal qq, InstVal zz, PowerState *ps,
+                        double tol, int32_t max_iter) {
+    hipLaunchKernelGGL(k_power_check, dim3((ninst + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ninst, lam, qq, zz,
+                       ps, tol, max_iter);
 }
 
 // ------------------------------------------------------------ scaling
@@ -388,8 +400,8 @@ void launch_row_stat(hipStream_t st, const CsrView &M, const CsrView *Q, int use
     hipLaunchKernelGGL(k_row_stat, dim3(ew_blocks(M.nrows)), dim3(kBlock), 0, st, M, q, Q ? 1 : 0, use_sum, out);
 }
 
-__global__ void k_block_scale(int32_t ncones, const int32_t *idx, double *t, int32_t strategy, int32_t phase,
-                              int32_t loc) {
+__global__ void k_block_scale(int32_t ncones, const int32_t *idx, const int32_t *cnt, double *t, int32_t strategy,
+                              int32_t phase, int32_t loc) {
     GRID_STRIDE(c, ncones) {
         const int s = idx[c], e = idx[c + 1];
         if (s >= e) continue;
@@ -399,15 +411,15 @@ __global__ void k_block_scale(int32_t ncones, const int32_t *idx, double *t, int
             ss += t[j] * t[j];
         }
         const double rms = sqrt(ss / (double)(e - s));
-        const double v = soc_block_value(e - s, bm, rms, strategy, phase, loc, ncones);
+        const double v = soc_block_value(e - s, bm, rms, strategy, phase, loc, cnt ? cnt[c] : ncones);
         for (int j = s; j < e; ++j) t[j] = v;
     }
 }
-void launch_block_scale(hipStream_t st, int32_t ncones, const int32_t *idx, double *t, int32_t strategy,
-                        int32_t phase, int32_t loc) {
+void launch_block_scale(hipStream_t st, int32_t ncones, const int32_t *idx, const int32_t *cone_cnt, double *t,
+                        int32_t strategy, int32_t phase, int32_t loc) {
     if (ncones > 0)
-        hipLaunchKernelGGL(k_block_scale, dim3(ew_blocks(ncones)), dim3(kBlock), 0, st, ncones, idx, t, strategy,
-                           phase, loc);
+        hipLaunchKernelGGL(k_block_scale, dim3(ew_blocks(ncones)), dim3(kBlock), 0, st, ncones, idx, cone_cnt, t,
+                           strategy, phase, loc);
 }
 __global__ void k_clip(int64_t n, double *v) {
     GRID_STRIDE(i, n) { const double x = v[i]; v[i] = x > 1e4 ? 1e4 : (x < 1e-4 ? 1e-4 : x); }
@@ -460,16 +472,19 @@ void launch_scale_rows(hipStream_t st, int32_t m, const double *trow, double *AL
                        double *row_norm) {
     if (m > 0) hipLaunchKernelGGL(k_scale_rows, dim3(ew_blocks(m)), dim3(kBlock), 0, st, m, trow, AL, AU, rhs, row_norm);
 }
-__global__ void k_scale_scalar(int64_t n, double *v, double mul, double div) {
-    GRID_STRIDE(i, n) v[i] = div != 1.0 ? v[i] / div : v[i] * mul;
+__global__ void k_scale_scalar(SegPlan sp, double *v, InstVal mul, InstVal div) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const double mu = ival(mul, ch.inst), dv = ival(div, ch.inst);
+    CHUNK_LOOP(i, ch) v[i] = dv != 1.0 ? v[i] / dv : v[i] * mu;
 }
-void launch_scale_scalar(hipStream_t st, int64_t n, double *v, double mul, double div) {
-    if (n > 0) hipLaunchKernelGGL(k_scale_scalar, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, v, mul, div);
+void launch_scale_scalar(hipStream_t st, const SegPlan &sp, double *v, InstVal mul, InstVal div) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_scale_scalar, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, v, mul, div);
 }
-__global__ __launch_bounds__(kBlock) void k_finite_stats(int64_t n, const double *v, double *partials) {
+__global__ __launch_bounds__(kBlock) void k_finite_stats(SegPlan sp, const double *v, double *partials) {
     __shared__ double sh[4];
+    const SegChunk ch = sp.chunks[blockIdx.x];
     double ss = 0, cnt = 0, mx = 0;
-    GRID_STRIDE(i, n) {
+    CHUNK_LOOP(i, ch) {
         const double x = v[i];
         if (isfinite(x)) {
             ss += x * x;
@@ -484,15 +499,17 @@ __global__ __launch_bounds__(kBlock) void k_finite_stats(int64_t n, const double
         partials[(int64_t)blockIdx.x * kMaxRed + 2] = c;
     }
 }
-void launch_finite_stats(hipStream_t st, int64_t n, const double *v, double *partials, int nblocks, double *out) {
-    hipLaunchKernelGGL(k_finite_stats, dim3(nblocks), dim3(kBlock), 0, st, n, v, partials);
-    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, st, nblocks, 3, 0x4u, partials, out);
+void launch_finite_stats(hipStream_t st, const SegPlan &sp, const double *v, double *partials, double *out,
+                         int ostride) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_finite_stats, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, v, partials);
+    finish_reduce(st, sp, 3, 0x4u, kMaxRed, partials, out, ostride);
 }
-__global__ __launch_bounds__(kBlock) void k_bound_stats(int32_t m, const double *AL, const double *AU,
+__global__ __launch_bounds__(kBlock) void k_bound_stats(SegPlan sp, const double *AL, const double *AU,
                                                         double *partials) {
     __shared__ double sh[4];
+    const SegChunk ch = sp.chunks[blockIdx.x];
     double ss = 0, mx = 0;
-    GRID_STRIDE(i, m) {
+    CHUNK_LOOP(i, ch) {
         const double a = AL[i] == -INFINITY ? 0.0 : AL[i];
         const double b = AU[i] == INFINITY ? 0.0 : AU[i];
         const double v = nmax(fabs(a), fabs(b));
@@ -505,21 +522,23 @@ __global__ __launch_bounds__(kBlock) void k_bound_stats(int32_t m, const double 
         partials[(int64_t)blockIdx.x * kMaxRed + 1] = c;
     }
 }
-void launch_bound_stats(hipStream_t st, int32_t m, const double *AL, const double *AU, double *partials,
-                        int nblocks, double *out) {
-    hipLaunchKernelGGL(k_bound_stats, dim3(nblocks), dim3(kBlock), 0, st, m, AL, AU, partials);
-    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, st, nblocks, 2, 0x2u, partials, out);
+void launch_bound_stats(hipStream_t st, const SegPlan &sp, const double *AL, const double *AU, double *partials,
+                        double *out, int ostride) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_bound_stats, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, AL, AU, partials);
+    finish_reduce(st, sp, 2, 0x2u, kMaxRed, partials, out, ostride);
 }
 // check_Q_diagonal: diag_Q and count of coordinates where colsum|Q| is not ≈ |diag|
 // (Q is symmetric here, so row sums stand for column sums); out[1] = off-diagonal entries
-__global__ __launch_bounds__(kBlock) void k_q_diag(int32_t n, const int32_t *rp, const int32_t *col,
+__global__ __launch_bounds__(kBlock) void k_q_diag(SegPlan sp, const int32_t *rp, const int32_t *col,
                                                    const double *val, double *diag, double *partials) {
     __shared__ double sh[4];
-    double bad = 0, off = 0;
-    GRID_STRIDE(i, n) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    double bad = 0, off = 0, mx = -INFINITY;
+    CHUNK_LOOP(i, ch) {
         double d = 0.0, rs = 0.0;
         for (int k = rp[i]; k < rp[i + 1]; ++k) {
             rs += fabs(val[k]);
+            mx = nmax(mx, val[k]);
             if (col[k] == i) d += val[k];
             else off += 1.0;
         }
@@ -528,19 +547,33 @@ __global__ __launch_bounds__(kBlock) void k_q_diag(int32_t n, const int32_t *rp,
         const double rtol = 1.4901161193847656e-08;
         if (!(a == b || fabs(a - b) <= rtol * fmax(a, b))) bad += 1.0;
     }
-    const double a = block_sum(bad, sh), b = block_sum(off, sh);
+    const double a = block_sum(bad, sh), b = block_sum(off, sh), c = block_max(mx, sh);
     if (threadIdx.x == 0) {
         partials[(int64_t)blockIdx.x * kMaxRed + 0] = a;
         partials[(int64_t)blockIdx.x * kMaxRed + 1] = b;
+        partials[(int64_t)blockIdx.x * kMaxRed + 2] = c;
     }
 }
-void launch_q_diag(hipStream_t st, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val,
-                   double *diag, double *partials, int nblocks, double *out) {
-    hipLaunchKernelGGL(k_q_diag, dim3(nblocks), dim3(kBlock), 0, st, n, rowptr, col, val, diag, partials);
-    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, st, nblocks, 2, 0x0u, partials, out);
+void launch_q_diag(hipStream_t st, const SegPlan &sp, const int32_t *rowptr, const int32_t *col, const double *val,
+                   double *diag, double *partials, double *out, int ostride) {
+    if (sp.nchunks > 0)
+        hipLaunchKernelGGL(k_q_diag, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, rowptr, col, val, diag, partials);
+    finish_reduce(st, sp, 3, 0x4u | (1u << 18), kMaxRed, partials, out, ostride);
 }
-__global__ void k_q_factors(int32_t n, const double *dq, double s, double *f2, double *f, double *f1, double *fm) {
-    GRID_STRIDE(i, n) {
+// v[k] *= f(inst) for every entry k of the rows (columns space) of each instance
+__global__ void k_scale_rows_inst(SegPlan sp, const int32_t *rp, double *val, InstVal f) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const double fv = ival(f, ch.inst);
+    CHUNK_LOOP(i, ch) for (int k = rp[i]; k < rp[i + 1]; ++k) val[k] = val[k] * fv;
+}
+void launch_scale_rows_inst(hipStream_t st, const SegPlan &sp, const int32_t *rowptr, double *val, InstVal f) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_scale_rows_inst, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, rowptr, val, f);
+}
+__global__ void k_q_factors(SegPlan sp, const double *dq, const InstScal *isc, double *f2, double *f, double *f1,
+                            double *fm) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const double s = isc[ch.inst].sigma;
+    CHUNK_LOOP(i, ch) {
         const double a = 1.0 / (1.0 + s * dq[i]);
         f2[i] = a;
         f[i] = s * a;
@@ -548,41 +581,60 @@ __global__ void k_q_factors(int32_t n, const double *dq, double s, double *f2, d
         fm[i] = (s * s) * a;
     }
 }
-void launch_q_factors(hipStream_t st, int32_t n, const double *dq, double sigma, double *fact2, double *fact,
-                      double *fact1, double *fact_m) {
-    if (n > 0) hipLaunchKernelGGL(k_q_factors, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, dq, sigma, fact2, fact, fact1, fact_m);
+void launch_q_factors(hipStream_t st, const SegPlan &sp, const double *dq, const InstScal *isc, double *fact2,
+                      double *fact, double *fact1, double *fact_m) {
+    if (sp.nchunks > 0)
+        hipLaunchKernelGGL(k_q_factors, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, dq, isc, fact2, fact, fact1, fact_m);
 }
-__global__ void k_init_zbar(int32_t n, const double *xb, const double *qx, const double *aty, const double *c,
-                            double sigma, double *zb) {
-    GRID_STRIDE(j, n) {
+__global__ void k_init_zbar(SegPlan sp, const double *xb, const double *qx, const double *aty, const double *c,
+                            const InstScal *isc, double *zb) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const double sigma = isc[ch.inst].sigma;
+    CHUNK_LOOP(j, ch) {
         const double tmp = qx ? (-qx[j] + aty[j]) - c[j] : aty[j] - c[j];
         const double zr = xb[j] + sigma * tmp;
         zb[j] = (xb[j] - zr) / sigma;
     }
 }
-void launch_init_zbar(hipStream_t st, int32_t n, const double *x_bar, const double *qx, const double *aty,
-                      const double *c, double sigma, double *z_bar) {
-    if (n > 0) hipLaunchKernelGGL(k_init_zbar, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, x_bar, qx, aty, c, sigma, z_bar);
+void launch_init_zbar(hipStream_t st, const SegPlan &sp, const double *x_bar, const double *qx, const double *aty,
+                      const double *c, const InstScal *isc, double *z_bar) {
+    if (sp.nchunks > 0)
+        hipLaunchKernelGGL(k_init_zbar, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, x_bar, qx, aty, c, isc, z_bar);
 }
-__global__ void k_init_s(int32_t m, const double *ax, const double *y, const double *AL, const double *AU, double f1,
-                         double *s) {
-    GRID_STRIDE(i, m) s[i] = jclamp(ax[i] - f1 * y[i], AL[i], AU[i]);
+__global__ void k_init_s(SegPlan sp, const double *ax, const double *y, const double *AL, const double *AU,
+                         const InstScal *isc, double *s) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const double f1 = isc[ch.inst].lambda_A * isc[ch.inst].sigma;
+    CHUNK_LOOP(i, ch) s[i] = jclamp(ax[i] - f1 * y[i], AL[i], AU[i]);
 }
-void launch_init_s(hipStream_t st, int32_t m, const double *ax, const double *y, const double *AL, const double *AU,
-                   double f1, double *s) {
-    if (m > 0) hipLaunchKernelGGL(k_init_s, dim3(ew_blocks(m)), dim3(kBlock), 0, st, m, ax, y, AL, AU, f1, s);
+void launch_init_s(hipStream_t st, const SegPlan &sp, const double *ax, const double *y, const double *AL,
+                   const double *AU, const InstScal *isc, double *s) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_init_s, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, ax, y, AL, AU, isc, s);
 }
-__global__ void k_scale_vd(int64_t n, const double *v, const double *d, double s, double *out) {
-    GRID_STRIDE(i, n) out[i] = (v[i] * d[i]) / s;
+__global__ void k_scale_vd(SegPlan sp, const double *v, const double *d, InstVal s, double *out) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const double sv = ival(s, ch.inst);
+    CHUNK_LOOP(i, ch) out[i] = (v[i] * d[i]) / sv;
 }
-void launch_scale_vd(hipStream_t st, int64_t n, const double *v, const double *d, double s, double *out) {
-    if (n > 0) hipLaunchKernelGGL(k_scale_vd, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, v, d, s, out);
+void launch_scale_vd(hipStream_t st, const SegPlan &sp, const double *v, const double *d, InstVal s, double *out) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_scale_vd, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, v, d, s, out);
 }
-__global__ void k_unscale(int64_t n, const double *v, const double *d, double a, int mul, double *out) {
-    GRID_STRIDE(i, n) out[i] = mul ? a * (v[i] * d[i]) : a * (v[i] / d[i]);
+__global__ void k_unscale(SegPlan sp, const double *v, const double *d, InstVal a, int mul, double *out) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const double av = ival(a, ch.inst);
+    CHUNK_LOOP(i, ch) out[i] = mul ? av * (v[i] * d[i]) : av * (v[i] / d[i]);
 }
-void launch_unscale(hipStream_t st, int64_t n, const double *v, const double *d, double a, int mul, double *out) {
-    if (n > 0) hipLaunchKernelGGL(k_unscale, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, v, d, a, mul, out);
+void launch_unscale(hipStream_t st, const SegPlan &sp, const double *v, const double *d, InstVal a, int mul,
+                    double *out) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_unscale, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, v, d, a, mul, out);
+}
+__global__ void k_copy_flagged(SegPlan sp, const double *src, double *dst, const int32_t *flag) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    if (!flag[ch.inst]) return;
+    CHUNK_LOOP(i, ch) dst[i] = src[i];
+}
+void launch_copy_flagged(hipStream_t st, const SegPlan &sp, const double *src, double *dst, const int32_t *flag) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_copy_flagged, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, src, dst, flag);
 }
 
 __global__ void k_gather_perm(int64_t nnz, const double *src, const int32_t *perm, double *dst) {

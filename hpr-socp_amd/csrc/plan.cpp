@@ -23,22 +23,26 @@ static int32_t choose_g(int64_t nnz, int64_t rows, const PlanConfig &cfg) {
 }
 
 HostPlan build_plan(const int32_t *rowptr, int32_t nrows, int32_t lin_end, const int32_t *cone_idx,
-                    int32_t ncones, const PlanConfig &cfg) {
+                    int32_t ncones, const PlanConfig &cfg, const int32_t *row_inst) {
     HostPlan P;
     auto nnz_of = [&](int32_t a, int32_t b) -> int64_t {
         return rowptr ? (int64_t)rowptr[b] - rowptr[a] : (int64_t)(b - a);
     };
+    auto inst_of = [&](int32_t row) -> int32_t {
+        return (row_inst && row >= 0 && row < nrows) ? row_inst[row] : 0;
+    };
+    auto cone_inst = [&](int32_t c) -> int32_t { return inst_of(cone_idx[c]); };
     const int64_t target = rowptr ? cfg.nnz_target : kTileRows;
     // linear rows
     int32_t r = 0;
     while (r < lin_end) {
         int32_t e = r + 1;
         int64_t nz = nnz_of(r, e);
-        while (e < lin_end && e - r < kTileRows && nz + nnz_of(e, e + 1) <= target) {
+        while (e < lin_end && e - r < kTileRows && nz + nnz_of(e, e + 1) <= target && inst_of(e) == inst_of(r)) {
             nz += nnz_of(e, e + 1);
             ++e;
         }
-        Tile t{r, e, TILE_LINEAR, choose_g(nz, e - r, cfg), -1, -1, 1, 0};
+        Tile t{r, e, TILE_LINEAR, choose_g(nz, e - r, cfg), -1, -1, 1, inst_of(r)};
         P.tiles.push_back(t);
         r = e;
     }
@@ -62,7 +66,7 @@ HostPlan build_plan(const int32_t *rowptr, int32_t nrows, int32_t lin_end, const
                     nz += nnz_of(b, b + 1);
                     ++b;
                 }
-                Tile t{a, b, TILE_HUGE, choose_g(nz, b - a, cfg), h, P.nslots + k, 1, 0};
+                Tile t{a, b, TILE_HUGE, choose_g(nz, b - a, cfg), h, P.nslots + k, 1, inst_of(s)};
                 huge.push_back(t);
                 ++k;
                 a = b;
@@ -81,6 +85,7 @@ HostPlan build_plan(const int32_t *rowptr, int32_t nrows, int32_t lin_end, const
             const int32_t sz1 = e1 - s1;
             if (sz1 > kTileRows || nnz_of(s1, e1) > huge_nnz) break;
             if (c1 > c && (rows + sz1 > kTileRows || nz + nnz_of(s1, e1) > target)) break;
+            if (c1 > c && cone_inst(c1) != cone_inst(c)) break;
             rows += sz1;
             nz += nnz_of(s1, e1);
             maxsz = std::max(maxsz, sz1);
@@ -88,7 +93,7 @@ HostPlan build_plan(const int32_t *rowptr, int32_t nrows, int32_t lin_end, const
         }
         const int32_t rb = cone_idx[c], re = cone_idx[c1];
         const int32_t g2 = std::min<int32_t>(std::max<int32_t>(pow2ceil(maxsz), 1), 64);
-        Tile t{rb, re, TILE_CONES, choose_g(nz, std::max(rows, 1), cfg), c, c1, g2, 0};
+        Tile t{rb, re, TILE_CONES, choose_g(nz, std::max(rows, 1), cfg), c, c1, g2, cone_inst(c)};
         P.tiles.push_back(t);
         c = c1;
     }

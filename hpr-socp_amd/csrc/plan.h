@@ -30,8 +30,10 @@ struct PlanConfig {
     bool stream = true;            // CSR-stream tiles when the tile fits kStreamNnz
 };
 
-// rowptr may be null (no SpMV: rows weigh one unit each).
+// rowptr may be null (no SpMV: rows weigh one unit each).  row_inst (null for
+// a single problem) gives each row's instance; tiles never straddle two
+// instances and carry the instance in Tile::inst.
 HostPlan build_plan(const int32_t *rowptr, int32_t nrows, int32_t lin_end, const int32_t *cone_idx,
-                    int32_t ncones, const PlanConfig &cfg);
+                    int32_t ncones, const PlanConfig &cfg, const int32_t *row_inst = nullptr);
 
 }  // namespace hpr

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <string>
 #include <vector>
@@ -152,6 +153,25 @@ struct IterView {
     HPR_SCRATCH_N(HPR_FIELD) HPR_SCRATCH_M(HPR_FIELD)
 #undef HPR_FIELD
 };
+// One problem instance inside the block-diagonal layout (half-open ranges).
+// Rows: linear rows of all instances first, then their SOC rows; columns:
+// boxed columns of all instances, then their SOC columns (the canonical order
+// of a single problem, applied to the block-diagonal union).
+struct InstRange {
+    int32_t lin0, lin1, con0, con1;  // rows
+    int32_t box0, box1, var0, var1;  // columns
+    int32_t kc0, kc1, kv0, kv1;      // constraint / variable cones (indices into con_idx / var_idx)
+    int32_t m() const { return (lin1 - lin0) + (con1 - con0); }
+    int32_t n() const { return (box1 - box0) + (var1 - var0); }
+};
+
+// Host-side scalars of one instance.
+struct InstHost {
+    double norm_b = 0, norm_c = 0, norm_b_org = 0, norm_c_org = 0, obj_const = 0;
+    double lambda_Q_diag = 0;  // max Q entry (diagonal Q), from check_Q_diagonal
+    int32_t Q_is_diag = 0;
+};
+
 struct hprsocp_handle {
     std::string err;
     hprsocp_params params{};
@@ -160,6 +180,18 @@ struct hprsocp_handle {
     int32_t m = 0, n = 0, m1 = 0, n1 = 0, ncon = 0, nvar = 0;
     int64_t nnzA = 0, nnzQ = 0;
     std::vector<void *> allocs;
+    // instances (1 for a single problem)
+    int32_t nb = 1;
+    std::vector<InstRange> rng;
+    std::vector<InstHost> ih;
+    InstScal *isc = nullptr;          // device, nb
+    InstScal *isc_h = nullptr;        // pinned host mirror
+    SegPlan seg_n, seg_m;             // per-instance chunks of columns / rows
+    int32_t max_chunks = 1;
+    int32_t *cinst_con = nullptr, *cinst_var = nullptr;  // instance of each cone
+    int32_t *ccnt_con = nullptr, *ccnt_var = nullptr;    // cone count of that instance
+    int32_t *flag = nullptr, *flag_h = nullptr;          // per-instance flags (restart copies)
+    std::vector<int32_t> h_row_inst, h_col_inst;
     // matrices
     CsrView A{}, AT{}, Q{}, F{}, FT{};
     double *A_v = nullptr, *AT_v = nullptr, *Q_v = nullptr, *Q_v2 = nullptr;
@@ -182,18 +214,17 @@ struct hprsocp_handle {
     double *fact1 = nullptr, *fact2 = nullptr, *fact = nullptr, *fact_m = nullptr;
     double *raw_n = nullptr, *raw_m = nullptr, *part_n = nullptr, *part_m = nullptr;
     double *cone_t = nullptr, *cone_ns = nullptr;
-    double *red_part = nullptr;  // reduction partials
-    int red_blocks = 1;
-    double *dscal = nullptr;     // device scalar slots
+    double *red_part = nullptr;  // reduction partials, 3 regions of red_region doubles
+    int64_t red_region = 0;
+    double *dscal = nullptr;     // device scalar slots, nb x SL_COUNT
     double *hscal = nullptr;     // pinned host mirror
+    double *ivtmp = nullptr, *ivtmp_h = nullptr;  // per-instance scratch values (2 x nb)
     PowerState *pstate = nullptr;
     double *tmp_m = nullptr, *tmp_n = nullptr, *tmp_n2 = nullptr;
-    // scalars
-    double sigma = 1, lambda_A = 0, lambda_Q = 0, b_scale = 1, c_scale = 1, obj_const = 0;
-    double norm_b = 0, norm_c = 0, norm_b_org = 0, norm_c_org = 0;
     bool lambda_set = false;
     double power_time = 0, setup_time = 0;
     int to_check = 1;
+    int64_t kstep = 0;  // updates since InstScal::inner0 was set
     // FP32 mirrors of the iteration (precision FP32_REFINE)
     bool fp32 = false;         // mirrors allocated
     bool fp32_active = false;  // updates run on the mirrors
@@ -204,6 +235,7 @@ struct hprsocp_handle {
     bool profile = false;
     std::vector<hipEvent_t> ev;
 
+    InstScal &I(int i) { return isc_h[i]; }
     template <class T>
     T *dalloc(int64_t count) {
         void *p = nullptr;
@@ -227,6 +259,9 @@ struct hprsocp_handle {
         if (st) (void)hipStreamSynchronize(st);
         for (void *p : allocs) (void)hipFree(p);
         if (hscal) (void)hipHostFree(hscal);
+        if (isc_h) (void)hipHostFree(isc_h);
+        if (flag_h) (void)hipHostFree(flag_h);
+        if (ivtmp_h) (void)hipHostFree(ivtmp_h);
         for (auto e : ev) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -269,27 +304,48 @@ void upload_plan(hprsocp_handle *h, const HostPlan &hp, DevPlan &dp) {
 }
 
 DevPlan make_plan(hprsocp_handle *h, const std::vector<int32_t> &rp, int32_t nrows, int32_t lin_end,
-                  const std::vector<int32_t> &cidx, int32_t ncones) {
+                  const std::vector<int32_t> &cidx, int32_t ncones, const std::vector<int32_t> *row_inst = nullptr) {
     PlanConfig cfg;
+    const int32_t *ri = (row_inst && h->nb > 1) ? row_inst->data() : nullptr;
     HostPlan hp = build_plan(rp.empty() ? nullptr : rp.data(), nrows, lin_end, cidx.empty() ? nullptr : cidx.data(),
-                             ncones, cfg);
+                             ncones, cfg, ri);
     DevPlan dp;
     upload_plan(h, hp, dp);
     return dp;
 }
 
 double *slot(hprsocp_handle *h, int k) { return h->dscal + k; }
+// slot k of instance i on the device / in the host mirror after read_slots
+InstVal islot(hprsocp_handle *h, int k) { return InstVal{h->dscal + k, SL_COUNT}; }
+double hs(hprsocp_handle *h, int i, int k) { return h->hscal[(int64_t)i * SL_COUNT + k]; }
 
-// copy device slots [k, k+cnt) to host and wait
-const double *read_slots(hprsocp_handle *h, int k, int cnt) {
-    HIPCHK(hipMemcpyAsync(h->hscal + k, h->dscal + k, sizeof(double) * cnt, hipMemcpyDeviceToHost, h->st));
+// copy every instance's device slots to the host mirror and wait
+void read_slots(hprsocp_handle *h) {
+    HIPCHK(hipMemcpyAsync(h->hscal, h->dscal, sizeof(double) * SL_COUNT * h->nb, hipMemcpyDeviceToHost, h->st));
     h->sync();
-    return h->hscal + k;
+}
+// publish the per-instance scalars (synchronous: the pinned mirror may change right after)
+void upload_isc(hprsocp_handle *h) {
+    HIPCHK(hipMemcpyAsync(h->isc, h->isc_h, sizeof(InstScal) * h->nb, hipMemcpyHostToDevice, h->st));
+    h->sync();
+}
+// per-instance scratch values: ivtmp[k*nb + i]
+InstVal ivals(hprsocp_handle *h, int k, const std::vector<double> &v) {
+    memcpy(h->ivtmp_h + (int64_t)k * h->nb, v.data(), sizeof(double) * h->nb);
+    HIPCHK(hipMemcpyAsync(h->ivtmp + (int64_t)k * h->nb, h->ivtmp_h + (int64_t)k * h->nb, sizeof(double) * h->nb,
+                          hipMemcpyHostToDevice, h->st));
+    h->sync();
+    return InstVal{h->ivtmp + (int64_t)k * h->nb, 1};
+}
+InstVal isc_field(hprsocp_handle *h, const double *field0) {
+    return InstVal{field0, (int32_t)(sizeof(InstScal) / sizeof(double))};
 }
 
-void reduce(hprsocp_handle *h, int64_t n, std::initializer_list<RedTerm> terms, int out_slot) {
+double *region(hprsocp_handle *h, int r) { return h->red_part + (int64_t)r * h->red_region; }
+
+void reduce(hprsocp_handle *h, const SegPlan &sp, std::initializer_list<RedTerm> terms, int out_slot, int reg = 0) {
     std::vector<RedTerm> t(terms);
-    launch_reduce(h->st, n, t.data(), (int)t.size(), h->red_part, reduce_blocks(n), slot(h, out_slot));
+    launch_reduce(h->st, sp, t.data(), (int)t.size(), region(h, reg), slot(h, out_slot), SL_COUNT);
 }
 
 // Q v into out (general Q; Q must have quadratic terms)
@@ -313,10 +369,12 @@ void equilibrate(hprsocp_handle *h, int use_sum, int phase, double *trow, double
     launch_row_stat(st, h->AT, qv, use_sum, tcol);
     if (h->m > 0) {
         launch_row_stat(st, h->A, nullptr, use_sum, trow);
-        launch_block_scale(st, h->ncon, h->con_idx, trow, h->params.soc_block_scaling_strategy, phase, LOC_CONSTRAINT);
+        launch_block_scale(st, h->ncon, h->con_idx, h->ccnt_con, trow, h->params.soc_block_scaling_strategy, phase,
+                           LOC_CONSTRAINT);
         launch_clip(st, h->m, trow);
     }
-    launch_block_scale(st, h->nvar, h->var_idx, tcol, h->params.soc_block_scaling_strategy, phase, LOC_VARIABLE);
+    launch_block_scale(st, h->nvar, h->var_idx, h->ccnt_var, tcol, h->params.soc_block_scaling_strategy, phase,
+                       LOC_VARIABLE);
     launch_clip(st, h->n, tcol);
     launch_scale_rows(st, h->m, trow, h->AL, h->AU, h->rhs, h->row_norm);
     launch_scale_cols(st, h->n, tcol, h->c, h->l, h->u, h->col_norm);
@@ -327,65 +385,80 @@ void equilibrate(hprsocp_handle *h, int use_sum, int phase, double *trow, double
     }
 }
 
-// bound norms: returns (l2 of max(|AL_nInf|,|AU_nInf|), inf-norm of same)
-void bound_norms(hprsocp_handle *h, double *l2, double *linf) {
-    if (h->m == 0) {
-        *l2 = *linf = 0.0;
-        return;
+// per instance: l2 and inf norm of max(|AL_nInf|,|AU_nInf|)
+void bound_norms(hprsocp_handle *h, std::vector<double> &l2, std::vector<double> &linf) {
+    l2.assign(h->nb, 0.0);
+    linf.assign(h->nb, 0.0);
+    if (h->m == 0) return;
+    launch_bound_stats(h->st, h->seg_m, h->AL, h->AU, region(h, 0), slot(h, SL_GEN), SL_COUNT);
+    read_slots(h);
+    for (int i = 0; i < h->nb; ++i) {
+        if (h->rng[i].m() == 0) continue;
+        l2[i] = sqrt(hs(h, i, SL_GEN));
+        linf[i] = hs(h, i, SL_GEN + 1);
     }
-    launch_bound_stats(h->st, h->m, h->AL, h->AU, h->red_part, reduce_blocks(h->m), slot(h, SL_GEN));
-    const double *r = read_slots(h, SL_GEN, 2);
-    *l2 = sqrt(r[0]);
-    *linf = r[1];
 }
-// [sumsq, count, absmax] of finite entries
-void finite_stats(hprsocp_handle *h, int64_t n, const double *v, double out[3]) {
-    if (n <= 0) {
-        out[0] = out[1] = out[2] = 0;
-        return;
-    }
-    launch_finite_stats(h->st, n, v, h->red_part, reduce_blocks(n), slot(h, SL_GEN));
-    const double *r = read_slots(h, SL_GEN, 3);
-    out[0] = r[0];
-    out[1] = r[1];
-    out[2] = r[2];
+// per instance [sumsq, count, absmax] of the finite entries of an m- or n-vector
+void finite_stats(hprsocp_handle *h, const SegPlan &sp, const double *v, std::vector<std::array<double, 3>> &out) {
+    launch_finite_stats(h->st, sp, v, region(h, 0), slot(h, SL_GEN), SL_COUNT);
+    read_slots(h);
+    out.resize(h->nb);
+    for (int i = 0; i < h->nb; ++i) out[i] = {hs(h, i, SL_GEN), hs(h, i, SL_GEN + 1), hs(h, i, SL_GEN + 2)};
 }
-double soc_rhs_inf(hprsocp_handle *h) {
-    if (h->ncon == 0 || h->m - h->m1 == 0) return 0.0;
-    reduce(h, h->m - h->m1, {RedTerm{h->rhs + h->m1, nullptr, nullptr, 1}}, SL_GEN);
-    return read_slots(h, SL_GEN, 1)[0];
+// per instance absmax of soc_rhs (zero on linear rows)
+std::vector<double> soc_rhs_inf(hprsocp_handle *h) {
+    std::vector<double> r(h->nb, 0.0);
+    if (h->m == 0) return r;
+    reduce(h, h->seg_m, {RedTerm{h->rhs, nullptr, nullptr, 1}}, SL_GEN);
+    read_slots(h);
+    for (int i = 0; i < h->nb; ++i) r[i] = (h->rng[i].con1 > h->rng[i].con0) ? hs(h, i, SL_GEN) : 0.0;
+    return r;
 }
-double vec_absmax(hprsocp_handle *h, int64_t n, const double *v) {
-    if (n <= 0) return 0.0;
-    reduce(h, n, {RedTerm{v, nullptr, nullptr, 1}}, SL_GEN);
-    return read_slots(h, SL_GEN, 1)[0];
+std::vector<double> vec_absmax_n(hprsocp_handle *h, const double *v) {
+    std::vector<double> r(h->nb, 0.0);
+    if (h->n == 0) return r;
+    reduce(h, h->seg_n, {RedTerm{v, nullptr, nullptr, 1}}, SL_GEN);
+    read_slots(h);
+    for (int i = 0; i < h->nb; ++i) r[i] = hs(h, i, SL_GEN);
+    return r;
 }
-double vec_nrm2(hprsocp_handle *h, int64_t n, const double *v) {
-    if (n <= 0) return 0.0;
-    reduce(h, n, {RedTerm{v, v, nullptr, 0}}, SL_GEN);
-    return sqrt(read_slots(h, SL_GEN, 1)[0]);
+std::vector<double> vec_nrm2_n(hprsocp_handle *h, const double *v) {
+    std::vector<double> r(h->nb, 0.0);
+    if (h->n == 0) return r;
+    reduce(h, h->seg_n, {RedTerm{v, v, nullptr, 0}}, SL_GEN);
+    read_slots(h);
+    for (int i = 0; i < h->nb; ++i) r[i] = sqrt(hs(h, i, SL_GEN));
+    return r;
 }
 
 // scaling!, reference src/utils/scaling.jl:785-1289 (CPU-path semantics: no
 // clipping before the SOC block aggregation; A^T scaled bit-identically to A).
+// Row/column equilibration is local to each instance of a block-diagonal
+// batch; the b/c scales and all norms are per instance.
 void do_scaling(hprsocp_handle *h) {
     hipStream_t st = h->st;
-    const int32_t m = h->m, n = h->n;
+    const int32_t m = h->m, n = h->n, nb = h->nb;
     launch_fill(st, m, 1.0, h->row_norm);
     launch_fill(st, n, 1.0, h->col_norm);
-    h->b_scale = h->c_scale = 1.0;
-    double bl2, blinf;
-    bound_norms(h, &bl2, &blinf);
-    const double rinf = soc_rhs_inf(h);
+    for (int i = 0; i < nb; ++i) h->I(i).b_scale = h->I(i).c_scale = 1.0;
+    std::vector<double> bl2, blinf;
+    bound_norms(h, bl2, blinf);
+    std::vector<double> rinf = soc_rhs_inf(h);
+    std::vector<double> cinf = vec_absmax_n(h, h->c);
+    for (int i = 0; i < nb; ++i) {
+        const bool hm = h->rng[i].m() > 0;
+        h->ih[i].norm_b_org = hm ? jmax(blinf[i], rinf[i]) : 0.0;
+        h->ih[i].norm_c_org = cinf[i];
+    }
     if (h->q_kind == HPRSOCP_Q_FACTORED) {  // operator Q: skip ALL scaling (scaling.jl:797-846)
-        h->norm_b_org = m > 0 ? jmax(blinf, rinf) : 0.0;
-        h->norm_c_org = vec_absmax(h, n, h->c);
-        h->norm_b = m > 0 ? jmax(bl2, rinf) : 0.0;
-        h->norm_c = vec_nrm2(h, n, h->c);
+        std::vector<double> cn = vec_nrm2_n(h, h->c);
+        for (int i = 0; i < nb; ++i) {
+            h->ih[i].norm_b = h->rng[i].m() > 0 ? jmax(bl2[i], rinf[i]) : 0.0;
+            h->ih[i].norm_c = cn[i];
+        }
+        upload_isc(h);
         return;
     }
-    h->norm_b_org = jmax(blinf, rinf);
-    h->norm_c_org = vec_absmax(h, n, h->c);
     double *trow = h->tmp_m, *tcol = h->tmp_n;
     launch_fill(st, m, 1.0, trow);
     launch_fill(st, n, 1.0, tcol);
@@ -398,87 +471,132 @@ void do_scaling(hprsocp_handle *h) {
     if (p.use_bc_scaling) {
         // compute_bc_scales, scaling.jl:721-760
         const int t = p.bc_scaling_norm_type;
-        auto finite_norm = [&](const double *v, int64_t len) {
-            double f[3];
-            finite_stats(h, len, v, f);
+        std::vector<std::array<double, 3>> fAL, fAU, fr, fc;
+        if (m > 0) {
+            finite_stats(h, h->seg_m, h->AL, fAL);
+            finite_stats(h, h->seg_m, h->AU, fAU);
+            finite_stats(h, h->seg_m, h->rhs, fr);
+        }
+        finite_stats(h, h->seg_n, h->c, fc);
+        auto finite_norm = [&](const std::array<double, 3> &f) {
             if (t == HPRSOCP_BC_LINF) return f[2];
             if (f[1] == 0) return 0.0;
             return t == HPRSOCP_BC_RMS ? sqrt(f[0] / f[1]) : sqrt(f[0]);
         };
-        auto vec_norm = [&](const double *v, int64_t len) {
+        auto vec_norm = [&](const std::array<double, 3> &f, int64_t len) {
             if (len == 0) return 0.0;
-            double f[3];
-            finite_stats(h, len, v, f);
             if (t == HPRSOCP_BC_LINF) return f[2];
             return t == HPRSOCP_BC_RMS ? sqrt(f[0] / (double)len) : sqrt(f[0]);
         };
-        double hb = 1.0;
-        hb = jmax(hb, finite_norm(h->AL, m));
-        hb = jmax(hb, finite_norm(h->AU, m));
-        hb = jmax(hb, (h->ncon == 0 || m - h->m1 == 0) ? 0.0 : vec_norm(h->rhs + h->m1, m - h->m1));
-        hb = jclamp(hb, 1e-4, 1e4);
-        const double hc = jclamp(jmax(1.0, vec_norm(h->c, n)), 1e-4, 1e4);
-        const double bs = hb, cs = jclamp(hc, hb / 1e2, hb / 1e-2);
-        if (h->nnzQ > 0) launch_scale_scalar(st, h->nnzQ, h->Q_v, bs / cs, 1.0);
-        if (m > 0) {
-            launch_scale_scalar(st, m, h->AL, 1.0, bs);
-            launch_scale_scalar(st, m, h->AU, 1.0, bs);
-            launch_scale_scalar(st, m, h->rhs, 1.0, bs);
+        std::vector<double> qmul(nb), bdiv(nb), cdiv(nb);
+        for (int i = 0; i < nb; ++i) {
+            const InstRange &R = h->rng[i];
+            double hb = 1.0;
+            if (R.m() > 0) {
+                hb = jmax(hb, finite_norm(fAL[i]));
+                hb = jmax(hb, finite_norm(fAU[i]));
+                const int32_t ncr = R.con1 - R.con0;
+                hb = jmax(hb, (R.kc1 == R.kc0 || ncr == 0) ? 0.0 : vec_norm(fr[i], ncr));
+            }
+            hb = jclamp(hb, 1e-4, 1e4);
+            const double hc = jclamp(jmax(1.0, vec_norm(fc[i], R.n())), 1e-4, 1e4);
+            const double bs = hb, cs = jclamp(hc, hb / 1e2, hb / 1e-2);
+            h->I(i).b_scale = bs;
+            h->I(i).c_scale = cs;
+            qmul[i] = bs / cs;
+            bdiv[i] = bs;
+            cdiv[i] = cs;
         }
-        launch_scale_scalar(st, n, h->c, 1.0, cs);
-        launch_scale_scalar(st, n, h->l, 1.0, bs);
-        launch_scale_scalar(st, n, h->u, 1.0, bs);
-        h->b_scale = bs;
-        h->c_scale = cs;
+        const InstVal one = ivals(h, 0, std::vector<double>(nb, 1.0));
+        if (h->nnzQ > 0) {
+            const InstVal qm = ivals(h, 1, qmul);
+            launch_scale_rows_inst(st, h->seg_n, h->Q.rowptr, h->Q_v, qm);
+        }
+        const InstVal bd = ivals(h, 1, bdiv);
+        if (m > 0) {
+            launch_scale_scalar(st, h->seg_m, h->AL, one, bd);
+            launch_scale_scalar(st, h->seg_m, h->AU, one, bd);
+            launch_scale_scalar(st, h->seg_m, h->rhs, one, bd);
+        }
+        launch_scale_scalar(st, h->seg_n, h->l, one, bd);
+        launch_scale_scalar(st, h->seg_n, h->u, one, bd);
+        const InstVal cd = ivals(h, 1, cdiv);  // (synchronises: the previous uses of slot 1 are done)
+        launch_scale_scalar(st, h->seg_n, h->c, one, cd);
+        h->sync();
     }
     // Q = (Q + Q')/2 (scaling.jl:1265-1269)
     if (h->nnzQ > 0) {
         launch_sym_q(st, h->nnzQ, h->Q_v, h->Q_tpos, h->Q_v2);
         launch_copy(st, h->nnzQ, h->Q_v2, h->Q_v);
     }
-    bound_norms(h, &bl2, &blinf);
-    h->norm_b = jmax(bl2, soc_rhs_inf(h));
-    h->norm_c = vec_nrm2(h, n, h->c);
+    bound_norms(h, bl2, blinf);
+    rinf = soc_rhs_inf(h);
+    std::vector<double> cn = vec_nrm2_n(h, h->c);
+    for (int i = 0; i < nb; ++i) {
+        h->ih[i].norm_b = jmax(bl2[i], rinf[i]);
+        h->ih[i].norm_c = cn[i];
+    }
+    upload_isc(h);
 }
 
 void update_q_factors(hprsocp_handle *h) {
-    launch_q_factors(h->st, h->n, h->diag_Q, h->sigma, h->fact2, h->fact, h->fact1, h->fact_m);
+    launch_q_factors(h->st, h->seg_n, h->diag_Q, h->isc, h->fact2, h->fact, h->fact1, h->fact_m);
 }
 
 // ------------------------------------------------------------- power iteration
-double power_iteration(hprsocp_handle *h, bool onA) {
+// power_iteration_A/Q (eigenvalues.jl:68-196) for every instance at once; each
+// instance starts from its own start vector and stops on its own test.
+std::vector<double> power_iteration(hprsocp_handle *h, bool onA) {
+    const int nb = h->nb;
     const int64_t len = onA ? h->m : h->n;
-    std::vector<double> z0((size_t)len);
-    start_vector(len, z0.data());
+    const SegPlan &sp = onA ? h->seg_m : h->seg_n;
+    std::vector<double> z0((size_t)len, 0.0);
+    for (int i = 0; i < nb; ++i) {
+        const InstRange &R = h->rng[i];
+        const int32_t a0 = onA ? R.lin0 : R.box0, a1 = onA ? R.lin1 : R.box1;
+        const int32_t b0 = onA ? R.con0 : R.var0, b1 = onA ? R.con1 : R.var1;
+        const int64_t li = (int64_t)(a1 - a0) + (b1 - b0);
+        std::vector<double> zi((size_t)li);
+        start_vector(li, zi.data());
+        std::copy(zi.begin(), zi.begin() + (a1 - a0), z0.begin() + a0);
+        std::copy(zi.begin() + (a1 - a0), zi.end(), z0.begin() + b0);
+    }
     double *z = onA ? h->tmp_m : h->tmp_n;
     double *q = onA ? h->ax : h->tmp_n2;
     HIPCHK(hipMemcpyAsync(z, z0.data(), sizeof(double) * len, hipMemcpyHostToDevice, h->st));
-    PowerState ps0{1.0, 1.0, 0, 0};
-    HIPCHK(hipMemcpyAsync(h->pstate, &ps0, sizeof(PowerState), hipMemcpyHostToDevice, h->st));
-    const int32_t *done = &h->pstate->done;
+    std::vector<PowerState> ps((size_t)nb, PowerState{1.0, 1.0, 0, 0});
+    for (int i = 0; i < nb; ++i) {
+        const InstRange &R = h->rng[i];
+        if ((onA ? R.m() : R.n()) == 0) ps[i].done = 1, ps[i].lambda = 0.0;
+    }
+    HIPCHK(hipMemcpyAsync(h->pstate, ps.data(), sizeof(PowerState) * nb, hipMemcpyHostToDevice, h->st));
+    h->sync();
     const int max_iter = 5000, batch = 10;
-    PowerState ps{};
     for (int it = 0; it < max_iter; it += batch) {
         for (int k = 0; k < batch; ++k) {
-            reduce(h, len, {RedTerm{z, z, nullptr, 0}}, SL_PW);
-            launch_scale_by_rnorm(h->st, len, z, slot(h, SL_PW), q, done);
+            reduce(h, sp, {RedTerm{z, z, nullptr, 0}}, SL_PW);
+            launch_scale_by_rnorm(h->st, sp, z, islot(h, SL_PW), q, h->pstate);
             if (onA) {
                 launch_spmv(h->st, h->plan_AT, h->AT, q, h->tempv);
                 launch_spmv(h->st, h->plan_A, h->A, h->tempv, z);
             } else {
                 qmap(h, q, z);
             }
-            reduce(h, len, {RedTerm{q, z, nullptr, 0}}, SL_PW + 1);
-            launch_power_residual(h->st, len, z, slot(h, SL_PW + 1), q, done);
-            reduce(h, len, {RedTerm{q, q, nullptr, 0}, RedTerm{z, z, nullptr, 0}}, SL_PW + 2);
-            launch_power_check(h->st, slot(h, SL_PW + 1), slot(h, SL_PW + 2), slot(h, SL_PW + 3), h->pstate,
+            reduce(h, sp, {RedTerm{q, z, nullptr, 0}}, SL_PW + 1);
+            launch_power_residual(h->st, sp, z, islot(h, SL_PW + 1), q, h->pstate);
+            reduce(h, sp, {RedTerm{q, q, nullptr, 0}, RedTerm{z, z, nullptr, 0}}, SL_PW + 2);
+            launch_power_check(h->st, nb, islot(h, SL_PW + 1), islot(h, SL_PW + 2), islot(h, SL_PW + 3), h->pstate,
                                1e-4, max_iter);
         }
-        HIPCHK(hipMemcpyAsync(&ps, h->pstate, sizeof(PowerState), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(hipMemcpyAsync(ps.data(), h->pstate, sizeof(PowerState) * nb, hipMemcpyDeviceToHost, h->st));
         h->sync();
-        if (ps.done) break;
+        bool all = true;
+        for (int i = 0; i < nb; ++i) all = all && ps[i].done;
+        if (all) break;
     }
-    return ps.lambda;
+    std::vector<double> lam((size_t)nb);
+    for (int i = 0; i < nb; ++i) lam[i] = ps[i].lambda;
+    return lam;
 }
 
 // ------------------------------------------------------------- XCD slicing
@@ -569,13 +687,14 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
 // with its epilogue plan (nnz-balanced tiles for the plain form, row tiles for
 // the sliced form).  HPRSOCP_XCD_SLICES=<P> forces P slices (tests).
 void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm, const double *dval,
-                 int32_t lin_end, const std::vector<int32_t> &cidx, int32_t ncones, const char *name) {
+                 int32_t lin_end, const std::vector<int32_t> &cidx, int32_t ncones, const char *name,
+                 const std::vector<int32_t> *row_inst) {
     const char *env = getenv("HPRSOCP_XCD_SLICES");
     int32_t P = env ? std::max(1, atoi(env)) : (hm.ncols * (int64_t)sizeof(double) > (2ll << 20) ? 8 : 1);
     if (P <= 1 || hm.ci.empty()) return;
     CsrView S = M;
     if (!build_slices(h, S, hm, dval, P)) return;
-    DevPlan splan = make_plan(h, {}, hm.nrows, lin_end, cidx, ncones);
+    DevPlan splan = make_plan(h, {}, hm.nrows, lin_end, cidx, ncones, row_inst);
     if (!env) {
         double *xin = h->dalloc<double>(hm.ncols);
         double *yout = h->dalloc<double>(hm.nrows);
@@ -639,6 +758,60 @@ int validate_csr(std::string &err, const char *name, int32_t nr, int32_t nc, int
             return HPRSOCP_ERR_INVALID;
         }
     return 0;
+}
+
+// ------------------------------------------------------------- instances
+SegPlan make_seg(hprsocp_handle *h, bool cols) {
+    std::vector<SegChunk> ch;
+    std::vector<int32_t> first;
+    for (int i = 0; i < h->nb; ++i) {
+        first.push_back((int32_t)ch.size());
+        const InstRange &R = h->rng[i];
+        const int32_t a[2][2] = {{cols ? R.box0 : R.lin0, cols ? R.box1 : R.lin1},
+                                 {cols ? R.var0 : R.con0, cols ? R.var1 : R.con1}};
+        for (int part = 0; part < 2; ++part)
+            for (int32_t s0 = a[part][0]; s0 < a[part][1]; s0 += kChunk)
+                ch.push_back(SegChunk{s0, std::min(s0 + kChunk, a[part][1]), i, 0});
+    }
+    first.push_back((int32_t)ch.size());
+    SegPlan sp;
+    sp.nchunks = (int32_t)ch.size();
+    sp.ninst = h->nb;
+    sp.chunks = h->upload<SegChunk>(ch.data(), (int64_t)ch.size());
+    sp.first = h->upload<int32_t>(first.data(), (int64_t)first.size());
+    return sp;
+}
+
+// Per-instance device state: chunk plans, cone -> instance maps, scalars, slots.
+void setup_instances(hprsocp_handle *h) {
+    const int nb = h->nb;
+    h->seg_n = make_seg(h, true);
+    h->seg_m = make_seg(h, false);
+    h->max_chunks = std::max(1, std::max(h->seg_n.nchunks, h->seg_m.nchunks));
+    std::vector<int32_t> ci((size_t)std::max(h->ncon, 1), 0), cc((size_t)std::max(h->ncon, 1), h->ncon);
+    std::vector<int32_t> vi((size_t)std::max(h->nvar, 1), 0), vc((size_t)std::max(h->nvar, 1), h->nvar);
+    for (int i = 0; i < nb; ++i) {
+        const InstRange &R = h->rng[i];
+        for (int32_t k = R.kc0; k < R.kc1; ++k) ci[k] = i, cc[k] = R.kc1 - R.kc0;
+        for (int32_t k = R.kv0; k < R.kv1; ++k) vi[k] = i, vc[k] = R.kv1 - R.kv0;
+    }
+    h->cinst_con = h->upload<int32_t>(ci.data(), (int64_t)ci.size());
+    h->ccnt_con = h->upload<int32_t>(cc.data(), (int64_t)cc.size());
+    h->cinst_var = h->upload<int32_t>(vi.data(), (int64_t)vi.size());
+    h->ccnt_var = h->upload<int32_t>(vc.data(), (int64_t)vc.size());
+    h->isc = h->dalloc<InstScal>(nb);
+    HIPCHK(hipHostMalloc((void **)&h->isc_h, sizeof(InstScal) * nb, hipHostMallocDefault));
+    for (int i = 0; i < nb; ++i) h->isc_h[i] = InstScal{1.0, 0.0, 0.0, 1.0, 1.0, 0.0, 1, 0};
+    h->flag = h->dalloc<int32_t>(nb);
+    HIPCHK(hipHostMalloc((void **)&h->flag_h, sizeof(int32_t) * nb, hipHostMallocDefault));
+    h->ivtmp = h->dalloc<double>(2 * (int64_t)nb);
+    HIPCHK(hipHostMalloc((void **)&h->ivtmp_h, sizeof(double) * 2 * nb, hipHostMallocDefault));
+    h->dscal = h->dalloc<double>((int64_t)SL_COUNT * nb);
+    HIPCHK(hipHostMalloc((void **)&h->hscal, sizeof(double) * SL_COUNT * nb, hipHostMallocDefault));
+    h->pstate = h->dalloc<PowerState>(nb);
+    h->red_region = (int64_t)std::max(h->max_chunks, nb) * kMaxGolden;
+    h->red_part = h->dalloc<double>(3 * h->red_region);
+    upload_isc(h);
 }
 
 // ------------------------------------------------------------- precision views
@@ -766,7 +939,10 @@ const char *hprsocp_last_error(const hprsocp_handle *h) { return h ? h->err.c_st
 
 void hprsocp_destroy(hprsocp_handle *h) { delete h; }
 
-static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, hprsocp_handle *h) {
+// `ranges`/`objc` describe the instances of a block-diagonal batch (null: the
+// problem is one instance).
+static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, hprsocp_handle *h,
+                       const std::vector<InstRange> *ranges = nullptr, const std::vector<double> *objc = nullptr) {
     const double t0 = now_s();
     if (params) h->params = *params;
     else hprsocp_default_params(&h->params);
@@ -794,7 +970,6 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     if (!p->c || !p->l || !p->u || (m > 0 && (!p->AL || !p->AU))) throw Err{HPRSOCP_ERR_INVALID, "missing vector"};
     h->m = m;
     h->n = n;
-    h->obj_const = p->obj_constant;
     h->m1 = p->number_linear_con;
     h->n1 = p->number_lu_x;
     h->ncon = p->n_soc_con;
@@ -819,6 +994,25 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     for (int32_t i = 0; i < h->nvar; ++i)
         if (h->h_var_idx[i + 1] < h->h_var_idx[i]) throw Err{HPRSOCP_ERR_INVALID, "soc_var_idx not monotone"};
 
+    // ---- instances
+    if (ranges) {
+        h->rng = *ranges;
+    } else {
+        h->rng.assign(1, InstRange{0, h->m1, h->m1, m, 0, h->n1, h->n1, n, 0, h->ncon, 0, h->nvar});
+    }
+    h->nb = (int32_t)h->rng.size();
+    h->ih.assign(h->nb, InstHost{});
+    for (int i = 0; i < h->nb; ++i) h->ih[i].obj_const = objc ? (*objc)[i] : p->obj_constant;
+    h->h_row_inst.assign((size_t)m, 0);
+    h->h_col_inst.assign((size_t)n, 0);
+    for (int i = 0; i < h->nb; ++i) {
+        const InstRange &R = h->rng[i];
+        for (int32_t r = R.lin0; r < R.lin1; ++r) h->h_row_inst[r] = i;
+        for (int32_t r = R.con0; r < R.con1; ++r) h->h_row_inst[r] = i;
+        for (int32_t c = R.box0; c < R.box1; ++c) h->h_col_inst[c] = i;
+        for (int32_t c = R.var0; c < R.var1; ++c) h->h_col_inst[c] = i;
+    }
+
     // ---- device
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
@@ -830,6 +1024,7 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     h->device = P.device_number;
     HIPCHK(hipSetDevice(h->device));
     HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    setup_instances(h);
 
     // ---- host-side matrices: A, A^T, Q (symmetrized, canonical), F, F^T
     HostCsr A;
@@ -919,7 +1114,7 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
                     break;
                 }
         h->q_struct_diag = sd;
-        h->plan_Q = make_plan(h, Qh.rp, n, n, {}, 0);
+        h->plan_Q = make_plan(h, Qh.rp, n, n, {}, 0, &h->h_col_inst);
     }
     HostCsr F, FT;
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
@@ -945,8 +1140,8 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
         h->plan_FT = make_plan(h, FT.rp, n, n, {}, 0);
     }
     // plans: A rows (linear | constraint cones), A^T rows (box | variable cones)
-    h->plan_A = make_plan(h, A.rp, m, h->m1, h->h_con_idx, h->ncon);
-    h->plan_AT = make_plan(h, AT.rp, n, h->n1, h->h_var_idx, h->nvar);
+    h->plan_A = make_plan(h, A.rp, m, h->m1, h->h_con_idx, h->ncon, &h->h_row_inst);
+    h->plan_AT = make_plan(h, AT.rp, n, h->n1, h->h_var_idx, h->nvar, &h->h_col_inst);
     {
         std::vector<int32_t> rc((size_t)m, -1);
         for (int32_t k = 0; k < h->ncon; ++k)
@@ -977,40 +1172,47 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     h->part_m = h->dalloc<double>(h->plan_A.nslots);
     h->cone_t = h->dalloc<double>(h->ncon);
     h->cone_ns = h->dalloc<double>(h->ncon);
-    h->red_blocks = 1024;
-    h->red_part = h->dalloc<double>((int64_t)h->red_blocks * kMaxGolden);
-    h->dscal = h->dalloc<double>(SL_COUNT);
-    HIPCHK(hipHostMalloc((void **)&h->hscal, sizeof(double) * SL_COUNT, hipHostMallocDefault));
-    h->pstate = (PowerState *)h->dalloc<double>(4);
 
     do_scaling(h);
     // XCD column slices of every SpMV operator, kept where they measure faster
     // (values are final after scaling)
-    tune_slices(h, h->A, h->plan_A, A, h->A_v, h->m1, h->h_con_idx, h->ncon, "A");
-    tune_slices(h, h->AT, h->plan_AT, AT, h->AT_v, h->n1, h->h_var_idx, h->nvar, "A^T");
+    tune_slices(h, h->A, h->plan_A, A, h->A_v, h->m1, h->h_con_idx, h->ncon, "A", &h->h_row_inst);
+    tune_slices(h, h->AT, h->plan_AT, AT, h->AT_v, h->n1, h->h_var_idx, h->nvar, "A^T", &h->h_col_inst);
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
-        tune_slices(h, h->F, h->plan_F, F, h->F.val, F.nrows, {}, 0, "F");
-        tune_slices(h, h->FT, h->plan_FT, FT, h->FT.val, n, {}, 0, "F^T");
+        tune_slices(h, h->F, h->plan_F, F, h->F.val, F.nrows, {}, 0, "F", nullptr);
+        tune_slices(h, h->FT, h->plan_FT, FT, h->FT.val, n, {}, 0, "F^T", nullptr);
     } else if (h->nnzQ > 0 && !h->q_struct_diag) {
-        tune_slices(h, h->Q, h->plan_Q, Qh, h->Q_v, n, {}, 0, "Q");
+        tune_slices(h, h->Q, h->plan_Q, Qh, h->Q_v, n, {}, 0, "Q", &h->h_col_inst);
     }
-    // check_Q_diagonal (scaling.jl:1312-1353), after scaling as in solve()
+    // check_Q_diagonal (scaling.jl:1312-1353), after scaling as in solve(); every
+    // instance of a batch must land in the same class
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
         h->Q_is_diag = 0;
     } else if (h->nnzQ == 0) {
         h->Q_is_diag = 1;
     } else {
-        launch_q_diag(h->st, n, h->Q.rowptr, h->Q.col, h->Q_v, h->diag_Q, h->red_part, reduce_blocks(n),
-                      slot(h, SL_GEN));
-        h->Q_is_diag = read_slots(h, SL_GEN, 2)[0] == 0.0;
+        launch_q_diag(h->st, h->seg_n, h->Q.rowptr, h->Q.col, h->Q_v, h->diag_Q, region(h, 0), slot(h, SL_GEN),
+                      SL_COUNT);
+        read_slots(h);
+        for (int i = 0; i < h->nb; ++i) {
+            h->ih[i].Q_is_diag = hs(h, i, SL_GEN) == 0.0;
+            h->ih[i].lambda_Q_diag = hs(h, i, SL_GEN + 2);
+        }
+        h->Q_is_diag = h->ih[0].Q_is_diag;
+        for (int i = 1; i < h->nb; ++i)
+            if (h->ih[i].Q_is_diag != h->Q_is_diag)
+                throw Err{HPRSOCP_ERR_INVALID, "batch instances must all have a diagonal Q or all a general Q"};
     }
     // sigma_0 (allocate_workspace, algorithm.jl:1538-1551)
-    if (P.sigma == -1.0) {
-        const double nb = h->norm_b, nc = h->norm_c;
-        h->sigma = (nc > 1e-16 && nb > 1e-16 && nb < 1e16 && nc < 1e16) ? nb / nc : 1.0;
-    } else {
-        h->sigma = P.sigma;
+    for (int i = 0; i < h->nb; ++i) {
+        if (P.sigma == -1.0) {
+            const double nbv = h->ih[i].norm_b, nc = h->ih[i].norm_c;
+            h->I(i).sigma = (nc > 1e-16 && nbv > 1e-16 && nbv < 1e16 && nc < 1e16) ? nbv / nc : 1.0;
+        } else {
+            h->I(i).sigma = P.sigma;
+        }
     }
+    upload_isc(h);
     if (h->Q_is_diag) update_q_factors(h);
     if (P.precision == HPRSOCP_PREC_FP32_REFINE) {
         setup_fp32(h);
@@ -1053,67 +1255,89 @@ int hprsocp_create(const hprsocp_problem *prob, const hprsocp_params *params, hp
 int hprsocp_estimate_lambda(hprsocp_handle *h) {
     ABI_TRY(h, {
         const double t0 = now_s();
-        h->lambda_A = h->m > 0 ? power_iteration(h, true) * 1.05 : 0.0;
-        if (h->q_kind == HPRSOCP_Q_FACTORED) {
-            h->lambda_Q = power_iteration(h, false) * 1.05;
-        } else if (h->nnzQ > 0) {
-            if (!h->Q_is_diag) h->lambda_Q = power_iteration(h, false) * 1.05;
-            else {
-                reduce(h, h->nnzQ, {RedTerm{h->Q_v, nullptr, nullptr, 3}}, SL_GEN);
-                h->lambda_Q = read_slots(h, SL_GEN, 1)[0];
-            }
-        } else {
-            h->lambda_Q = 0.0;
+        const int nb = h->nb;
+        std::vector<double> la(nb, 0.0), lq(nb, 0.0);
+        if (h->m > 0) {
+            la = power_iteration(h, true);
+            for (double &v : la) v *= 1.05;
         }
+        if (h->q_kind == HPRSOCP_Q_FACTORED || (h->nnzQ > 0 && !h->Q_is_diag)) {
+            lq = power_iteration(h, false);
+            for (double &v : lq) v *= 1.05;
+        } else if (h->nnzQ > 0) {
+            for (int i = 0; i < nb; ++i) lq[i] = std::isfinite(h->ih[i].lambda_Q_diag) ? h->ih[i].lambda_Q_diag : 0.0;
+        }
+        for (int i = 0; i < nb; ++i) {
+            h->I(i).lambda_A = la[i];
+            h->I(i).lambda_Q = lq[i];
+        }
+        upload_isc(h);
         h->lambda_set = true;
         h->power_time = now_s() - t0;
     });
     return 0;
 }
 
-int hprsocp_set_initial(hprsocp_handle *h, const double *x0, const double *y0) {
-    ABI_TRY(h, {
-        hipStream_t st = h->st;
-        const int32_t m = h->m, n = h->n;
-        if (x0) {
-            HIPCHK(hipMemcpyAsync(h->tmp_n, x0, sizeof(double) * n, hipMemcpyHostToDevice, st));
-            launch_scale_vd(st, n, h->tmp_n, h->col_norm, h->b_scale, h->x);
-            launch_copy(st, n, h->x, h->x_bar);
-            launch_copy(st, n, h->x, h->last_x);
-            if (h->has_quad) {
-                launch_copy(st, n, h->x, h->w);
-                launch_copy(st, n, h->x, h->w_bar);
-                launch_copy(st, n, h->x, h->last_w);
-            }
-        }
-        if (y0) {
-            HIPCHK(hipMemcpyAsync(h->tmp_m, y0, sizeof(double) * m, hipMemcpyHostToDevice, st));
-            launch_scale_vd(st, m, h->tmp_m, h->row_norm, h->c_scale, h->y);
-            launch_copy(st, m, h->y, h->y_bar);
-            launch_copy(st, m, h->y, h->last_y);
-            if (m > 0) {
-                if (h->has_quad) {
-                    launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->aty_bar);
-                    launch_copy(st, n, h->aty_bar, h->aty);
-                    launch_copy(st, n, h->aty_bar, h->last_aty);
-                } else {
-                    launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->aty);
-                }
-            }
-            if (h->has_quad) {
-                qmap(h, h->x_bar, h->qx);
-                launch_init_zbar(st, n, h->x_bar, h->qx, h->aty_bar, h->c, h->sigma, h->z_bar);
-            } else {
-                launch_init_zbar(st, n, h->x_bar, nullptr, h->aty, h->c, h->sigma, h->z_bar);
-            }
-            if (m > 0) {
-                launch_spmv(st, h->plan_A, h->A, h->x_bar, h->ax);
-                launch_init_s(st, m, h->ax, h->y, h->AL, h->AU, h->lambda_A * h->sigma, h->s);
-            }
-        }
-        h->fp32_dirty = true;
+}  // extern "C"
+
+namespace {
+// process_initial_points! (algorithm.jl:2612-2677) for the instances flagged in
+// fx (x0 given) / fy (y0 given); x0/y0 are full-length (block-diagonal layout).
+void set_initial_impl(hprsocp_handle *h, const double *x0, const double *y0, const std::vector<int32_t> &fx,
+                      const std::vector<int32_t> &fy) {
+    hipStream_t st = h->st;
+    const int32_t m = h->m, n = h->n, nb = h->nb;
+    auto set_flags = [&](const std::vector<int32_t> &f) {
+        memcpy(h->flag_h, f.data(), sizeof(int32_t) * nb);
+        HIPCHK(hipMemcpyAsync(h->flag, h->flag_h, sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
         h->sync();
-    });
+    };
+    const InstVal bsv = isc_field(h, &h->isc->b_scale), csv = isc_field(h, &h->isc->c_scale);
+    if (x0) {
+        set_flags(fx);
+        HIPCHK(hipMemcpyAsync(h->tmp_n, x0, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        launch_scale_vd(st, h->seg_n, h->tmp_n, h->col_norm, bsv, h->tmp_n2);
+        double *dst[] = {h->x, h->x_bar, h->last_x, h->w, h->w_bar, h->last_w};
+        for (int k = 0; k < (h->has_quad ? 6 : 3); ++k) launch_copy_flagged(st, h->seg_n, h->tmp_n2, dst[k], h->flag);
+    }
+    if (y0) {
+        set_flags(fy);
+        HIPCHK(hipMemcpyAsync(h->tmp_m, y0, sizeof(double) * m, hipMemcpyHostToDevice, st));
+        launch_scale_vd(st, h->seg_m, h->tmp_m, h->row_norm, csv, h->rp);
+        launch_copy_flagged(st, h->seg_m, h->rp, h->y, h->flag);
+        launch_copy_flagged(st, h->seg_m, h->rp, h->y_bar, h->flag);
+        launch_copy_flagged(st, h->seg_m, h->rp, h->last_y, h->flag);
+        double *aty_dst = h->has_quad ? h->aty_bar : h->aty;
+        if (m > 0) {
+            launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->tmp_n2);
+            launch_copy_flagged(st, h->seg_n, h->tmp_n2, aty_dst, h->flag);
+            if (h->has_quad) {
+                launch_copy_flagged(st, h->seg_n, h->tmp_n2, h->aty, h->flag);
+                launch_copy_flagged(st, h->seg_n, h->tmp_n2, h->last_aty, h->flag);
+            }
+        }
+        if (h->has_quad) {
+            qmap(h, h->x_bar, h->qx);
+            launch_init_zbar(st, h->seg_n, h->x_bar, h->qx, h->aty_bar, h->c, h->isc, h->tmp_n2);
+        } else {
+            launch_init_zbar(st, h->seg_n, h->x_bar, nullptr, h->aty, h->c, h->isc, h->tmp_n2);
+        }
+        launch_copy_flagged(st, h->seg_n, h->tmp_n2, h->z_bar, h->flag);
+        if (m > 0) {
+            launch_spmv(st, h->plan_A, h->A, h->x_bar, h->ax);
+            launch_init_s(st, h->seg_m, h->ax, h->y, h->AL, h->AU, h->isc, h->rp);
+            launch_copy_flagged(st, h->seg_m, h->rp, h->s, h->flag);
+        }
+    }
+    h->fp32_dirty = true;
+    h->sync();
+}
+}  // namespace
+
+extern "C" {
+
+int hprsocp_set_initial(hprsocp_handle *h, const double *x0, const double *y0) {
+    ABI_TRY(h, { set_initial_impl(h, x0, y0, std::vector<int32_t>(h->nb, 1), std::vector<int32_t>(h->nb, 1)); });
     return 0;
 }
 
@@ -1131,37 +1355,33 @@ static void qmap_t(hprsocp_handle *h, IterView<T> &V, const T *v, T *out) {
 }
 
 // One HPR iteration (main_update_gpu!, algorithm.jl:2284-2309) on the
-// matrices and vectors of V; scalars are rounded to T.
+// matrices and vectors of V.  Scalars come per instance from h->isc (sigma,
+// lambdas, Halpern counter inner0 + kstep), rounded to T inside the kernels.
 template <typename T>
-static void do_update_t(hprsocp_handle *h, IterView<T> &V, int64_t inner, int full) {
+static void do_update_t(hprsocp_handle *h, IterView<T> &V, int64_t kstep, int full) {
     hipStream_t st = h->st;
-    const double h1d = 1.0 / ((double)inner + 2.0);
-    const T h1 = (T)h1d, h2 = (T)(1.0 - h1d), sig = (T)h->sigma;
     const int32_t m = h->m;
     auto ev = [&](int k) {
         if (h->profile) HIPCHK(hipEventRecord(h->ev[k], st));
     };
     if (!h->has_quad) {
         ev(0);
-        NoQPrimalArgsT<T> a{V.y, V.x, V.x_hat, V.x_bar, V.z_bar, V.dx, V.last_x, V.c, V.l, V.u, sig, h1, h2, full};
+        NoQPrimalArgsT<T> a{V.y, V.x, V.x_hat, V.x_bar, V.z_bar, V.dx, V.last_x, V.c, V.l, V.u, h->isc, kstep, full};
         launch_noq_primal<T>(st, h->plan_AT, V.AT, h->var_idx, a, V.raw_n, h->part_n);
         ev(1);
         if (m > 0) {
-            const double f1 = h->lambda_A * h->sigma;
-            DualArgsT<T> d{V.x_hat, V.y, V.y_bar, V.s, V.dy, V.last_y, V.AL, V.AU, V.rhs,
-                           (T)f1, (T)(1.0 / f1), h1, h2, full, 0};
+            DualArgsT<T> d{V.x_hat, V.y, V.y_bar, V.s, V.dy, V.last_y, V.AL, V.AU, V.rhs, h->isc, kstep, full, 0};
             launch_dual<T>(st, h->plan_A, V.A, h->con_idx, d, V.raw_m, h->part_m);
         }
         ev(2);
         return;
     }
-    const double f2s = 1.0 / (1.0 + h->sigma * h->lambda_Q), f1s = 1.0 - f2s;
     const int sdiag = h->q_struct_diag && h->q_kind == HPRSOCP_Q_CSR;
     ev(0);
     if (!sdiag) qmap_t<T>(h, V, V.w, V.qw);
     ev(1);
     QPPrimalArgsT<T> a{V.x, V.x_hat, V.x_bar, V.z_bar, V.dx, V.w_bar, V.tempv, V.w, V.last_x, V.c,
-                       V.l, V.u, V.aty, V.qw, V.diag_Q, V.fact1, V.fact2, sig, h1, h2, (T)f1s, (T)f2s,
+                       V.l, V.u, V.aty, V.qw, V.diag_Q, V.fact1, V.fact2, h->isc, kstep,
                        full, sdiag, h->Q_is_diag};
     launch_qp_primal<T>(st, h->plan_AT, h->var_idx, a, V.raw_n, h->part_n);
     ev(2);
@@ -1169,41 +1389,48 @@ static void do_update_t(hprsocp_handle *h, IterView<T> &V, int64_t inner, int fu
         if (!sdiag) {
             if (h->q_kind == HPRSOCP_Q_FACTORED) {
                 launch_spmv<T>(st, h->plan_F, V.F, V.w_bar, V.Ftmp);
-                TempvArgsT<T> t{V.x_hat, V.qw, V.tempv, V.Fdiag, V.w_bar, sig};
+                TempvArgsT<T> t{V.x_hat, V.qw, V.tempv, V.Fdiag, V.w_bar, h->isc};
                 launch_spmv_tempv<T>(st, h->plan_FT, V.FT, V.Ftmp, t);
             } else {
-                TempvArgsT<T> t{V.x_hat, V.qw, V.tempv, nullptr, nullptr, sig};
+                TempvArgsT<T> t{V.x_hat, V.qw, V.tempv, nullptr, nullptr, h->isc};
                 launch_spmv_tempv<T>(st, h->plan_Q, V.Q, V.w_bar, t);
             }
         }
         ev(3);
-        const double f1 = h->lambda_A * h->sigma;
-        DualArgsT<T> d{V.tempv, V.y, V.y_bar, V.s, V.dy, V.last_y, V.AL, V.AU, V.rhs,
-                       (T)f1, (T)(1.0 / f1), h1, h2, full, 1};
+        DualArgsT<T> d{V.tempv, V.y, V.y_bar, V.s, V.dy, V.last_y, V.AL, V.AU, V.rhs, h->isc, kstep, full, 1};
         launch_dual<T>(st, h->plan_A, V.A, h->con_idx, d, V.raw_m, h->part_m);
     } else {
         ev(3);
     }
     ev(4);
     W2ArgsT<T> w2{V.y_bar, V.w, V.w_bar, V.aty, V.dw, V.atdy, V.aty_bar, V.last_w, V.last_aty, V.fact,
-                  (T)(h->sigma / (1.0 + h->sigma * h->lambda_Q)), h1, h2, full, h->Q_is_diag};
+                  h->isc, kstep, full, h->Q_is_diag};
     launch_w2<T>(st, h->plan_AT, V.AT, w2);
     ev(5);
 }
 }  // extern "C++"
 
+// Sets every instance's Halpern counter (InstScal::inner0) and restarts the
+// step count of the following updates.
+static void set_inner(hprsocp_handle *h, const std::vector<int64_t> &inner) {
+    for (int i = 0; i < h->nb; ++i) h->I(i).inner0 = (double)inner[i];
+    upload_isc(h);
+    h->kstep = 0;
+}
+
 // One iteration in the handle's current precision.  In FP32 the mirrors are
 // refreshed from FP64 first if the FP64 state changed, and the FP64 state is
 // refreshed from the mirrors after a full (check) iteration or when `sync64`.
-static void do_update(hprsocp_handle *h, int64_t inner, int full, bool sync64 = false) {
+static void do_update(hprsocp_handle *h, int full, bool sync64 = false) {
     h->to_check = full;
+    const int64_t k = h->kstep++;
     if (h->fp32_active) {
         if (h->fp32_dirty) demote_state(h);
-        do_update_t<float>(h, h->v32, inner, full);
+        do_update_t<float>(h, h->v32, k, full);
         if (full || sync64) promote_state(h);
     } else {
         IterView<double> V = view64(h);
-        do_update_t<double>(h, V, inner, full);
+        do_update_t<double>(h, V, k, full);
     }
 }
 
@@ -1213,59 +1440,68 @@ int hprsocp_update(hprsocp_handle *h, int64_t inner, int32_t to_check) {
         return HPRSOCP_ERR_STATE;
     }
     ABI_TRY(h, {
-        do_update(h, inner, to_check ? 1 : 0, true);
+        set_inner(h, std::vector<int64_t>(h->nb, inner));
+        do_update(h, to_check ? 1 : 0, true);
         HIPCHK(hipGetLastError());
     });
     return 0;
 }
 
-// compute_residuals! (algorithm.jl:113-196)
+// compute_residuals! (algorithm.jl:113-196) for every instance; r has nb entries.
 static void residuals_impl(hprsocp_handle *h, int64_t iter, hprsocp_residuals *r) {
     hipStream_t st = h->st;
     const int32_t m = h->m, n = h->n;
-    const double bs = h->b_scale, cs = h->c_scale;
     const bool skip_q = !h->has_quad;
     if (!skip_q) qmap(h, h->x_bar, h->qx);
     if (m > 0) launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->atdy);
     else launch_fill(st, n, 0.0, h->atdy);
-    const int nbn = reduce_blocks(n);
-    ResidNArgs an{h->c, h->x_bar, h->z_bar, h->col_norm, skip_q ? nullptr : h->qx, h->atdy, h->rd, cs};
-    launch_resid_n(st, n, an, h->red_part, nbn, slot(h, SL_RN));
+    ResidNArgs an{h->c, h->x_bar, h->z_bar, h->col_norm, skip_q ? nullptr : h->qx, h->atdy, h->rd, h->isc};
+    launch_resid_n(st, h->seg_n, an, region(h, 0), slot(h, SL_RN), SL_COUNT);
     if (m > 0) {
         launch_spmv(st, h->plan_A, h->A, h->x_bar, h->ax);
-        launch_ax_to_org(st, m, h->ax, h->row_norm, bs);
-        launch_cone_rp_stats(st, h->ncon, h->con_idx, h->ax, h->rhs, h->row_norm, bs, h->cone_t, h->cone_ns);
+        launch_ax_to_org(st, h->seg_m, h->ax, h->row_norm, h->isc);
+        launch_cone_rp_stats(st, h->ncon, h->con_idx, h->cinst_con, h->ax, h->rhs, h->row_norm, h->isc, h->cone_t,
+                             h->cone_ns);
         ResidMArgs am{h->y_bar, h->s, h->rhs, h->AL, h->AU, h->row_norm, h->ax, h->rp, h->row_cone, h->con_idx,
-                      h->cone_t, h->cone_ns, bs, h->m1};
-        launch_resid_m(st, m, am, h->red_part + (int64_t)h->red_blocks * kMaxRed * 2, reduce_blocks(m),
-                       slot(h, SL_RM));
+                      h->cone_t, h->cone_ns, h->isc, h->m1};
+        launch_resid_m(st, h->seg_m, am, region(h, 1), slot(h, SL_RM), SL_COUNT);
     }
     if (iter == 0)
-        launch_bounds_violation(st, n, h->nvar, h->var_idx, h->x_bar, h->l, h->u, h->col_norm, bs, h->dx,
-                                h->red_part + (int64_t)h->red_blocks * kMaxRed * 4, reduce_blocks(n),
-                                slot(h, SL_BV));
-    const double *v = read_slots(h, 0, SL_BV + 1);
-    const double *rn = v + SL_RN, *rm = v + SL_RM;
-    r->primal_obj = bs * cs * (rn[0] + 0.5 * rn[1]) + h->obj_const;
-    r->dual_obj = bs * cs * (-0.5 * rn[1] + rn[2]) + h->obj_const;
-    if (m > 0) r->dual_obj += bs * cs * (rm[0] + rm[1]);
-    r->rel_gap = fabs(r->primal_obj - r->dual_obj) / (1.0 + jmax(fabs(r->primal_obj), fabs(r->dual_obj)));
-    const double qx_inf = skip_q ? 0.0 : rn[5];
-    r->err_Rd = rn[3] / (1.0 + jmax(jmax(h->norm_c_org, rn[4]), qx_inf));
-    if (m > 0) {
-        const double denom = 1.0 + jmax(h->norm_b_org, rm[4]);
-        r->err_Rp = jmax(rm[2], rm[3]) / denom;
-        r->err_Rp_linear = h->m1 > 0 ? rm[2] / denom : 0.0;
-        r->err_Rp_soc = h->m1 < m ? rm[3] / denom : 0.0;
-    } else {
-        r->err_Rp = r->err_Rp_linear = r->err_Rp_soc = 0.0;
+        launch_bounds_violation(st, h->seg_n, h->nvar, h->var_idx, h->cinst_var, h->x_bar, h->l, h->u, h->col_norm,
+                                h->isc, h->dx, region(h, 2), slot(h, SL_BV), SL_COUNT);
+    read_slots(h);
+    for (int i = 0; i < h->nb; ++i) {
+        const double bs = h->I(i).b_scale, cs = h->I(i).c_scale;
+        const InstHost &H = h->ih[i];
+        const bool hm = h->rng[i].m() > 0;
+        const double *v = h->hscal + (int64_t)i * SL_COUNT;
+        const double *rn = v + SL_RN, *rm = v + SL_RM;
+        hprsocp_residuals &R = r[i];
+        R.primal_obj = bs * cs * (rn[0] + 0.5 * rn[1]) + H.obj_const;
+        R.dual_obj = bs * cs * (-0.5 * rn[1] + rn[2]) + H.obj_const;
+        if (hm) R.dual_obj += bs * cs * (rm[0] + rm[1]);
+        R.rel_gap = fabs(R.primal_obj - R.dual_obj) / (1.0 + jmax(fabs(R.primal_obj), fabs(R.dual_obj)));
+        const double qx_inf = skip_q ? 0.0 : rn[5];
+        R.err_Rd = rn[3] / (1.0 + jmax(jmax(H.norm_c_org, rn[4]), qx_inf));
+        if (hm) {
+            const double denom = 1.0 + jmax(H.norm_b_org, rm[4]);
+            R.err_Rp = jmax(rm[2], rm[3]) / denom;
+            R.err_Rp_linear = h->rng[i].lin1 > h->rng[i].lin0 ? rm[2] / denom : 0.0;
+            R.err_Rp_soc = h->rng[i].con1 > h->rng[i].con0 ? rm[3] / denom : 0.0;
+        } else {
+            R.err_Rp = R.err_Rp_linear = R.err_Rp_soc = 0.0;
+        }
+        if (iter == 0) R.err_Rp = jmax(R.err_Rp, v[SL_BV]);
+        R.kkt = jmax(jmax(R.err_Rp, R.err_Rd), R.rel_gap);
     }
-    if (iter == 0) r->err_Rp = jmax(r->err_Rp, v[SL_BV]);
-    r->kkt = jmax(jmax(r->err_Rp, r->err_Rd), r->rel_gap);
 }
 
 int hprsocp_compute_residuals(hprsocp_handle *h, int64_t iter, hprsocp_residuals *out) {
-    ABI_TRY(h, { residuals_impl(h, iter, out); });
+    ABI_TRY(h, {
+        std::vector<hprsocp_residuals> r(h->nb);
+        residuals_impl(h, iter, r.data());
+        *out = r[0];
+    });
     return 0;
 }
 
@@ -1304,14 +1540,25 @@ int hprsocp_probe_copy(hprsocp_handle *h, int64_t bytes, double *ms) {
 namespace {
 
 // Enqueues the reductions of compute_M_norm! (algorithm.jl:1302-1359) into 8
-// device slots starting at `base`; no host synchronisation.
+// device slots per instance starting at `base`; no host synchronisation.
 struct MnormCtx {
-    double sigma, lambda_A, lambda_Q;
+    std::vector<double> sigma, lambda_A, lambda_Q;
     bool pending = false;
+    void snap(hprsocp_handle *h) {
+        sigma.resize(h->nb);
+        lambda_A.resize(h->nb);
+        lambda_Q.resize(h->nb);
+        for (int i = 0; i < h->nb; ++i) {
+            sigma[i] = h->I(i).sigma;
+            lambda_A[i] = h->I(i).lambda_A;
+            lambda_Q[i] = h->I(i).lambda_Q;
+        }
+        pending = true;
+    }
 };
 
 void mnorm_enqueue(hprsocp_handle *h, int base) {
-    const int32_t m = h->m, n = h->n;
+    const int32_t m = h->m;
     const bool q = h->has_quad;
     hipStream_t st = h->st;
     if (q) qmap(h, h->dw, h->dqw);
@@ -1331,24 +1578,20 @@ void mnorm_enqueue(hprsocp_handle *h, int base) {
         tn.push_back(m > 0 ? RedTerm{h->atdy, h->qatdy, nullptr, 0} : RedTerm{h->dx, h->dx, nullptr, 0});
         tn.push_back(RedTerm{h->dw, h->dqw, nullptr, 0});
     }
-    launch_reduce(st, n, tn.data(), (int)tn.size(), h->red_part, reduce_blocks(n), slot(h, base));
-    if (m > 0) {
-        RedTerm ty{h->dy, h->dy, nullptr, 0};
-        launch_reduce(st, m, &ty, 1, h->red_part + (int64_t)h->red_blocks * kMaxRed * 2, reduce_blocks(m),
-                      slot(h, base + 7));  // d2
-    } else {
-        launch_fill(st, 1, 0.0, slot(h, base + 7));
-    }
+    launch_reduce(st, h->seg_n, tn.data(), (int)tn.size(), region(h, 0), slot(h, base), SL_COUNT);
+    RedTerm ty{h->dy, h->dy, nullptr, 0};
+    launch_reduce(st, h->seg_m, &ty, 1, region(h, 1), slot(h, base + 7), SL_COUNT);  // d2
 }
 
-double mnorm_finish(hprsocp_handle *h, const double *d, const MnormCtx &c) {
+// M-norm of instance i from its 8 reduced dots d (host copy)
+double mnorm_finish(hprsocp_handle *h, int i, const double *d, const MnormCtx &c) {
     const bool q = h->has_quad;
-    const int32_t m = h->m;
-    const double sig = c.sigma;
+    const bool hm = h->rng[i].m() > 0;
+    const double sig = c.sigma[i];
     double M1 = 0.0, M2 = (1.0 / sig) * d[0], M3 = 0.0;
     if (q) M2 -= 2.0 * d[1];
-    if (m > 0) {
-        M1 = sig * c.lambda_A * d[7];
+    if (hm) {
+        M1 = sig * c.lambda_A[i] * d[7];
         if (q) M1 -= 2.0 * sig * d[2];
         M2 += 2.0 * d[3];
         if (q) {
@@ -1356,12 +1599,12 @@ double mnorm_finish(hprsocp_handle *h, const double *d, const MnormCtx &c) {
                 M3 = d[4];
                 M1 += sig * d[5];
             } else {
-                M3 = (sig * sig) / (1.0 + sig * c.lambda_Q) * d[4];
-                M1 += sig * c.lambda_Q * d[5];
+                M3 = (sig * sig) / (1.0 + sig * c.lambda_Q[i]) * d[4];
+                M1 += sig * c.lambda_Q[i] * d[5];
             }
         }
     } else if (!h->Q_is_diag && q) {
-        M1 = sig * c.lambda_Q * d[5];
+        M1 = sig * c.lambda_Q[i] * d[5];
     }
     M2 += jmax(M1, 0.0);
     const double Mn = jmax(M2, 0.0) + jmax(M3, 0.0);
@@ -1450,29 +1693,47 @@ double golden_scalar(double ap, double bp, double cp, double dp, double lo, doub
     return 0.5 * (a + b);
 }
 
-// unified_golden_Q_diag (unified_operations.jl:411-449) with every evaluation of
-// g(x) = sum_i ATdy_i * QATdy_i / (1 + x d_i) batched on the device.
-double golden_q_diag(hprsocp_handle *h, double a, double b, double lo, double hi, double tol) {
-    const int nb = std::min(reduce_blocks(h->n), 1024);
-    auto eval = [&](const std::vector<double> &xs, std::vector<double> &gs) {
-        memcpy(h->hscal + SL_GOLD, xs.data(), sizeof(double) * xs.size());
-        HIPCHK(hipMemcpyAsync(h->dscal + SL_GOLD, h->hscal + SL_GOLD, sizeof(double) * xs.size(),
-                              hipMemcpyHostToDevice, h->st));
-        launch_golden_eval(h->st, h->n, h->atdy, h->qatdy, h->diag_Q, h->dscal + SL_GOLD, (int)xs.size(),
-                           h->red_part, nb, h->dscal + SL_GOLD);
-        const double *r = read_slots(h, SL_GOLD, (int)xs.size());
-        gs.assign(r, r + xs.size());
-    };
-    return golden_speculative(a, b, lo, hi, tol, 5, eval);
+// unified_golden_Q_diag (unified_operations.jl:411-449) for the listed
+// instances at once: every evaluation of g(x) = sum_j ATdy_j * QATdy_j / (1 + x d_j)
+// of all their searches runs in one batched device pass.
+std::vector<double> golden_q_diag(hprsocp_handle *h, const std::vector<int> &ids, const std::vector<double> &a,
+                                  const std::vector<double> &b, double lo, double hi, double tol) {
+    std::vector<GoldenWalk> walks;
+    for (size_t k = 0; k < ids.size(); ++k) walks.emplace_back(a[k], b[k], lo, hi, tol, 5);
+    const int nb = h->nb;
+    std::vector<double> xs((size_t)nb * kMaxGolden, 1.0);
+    while (true) {
+        int npts = 0;
+        for (size_t k = 0; k < ids.size(); ++k) {
+            if (walks[k].done()) continue;
+            const std::vector<double> &p = walks[k].next_points();
+            std::copy(p.begin(), p.end(), xs.begin() + (int64_t)ids[k] * kMaxGolden);
+            npts = std::max(npts, (int)p.size());
+        }
+        if (npts == 0) break;
+        double *xd = region(h, 2);  // nb * kMaxGolden points (region 2 is free here)
+        HIPCHK(hipMemcpyAsync(xd, xs.data(), sizeof(double) * xs.size(), hipMemcpyHostToDevice, h->st));
+        launch_golden_eval(h->st, h->seg_n, h->atdy, h->qatdy, h->diag_Q, xd, npts, region(h, 0), slot(h, SL_GOLD),
+                           SL_COUNT);
+        read_slots(h);
+        for (size_t k = 0; k < ids.size(); ++k)
+            if (!walks[k].done()) walks[k].consume(h->hscal + (int64_t)ids[k] * SL_COUNT + SL_GOLD);
+    }
+    std::vector<double> r(ids.size());
+    for (size_t k = 0; k < ids.size(); ++k) r[k] = walks[k].result();
+    return r;
 }
 
-// update_sigma!, algorithm.jl:729-990
-void update_sigma(hprsocp_handle *h, Restart &r, const hprsocp_residuals &res) {
-    if (r.restart_flag < 1) return;
+// update_sigma!, algorithm.jl:729-990, for every active instance with a restart
+void update_sigma(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<hprsocp_residuals> &RS,
+                  const std::vector<char> &active) {
+    std::vector<int> ids;
+    for (int i = 0; i < h->nb; ++i)
+        if (active[i] && R[i].restart_flag >= 1) ids.push_back(i);
+    if (ids.empty()) return;
     hipStream_t st = h->st;
     const int32_t m = h->m, n = h->n;
-    const bool skip_q = !h->has_quad, has_m = m > 0, qd = h->Q_is_diag;
-    const double sigma_old = h->sigma;
+    const bool skip_q = !h->has_quad, any_m = m > 0, qd = h->Q_is_diag;
     const double SMIN = 1e-12, SMAX = 1e12, THETA_MAX = 1.0, IMB = 1e6, FLOOR = 2.0, THR = 1e-8, EPS = 1e-16;
     const double LOG100 = log(100.0);
     if (!skip_q) {
@@ -1480,7 +1741,7 @@ void update_sigma(hprsocp_handle *h, Restart &r, const hprsocp_residuals &res) {
         qmap(h, h->dw, h->dqw);
     }
     launch_axpby(st, n, 1.0, h->x_bar, -1.0, h->last_x, h->dx);
-    if (has_m) {
+    if (any_m) {
         launch_axpby(st, m, 1.0, h->y_bar, -1.0, h->last_y, h->dy);
         if (!skip_q) {
             launch_axpby(st, n, 1.0, h->aty_bar, -1.0, h->last_aty, h->atdy);
@@ -1492,124 +1753,168 @@ void update_sigma(hprsocp_handle *h, Restart &r, const hprsocp_residuals &res) {
     // n-side dots: b=|dx|^2, <dQw,ATdy>, |dQw|^2, <dw,dQw>, <ATdy,QATdy>
     std::vector<RedTerm> tn{RedTerm{h->dx, h->dx, nullptr, 0}};
     if (!skip_q) {
-        tn.push_back(has_m ? RedTerm{h->dqw, h->atdy, nullptr, 0} : RedTerm{h->dx, h->dx, nullptr, 0});
+        tn.push_back(any_m ? RedTerm{h->dqw, h->atdy, nullptr, 0} : RedTerm{h->dx, h->dx, nullptr, 0});
         tn.push_back(RedTerm{h->dqw, h->dqw, nullptr, 0});
         tn.push_back(RedTerm{h->dw, h->dqw, nullptr, 0});
-        tn.push_back(has_m ? RedTerm{h->atdy, h->qatdy, nullptr, 0} : RedTerm{h->dx, h->dx, nullptr, 0});
+        tn.push_back(any_m ? RedTerm{h->atdy, h->qatdy, nullptr, 0} : RedTerm{h->dx, h->dx, nullptr, 0});
     }
-    launch_reduce(st, n, tn.data(), (int)tn.size(), h->red_part, reduce_blocks(n), slot(h, SL_SIG));
-    if (has_m) {
-        RedTerm ty{h->dy, h->dy, nullptr, 0};
-        launch_reduce(st, m, &ty, 1, h->red_part + (int64_t)h->red_blocks * kMaxRed * 2, reduce_blocks(m),
-                      slot(h, SL_SIG + 7));
-    }
-    const double *d = read_slots(h, SL_SIG, 8);
-    const double dd_x = d[0], dd_y = has_m ? d[7] : 0.0;
-    double a = 0.0, b = dd_x, c = 0.0, dl = 0.0;
-    if (has_m) {
-        if (!skip_q) a = h->lambda_A * dd_y - 2.0 * d[1];
-        else a = h->lambda_A * dd_y;
-    }
-    if (!skip_q) {
-        if (qd) {
-            const double nq = sqrt(d[2]);
-            a += nq * nq;
-        } else {
-            a += h->lambda_Q * d[3];
-            if (has_m) {
-                c = d[4];
-                dl = h->lambda_Q;
+    launch_reduce(st, h->seg_n, tn.data(), (int)tn.size(), region(h, 0), slot(h, SL_SIG), SL_COUNT);
+    RedTerm ty{h->dy, h->dy, nullptr, 0};
+    launch_reduce(st, h->seg_m, &ty, 1, region(h, 1), slot(h, SL_SIG + 7), SL_COUNT);
+    read_slots(h);
+    // model sigma per instance
+    const size_t K = ids.size();
+    std::vector<double> sm(K), av(K), bv(K), cv(K), dv(K);
+    std::vector<int> gold_ids;
+    std::vector<double> ga, gb;
+    for (size_t k = 0; k < K; ++k) {
+        const int i = ids[k];
+        const double *d = h->hscal + (int64_t)i * SL_COUNT + SL_SIG;
+        const bool has_m = h->rng[i].m() > 0;
+        const double dd_x = d[0], dd_y = has_m ? d[7] : 0.0;
+        double a = 0.0, b = dd_x, c = 0.0, dl = 0.0;
+        if (has_m) {
+            if (!skip_q) a = h->I(i).lambda_A * dd_y - 2.0 * d[1];
+            else a = h->I(i).lambda_A * dd_y;
+        }
+        if (!skip_q) {
+            if (qd) {
+                const double nq = sqrt(d[2]);
+                a += nq * nq;
+            } else {
+                a += h->I(i).lambda_Q * d[3];
+                if (has_m) {
+                    c = d[4];
+                    dl = h->I(i).lambda_Q;
+                }
             }
         }
-    }
-    a = jmax(a, EPS);
-    b = jmax(b, EPS);
-    double sm;
-    if (skip_q) sm = sqrt(b / a);
-    else if (qd) sm = has_m ? golden_q_diag(h, a, b, 1e-12, 1e12, 1e-13) : sqrt(b / a);
-    else sm = has_m ? golden_scalar(a, b, c, dl, 1e-12, 1e12, 1e-13) : sqrt(b / a);
-
-    const double sref = r.best_sigma, ck = res.kkt, cg = r.current_gap, bk = r.best_kkt, bg = r.best_gap;
-    const bool ref_ok = (ck <= 10.0 * bk) && (cg <= 10.0 * bg);
-    double pr = sqrt((ck / bk) * (cg / bg));
-    pr = jclamp(pr, 1.0, 10.0);
-    double sbase = sm;
-    if (ref_ok) {
-        const double theta = jclamp(exp(-0.3 * (pr - 1.0)), 0.0, THETA_MAX);
-        sbase = exp(theta * log(sm) + (1.0 - theta) * log(sref));
-        sbase = jclamp(sbase, SMIN, SMAX);
-    }
-    const double ep = jmax(res.err_Rp, EPS), ed = jmax(res.err_Rd, EPS);
-    const double es = jmin(ep, ed), el = jmax(ep, ed), ir = el / es;
-    const bool both = el <= THR;
-    const bool one = (es <= 0.5 * THR) && (el >= 2.0 * THR);
-    const bool extreme = ir >= IMB;
-    const bool trig = !both && (one || extreme);
-    const bool keep = !both && ((es <= 0.8 * THR) || (ir >= 0.1 * IMB));
-    const int cdir = ed >= ep ? 1 : -1;
-    if (both) {
-        r.corr_active = false;
-        r.corr_hold = -1;
-        r.corr_dir = 0;
-    } else if (trig || (r.corr_active && keep)) {
-        r.corr_active = true;
-        r.corr_hold = 0;
-        r.corr_dir = cdir;
-    } else {
-        r.corr_active = false;
-        r.corr_hold = -1;
-        r.corr_dir = 0;
-    }
-    double beta = 0.0;
-    if (!both) {
-        if (one) beta = 0.5;
-        if (extreme) beta = 1.0;
-        else if (r.corr_active) beta = 0.5;
-    }
-    double kappa = 1.0;
-    if (beta > 0.0) {
-        double rk = ed / ep;
-        if (r.corr_active) {
-            if (r.corr_dir > 0) rk = jmax(rk, FLOOR);
-            else if (r.corr_dir < 0) rk = jmin(rk, 1.0 / FLOOR);
+        a = jmax(a, EPS);
+        b = jmax(b, EPS);
+        av[k] = a;
+        bv[k] = b;
+        cv[k] = c;
+        dv[k] = dl;
+        if (skip_q || !has_m) {
+            sm[k] = sqrt(b / a);
+        } else if (qd) {
+            gold_ids.push_back(i);
+            ga.push_back(a);
+            gb.push_back(b);
+            sm[k] = NAN;  // filled below
+        } else {
+            sm[k] = golden_scalar(a, b, c, dl, 1e-12, 1e12, 1e-13);
         }
-        kappa = exp(beta * jclamp(log(rk), -LOG100, LOG100));
     }
-    double sn = sbase * kappa;
-    if (r.corr_active) {
-        if (r.corr_dir > 0) sn = jmax(sn, sigma_old * 1.5);
-        else if (r.corr_dir < 0) sn = jmin(sn, sigma_old / 1.5);
+    if (!gold_ids.empty()) {
+        const std::vector<double> g = golden_q_diag(h, gold_ids, ga, gb, 1e-12, 1e12, 1e-13);
+        size_t q = 0;
+        for (size_t k = 0; k < K; ++k)
+            if (ids[k] == (q < gold_ids.size() ? gold_ids[q] : -1)) sm[k] = g[q++];
     }
-    sn = jclamp(sn, SMIN, SMAX);
-    h->sigma = sn;
-    r.best_iter += 1;
-    if (qd && !skip_q && fabs(sigma_old - sn) > 1e-15) update_q_factors(h);
+    bool changed = false;
+    for (size_t k = 0; k < K; ++k) {
+        const int i = ids[k];
+        Restart &r = R[i];
+        const hprsocp_residuals &res = RS[i];
+        const double sigma_old = h->I(i).sigma;
+        const double sref = r.best_sigma, ck = res.kkt, cg = r.current_gap, bk = r.best_kkt, bg = r.best_gap;
+        const bool ref_ok = (ck <= 10.0 * bk) && (cg <= 10.0 * bg);
+        double pr = sqrt((ck / bk) * (cg / bg));
+        pr = jclamp(pr, 1.0, 10.0);
+        double sbase = sm[k];
+        if (ref_ok) {
+            const double theta = jclamp(exp(-0.3 * (pr - 1.0)), 0.0, THETA_MAX);
+            sbase = exp(theta * log(sm[k]) + (1.0 - theta) * log(sref));
+            sbase = jclamp(sbase, SMIN, SMAX);
+        }
+        const double ep = jmax(res.err_Rp, EPS), ed = jmax(res.err_Rd, EPS);
+        const double es = jmin(ep, ed), el = jmax(ep, ed), ir = el / es;
+        const bool both = el <= THR;
+        const bool one = (es <= 0.5 * THR) && (el >= 2.0 * THR);
+        const bool extreme = ir >= IMB;
+        const bool trig = !both && (one || extreme);
+        const bool keep = !both && ((es <= 0.8 * THR) || (ir >= 0.1 * IMB));
+        const int cdir = ed >= ep ? 1 : -1;
+        if (both) {
+            r.corr_active = false;
+            r.corr_hold = -1;
+            r.corr_dir = 0;
+        } else if (trig || (r.corr_active && keep)) {
+            r.corr_active = true;
+            r.corr_hold = 0;
+            r.corr_dir = cdir;
+        } else {
+            r.corr_active = false;
+            r.corr_hold = -1;
+            r.corr_dir = 0;
+        }
+        double beta = 0.0;
+        if (!both) {
+            if (one) beta = 0.5;
+            if (extreme) beta = 1.0;
+            else if (r.corr_active) beta = 0.5;
+        }
+        double kappa = 1.0;
+        if (beta > 0.0) {
+            double rk = ed / ep;
+            if (r.corr_active) {
+                if (r.corr_dir > 0) rk = jmax(rk, FLOOR);
+                else if (r.corr_dir < 0) rk = jmin(rk, 1.0 / FLOOR);
+            }
+            kappa = exp(beta * jclamp(log(rk), -LOG100, LOG100));
+        }
+        double sn = sbase * kappa;
+        if (r.corr_active) {
+            if (r.corr_dir > 0) sn = jmax(sn, sigma_old * 1.5);
+            else if (r.corr_dir < 0) sn = jmin(sn, sigma_old / 1.5);
+        }
+        sn = jclamp(sn, SMIN, SMAX);
+        h->I(i).sigma = sn;
+        r.best_iter += 1;
+        if (fabs(sigma_old - sn) > 1e-15) changed = true;
+    }
+    upload_isc(h);
+    if (qd && !skip_q && changed) update_q_factors(h);
 }
 
-// do_restart!, algorithm.jl:1130-1166
-void do_restart(hprsocp_handle *h, Restart &r) {
-    if (r.restart_flag <= 0) return;
-    hipStream_t st = h->st;
-    const int32_t m = h->m, n = h->n;
-    launch_copy(st, n, h->x_bar, h->x);
-    launch_copy(st, n, h->x_bar, h->last_x);
-    if (h->has_quad) {
-        launch_copy(st, n, h->w_bar, h->w);
-        launch_copy(st, n, h->w_bar, h->last_w);
+// do_restart!, algorithm.jl:1130-1166, for the active instances with a restart flag
+bool do_restart(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<char> &active) {
+    bool any = false;
+    for (int i = 0; i < h->nb; ++i) {
+        h->flag_h[i] = active[i] && R[i].restart_flag > 0;
+        any = any || h->flag_h[i];
     }
-    if (m > 0) {
-        launch_copy(st, m, h->y_bar, h->y);
-        launch_copy(st, m, h->y_bar, h->last_y);
+    if (!any) return false;
+    hipStream_t st = h->st;
+    HIPCHK(hipMemcpyAsync(h->flag, h->flag_h, sizeof(int32_t) * h->nb, hipMemcpyHostToDevice, st));
+    const SegPlan &sn = h->seg_n, &sm = h->seg_m;
+    launch_copy_flagged(st, sn, h->x_bar, h->x, h->flag);
+    launch_copy_flagged(st, sn, h->x_bar, h->last_x, h->flag);
+    if (h->has_quad) {
+        launch_copy_flagged(st, sn, h->w_bar, h->w, h->flag);
+        launch_copy_flagged(st, sn, h->w_bar, h->last_w, h->flag);
+    }
+    if (h->m > 0) {
+        launch_copy_flagged(st, sm, h->y_bar, h->y, h->flag);
+        launch_copy_flagged(st, sm, h->y_bar, h->last_y, h->flag);
         if (h->has_quad) {
-            launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->aty_bar);
-            launch_copy(st, n, h->aty_bar, h->last_aty);
-            launch_copy(st, n, h->aty_bar, h->aty);
+            launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->tmp_n2);
+            launch_copy_flagged(st, sn, h->tmp_n2, h->aty_bar, h->flag);
+            launch_copy_flagged(st, sn, h->tmp_n2, h->last_aty, h->flag);
+            launch_copy_flagged(st, sn, h->tmp_n2, h->aty, h->flag);
         }
     }
-    r.last_gap = r.current_gap;
-    r.save_gap = INFINITY;
-    r.times += 1;
-    r.inner = 0;
+    h->sync();  // the flags' pinned mirror may change at the next check
+    for (int i = 0; i < h->nb; ++i) {
+        if (!h->flag_h[i]) continue;
+        Restart &r = R[i];
+        r.last_gap = r.current_gap;
+        r.save_gap = INFINITY;
+        r.times += 1;
+        r.inner = 0;
+    }
+    return true;
 }
 
 double print_step(int64_t iter) {
@@ -1624,143 +1929,200 @@ bool should_print(const hprsocp_params &p, int64_t iter, double t_start) {
     return (iter % p.print_frequency == 0) || iter == p.max_iter || (now_s() - t_start > p.time_limit);
 }
 
+// The solve loop (algorithm.jl:2765-2905) over every instance of the handle in
+// lockstep: one update launch per iteration serves all instances; residual
+// checks, restarts and sigma updates are per instance; an instance that stops
+// is retired (its tiles skip all work) and its result recorded.  res has nb
+// entries.
 void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
     const hprsocp_params &p = h->params;
     const double t_start = now_s();
     if (!h->lambda_set) hprsocp_estimate_lambda(h);
-    Restart r;
-    r.best_sigma = h->sigma;
-    if (h->Q_is_diag) update_q_factors(h);
-    if (p.verbose) {
-        printf("HPRSOCP SOLVER starts (MI355X HIP backend)...\n");
-        printf(" iter     errRp     errRd         p_obj           d_obj          gap        sigma       time\n");
+    const int nb = h->nb;
+    std::vector<Restart> R((size_t)nb);
+    for (int i = 0; i < nb; ++i) {
+        R[i].best_sigma = h->I(i).sigma;
+        h->I(i).active = 1;
     }
-    hprsocp_residuals rs{};
-    int64_t it4 = 0, it6 = 0;
-    double t4 = 0, t6 = 0;
-    bool first4 = true, first6 = true;
+    upload_isc(h);
+    if (h->Q_is_diag) update_q_factors(h);
+    const bool single = nb == 1;
+    if (p.verbose) {
+        printf("HPRSOCP SOLVER starts (MI355X HIP backend)%s...\n", single ? "" : " [batch]");
+        if (single) printf(" iter     errRp     errRd         p_obj           d_obj          gap        sigma       time\n");
+        else printf("batch of %d instances\n", nb);
+    }
+    std::vector<hprsocp_residuals> rs((size_t)nb);
+    std::vector<int64_t> it4(nb, 0), it6(nb, 0), inner(nb, 0);
+    std::vector<double> t4(nb, 0.0), t6(nb, 0.0);
+    std::vector<char> first4(nb, 1), first6(nb, 1), active(nb, 1);
+    int nactive = nb;
     const int64_t check_iter = p.check_iter;
     // deferred M-norms: slot 0 -> last_gap candidate, slot 1 -> current_gap
     MnormCtx mn[2];
+    std::vector<char> pend0(nb, 0), pend1(nb, 0);
     h->to_check = 1;
     // FP32 bulk: windows of FP32 iterations, FP64 checks; FP64 iterations take
-    // over once the KKT residual reaches refine_tol or stops improving
+    // over once every active instance reaches refine_tol or stops improving
     h->fp32_active = h->fp32;
     h->fp32_dirty = true;
-    double best32 = INFINITY;
-    int stall32 = 0;
-    int64_t iter32 = 0, restarts32 = 0;
+    std::vector<double> best32(nb, INFINITY);
+    std::vector<int> stall32(nb, 0);
+    std::vector<int64_t> restarts32(nb, 0);
+    int64_t iter32 = 0;
     for (int64_t iter = 0; iter <= p.max_iter; ++iter) {
         const bool print_yes = should_print(p, iter, t_start);
-        if (iter % check_iter == 0 || print_yes) residuals_impl(h, iter, &rs);  // synchronises the stream
-        if (iter % check_iter == 0 && (mn[0].pending || mn[1].pending)) {
+        const bool check = iter % check_iter == 0;
+        if (check || print_yes) residuals_impl(h, iter, rs.data());  // synchronises the stream
+        if (check && (mn[0].pending || mn[1].pending)) {
             // deferred M-norms (perform_iteration_step!, algorithm.jl:2731-2741); the gaps are
             // only consumed by check_restart at multiples of check_iter
-            read_slots(h, SL_MN0, 16);
-            for (int k = 0; k < 2; ++k)
-                if (mn[k].pending) {
-                    const double g = mnorm_finish(h, h->hscal + (k ? SL_MN1 : SL_MN0), mn[k]);
-                    if (k == 0) r.last_gap = jmin(g, r.last_gap);
-                    else r.current_gap = g;
-                    mn[k].pending = false;
+            read_slots(h);
+            for (int i = 0; i < nb; ++i) {
+                if (!active[i]) continue;
+                if (mn[0].pending && pend0[i])
+                    R[i].last_gap = jmin(mnorm_finish(h, i, h->hscal + (int64_t)i * SL_COUNT + SL_MN0, mn[0]),
+                                         R[i].last_gap);
+                if (mn[1].pending && pend1[i])
+                    R[i].current_gap = mnorm_finish(h, i, h->hscal + (int64_t)i * SL_COUNT + SL_MN1, mn[1]);
+            }
+            mn[0].pending = mn[1].pending = false;
+        }
+        const double tnow = now_s() - t_start;
+        std::vector<int> status(nb, HPRSOCP_STATUS_CONTINUE);
+        for (int i = 0; i < nb; ++i) {
+            if (!active[i]) continue;
+            if (rs[i].kkt < p.stoptol) status[i] = HPRSOCP_STATUS_OPTIMAL;
+            else if (iter == p.max_iter) status[i] = HPRSOCP_STATUS_MAX_ITER;
+            else if (tnow > p.time_limit) status[i] = HPRSOCP_STATUS_TIME_LIMIT;
+        }
+        if (h->fp32_active && iter > 0 && check) {
+            // the FP64 state holds this check's iterate (the last update was a full one).
+            // Stalled: no 1% improvement of the best KKT over fp32_stall_checks checks
+            // that also saw two restarts (a long restart cycle alone is not a stall).
+            bool all_ready = true, any = false;
+            double worst = 0.0;
+            for (int i = 0; i < nb; ++i) {
+                if (!active[i] || status[i] != HPRSOCP_STATUS_CONTINUE) continue;
+                any = true;
+                if (rs[i].kkt < 0.99 * best32[i]) {
+                    best32[i] = rs[i].kkt;
+                    stall32[i] = 0;
+                    restarts32[i] = R[i].times;
+                } else {
+                    ++stall32[i];
                 }
-        }
-        int status = HPRSOCP_STATUS_CONTINUE;
-        if (rs.kkt < p.stoptol) status = HPRSOCP_STATUS_OPTIMAL;
-        else if (iter == p.max_iter) status = HPRSOCP_STATUS_MAX_ITER;
-        else if (now_s() - t_start > p.time_limit) status = HPRSOCP_STATUS_TIME_LIMIT;
-        if (h->fp32_active && status == HPRSOCP_STATUS_CONTINUE && iter > 0 && iter % check_iter == 0) {
-            // the FP64 state holds this check's iterate (the last update was a full one)
-            // stalled: no 1% improvement of the best KKT over fp32_stall_checks
-            // checks that also saw two restarts (a long restart cycle alone is
-            // not a stall)
-            if (rs.kkt < 0.99 * best32) {
-                best32 = rs.kkt;
-                stall32 = 0;
-                restarts32 = r.times;
-            } else {
-                ++stall32;
+                const bool stalled = stall32[i] >= p.fp32_stall_checks && R[i].times - restarts32[i] >= 2;
+                all_ready = all_ready && (rs[i].kkt < p.refine_tol || stalled);
+                worst = jmax(worst, rs[i].kkt);
             }
-            const bool stalled = stall32 >= p.fp32_stall_checks && r.times - restarts32 >= 2;
-            if (rs.kkt < p.refine_tol || stalled) {
+            if (any && all_ready) {
                 h->fp32_active = false;
-                if (p.verbose)
-                    printf("precision: FP64 iterations from iteration %lld (KKT %.3e, %s)\n", (long long)iter, rs.kkt,
-                           rs.kkt < p.refine_tol ? "refine_tol reached" : "FP32 stalled");
+                if (p.verbose) printf("precision: FP64 iterations from iteration %lld (KKT %.3e)\n", (long long)iter, worst);
             }
         }
-        check_restart(r, iter, check_iter, h->sigma, rs.kkt);
-        update_sigma(h, r, rs);
-        do_restart(h, r);
-        if (r.restart_flag > 0) h->fp32_dirty = true;
-        if ((print_yes || status != HPRSOCP_STATUS_CONTINUE) && p.verbose)
+        // as in the reference loop, restart logic runs also at an instance's last iteration
+        for (int i = 0; i < nb; ++i)
+            if (active[i]) check_restart(R[i], iter, check_iter, h->I(i).sigma, rs[i].kkt);
+            else R[i].restart_flag = 0;
+        update_sigma(h, R, rs, active);
+        const bool restarted = do_restart(h, R, active);
+        if (restarted) h->fp32_dirty = true;
+        if (p.verbose && single && (print_yes || status[0] != HPRSOCP_STATUS_CONTINUE))
             printf("%5lld    %3.2e    %3.2e    %+7.6e    %+7.6e    %3.2e    %3.2e    %6.2f\n", (long long)iter,
-                   rs.err_Rp, rs.err_Rd, rs.primal_obj, rs.dual_obj, rs.rel_gap, h->sigma, now_s() - t_start);
-        if (rs.kkt < 1e-4 && first4) {
-            t4 = now_s() - t_start;
-            it4 = iter;
-            first4 = false;
-        }
-        if (rs.kkt < 1e-6 && first6) {
-            t6 = now_s() - t_start;
-            it6 = iter;
-            first6 = false;
-        }
-        if (status != HPRSOCP_STATUS_CONTINUE) {
-            h->sync();
+                   rs[0].err_Rp, rs[0].err_Rd, rs[0].primal_obj, rs[0].dual_obj, rs[0].rel_gap, h->I(0).sigma,
+                   now_s() - t_start);
+        bool retired = false;
+        for (int i = 0; i < nb; ++i) {
+            if (!active[i] || !(check || print_yes)) continue;
+            if (rs[i].kkt < 1e-4 && first4[i]) {
+                t4[i] = now_s() - t_start;
+                it4[i] = iter;
+                first4[i] = 0;
+            }
+            if (rs[i].kkt < 1e-6 && first6[i]) {
+                t6[i] = now_s() - t_start;
+                it6[i] = iter;
+                first6[i] = 0;
+            }
+            if (status[i] == HPRSOCP_STATUS_CONTINUE) continue;
             const double tt = now_s() - t_start;
-            memset(res, 0, sizeof(*res));
-            res->iter = iter;
-            res->time = tt;
-            res->power_time = h->power_time;
-            res->setup_time = h->setup_time;
-            res->residuals = rs.kkt;
-            res->primal_obj = rs.primal_obj;
-            res->dual_obj = rs.dual_obj;
-            res->gap = rs.rel_gap;
-            res->err_Rp = rs.err_Rp;
-            res->err_Rp_linear = rs.err_Rp_linear;
-            res->err_Rp_soc = rs.err_Rp_soc;
-            res->err_Rd = rs.err_Rd;
-            res->status = status;
-            res->time_4 = t4 == 0.0 ? tt : t4;
-            res->iter_4 = it4 == 0 ? iter : it4;
-            res->time_6 = t6 == 0.0 ? tt : t6;
-            res->iter_6 = it6 == 0 ? iter : it6;
-            res->sigma = h->sigma;
-            res->lambda_max_A = h->lambda_A;
-            res->lambda_max_Q = h->lambda_Q;
-            res->restarts = (int32_t)r.times;
-            res->iter_fp32 = iter32;
-            h->fp32_active = h->fp32;
-            if (p.verbose) {
+            hprsocp_result &o = res[i];
+            memset(&o, 0, sizeof(o));
+            o.iter = iter;
+            o.time = tt;
+            o.power_time = h->power_time;
+            o.setup_time = h->setup_time;
+            o.residuals = rs[i].kkt;
+            o.primal_obj = rs[i].primal_obj;
+            o.dual_obj = rs[i].dual_obj;
+            o.gap = rs[i].rel_gap;
+            o.err_Rp = rs[i].err_Rp;
+            o.err_Rp_linear = rs[i].err_Rp_linear;
+            o.err_Rp_soc = rs[i].err_Rp_soc;
+            o.err_Rd = rs[i].err_Rd;
+            o.status = status[i];
+            o.time_4 = t4[i] == 0.0 ? tt : t4[i];
+            o.iter_4 = it4[i] == 0 ? iter : it4[i];
+            o.time_6 = t6[i] == 0.0 ? tt : t6[i];
+            o.iter_6 = it6[i] == 0 ? iter : it6[i];
+            o.sigma = h->I(i).sigma;
+            o.lambda_max_A = h->I(i).lambda_A;
+            o.lambda_max_Q = h->I(i).lambda_Q;
+            o.restarts = (int32_t)R[i].times;
+            o.iter_fp32 = iter32;
+            active[i] = 0;
+            h->I(i).active = 0;
+            --nactive;
+            retired = true;
+            if (p.verbose && single) {
                 printf("SOLUTION SUMMARY: status %d, iterations %lld, time %.3f s, primal %.12e, KKT %.3e\n",
-                       status, (long long)iter, tt, rs.primal_obj, rs.kkt);
+                       status[i], (long long)iter, tt, rs[i].primal_obj, rs[i].kkt);
                 fflush(stdout);
             }
+        }
+        if (nactive == 0) {
+            h->sync();
+            if (p.verbose && !single) {
+                printf("batch done: %d instances, %lld iterations, %.3f s\n", nb, (long long)iter, now_s() - t_start);
+                fflush(stdout);
+            }
+            h->fp32_active = h->fp32;
+            for (int i = 0; i < nb; ++i) h->I(i).active = 1;  // the handle stays usable after the solve
+            upload_isc(h);
             return;
         }
+        if (p.verbose && !single && check && iter > 0 && (iter / check_iter) % 20 == 0) {
+            double worst = 0;
+            for (int i = 0; i < nb; ++i)
+                if (active[i]) worst = jmax(worst, rs[i].kkt);
+            printf("iter %lld: %d active, worst KKT %.3e, %.2f s\n", (long long)iter, nactive, worst, now_s() - t_start);
+        }
+        // Halpern counters: set once per check window (restarts happen only at checks)
+        if (check || retired) {
+            for (int i = 0; i < nb; ++i) inner[i] = R[i].inner;
+            set_inner(h, inner);
+        }
         const int64_t nx = iter + 1;
-        bool tc = (nx % check_iter == 0) || (r.restart_flag > 0);
+        bool tc = (nx % check_iter == 0) || restarted;
         if (p.print_frequency == -1) tc = tc || rem_zero(nx, print_step(nx));
         else tc = tc || (nx % p.print_frequency == 0);
         // perform_iteration_step!, algorithm.jl:2722-2744
         if (h->fp32_active) ++iter32;
-        do_update(h, r.inner, tc ? 1 : 0);
-        const bool need_last = r.restart_flag > 0, need_cur = ((iter + 1) % check_iter) == 0;
-        if (need_last) {
+        do_update(h, tc ? 1 : 0);
+        const bool need_cur = (nx % check_iter) == 0;
+        if (restarted) {
             mnorm_enqueue(h, SL_MN0);
-            mn[0] = MnormCtx{h->sigma, h->lambda_A, h->lambda_Q, true};
+            mn[0].snap(h);
+            for (int i = 0; i < nb; ++i) pend0[i] = h->flag_h[i];
         }
         if (need_cur) {
-            if (need_last) {  // same value: copy the slot block
-                launch_copy(h->st, 8, slot(h, SL_MN0), slot(h, SL_MN1));
-            } else {
-                mnorm_enqueue(h, SL_MN1);
-            }
-            mn[1] = MnormCtx{h->sigma, h->lambda_A, h->lambda_Q, true};
+            mnorm_enqueue(h, SL_MN1);
+            mn[1].snap(h);
+            for (int i = 0; i < nb; ++i) pend1[i] = active[i];
         }
-        r.inner += 1;
+        for (int i = 0; i < nb; ++i)
+            if (active[i]) R[i].inner += 1;
     }
 }
 
@@ -1771,8 +2133,10 @@ extern "C" {
 int hprsocp_mnorm(hprsocp_handle *h, double *out) {
     ABI_TRY(h, {
         mnorm_enqueue(h, SL_MN0);
-        read_slots(h, SL_MN0, 8);
-        *out = mnorm_finish(h, h->hscal + SL_MN0, MnormCtx{h->sigma, h->lambda_A, h->lambda_Q, true});
+        read_slots(h);
+        MnormCtx c;
+        c.snap(h);
+        *out = mnorm_finish(h, 0, h->hscal + SL_MN0, c);
     });
     return 0;
 }
@@ -1782,31 +2146,48 @@ int hprsocp_solve(hprsocp_handle *h, hprsocp_result *res) {
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// collect_results! (algorithm.jl:1222-1256) for instance i: unscaled x_bar,
+// y_bar, z_bar, w_bar gathered from the instance's column / row ranges.
+void get_solution_impl(hprsocp_handle *h, int i, double *x, double *y, double *z, double *w) {
+    hipStream_t st = h->st;
+    const InstRange &R = h->rng[i];
+    const InstVal bsv = isc_field(h, &h->isc->b_scale), csv = isc_field(h, &h->isc->c_scale);
+    auto fetch_n = [&](double *dst) {
+        const int32_t nbx = R.box1 - R.box0, nvr = R.var1 - R.var0;
+        if (nbx > 0) HIPCHK(hipMemcpyAsync(dst, h->tmp_n + R.box0, sizeof(double) * nbx, hipMemcpyDeviceToHost, st));
+        if (nvr > 0)
+            HIPCHK(hipMemcpyAsync(dst + nbx, h->tmp_n + R.var0, sizeof(double) * nvr, hipMemcpyDeviceToHost, st));
+        h->sync();
+    };
+    if (x) {
+        launch_unscale(st, h->seg_n, h->x_bar, h->col_norm, bsv, 0, h->tmp_n);
+        fetch_n(x);
+    }
+    if (w) {
+        launch_unscale(st, h->seg_n, h->has_quad ? h->w_bar : h->x_bar, h->col_norm, bsv, 0, h->tmp_n);
+        fetch_n(w);
+    }
+    if (z) {
+        launch_unscale(st, h->seg_n, h->z_bar, h->col_norm, csv, 1, h->tmp_n);
+        fetch_n(z);
+    }
+    if (y && R.m() > 0) {
+        launch_unscale(st, h->seg_m, h->y_bar, h->row_norm, csv, 0, h->tmp_m);
+        const int32_t nl = R.lin1 - R.lin0, nc = R.con1 - R.con0;
+        if (nl > 0) HIPCHK(hipMemcpyAsync(y, h->tmp_m + R.lin0, sizeof(double) * nl, hipMemcpyDeviceToHost, st));
+        if (nc > 0) HIPCHK(hipMemcpyAsync(y + nl, h->tmp_m + R.con0, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
+        h->sync();
+    }
+}
+}  // namespace
+
+extern "C" {
+
 int hprsocp_get_solution(hprsocp_handle *h, double *x, double *y, double *z, double *w) {
-    ABI_TRY(h, {
-        hipStream_t st = h->st;
-        const int32_t m = h->m, n = h->n;
-        if (x) {
-            launch_unscale(st, n, h->x_bar, h->col_norm, h->b_scale, 0, h->tmp_n);
-            HIPCHK(hipMemcpyAsync(x, h->tmp_n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-            h->sync();
-        }
-        if (w) {
-            launch_unscale(st, n, h->has_quad ? h->w_bar : h->x_bar, h->col_norm, h->b_scale, 0, h->tmp_n);
-            HIPCHK(hipMemcpyAsync(w, h->tmp_n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-            h->sync();
-        }
-        if (z) {
-            launch_unscale(st, n, h->z_bar, h->col_norm, h->c_scale, 1, h->tmp_n);
-            HIPCHK(hipMemcpyAsync(z, h->tmp_n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-            h->sync();
-        }
-        if (y && m > 0) {
-            launch_unscale(st, m, h->y_bar, h->row_norm, h->c_scale, 0, h->tmp_m);
-            HIPCHK(hipMemcpyAsync(y, h->tmp_m, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-            h->sync();
-        }
-    });
+    ABI_TRY(h, { get_solution_impl(h, 0, x, y, z, w); });
     return 0;
 }
 
@@ -1910,15 +2291,15 @@ int hprsocp_set_vector(hprsocp_handle *h, int32_t id, const double *in) {
 }
 
 int hprsocp_get_scalars(hprsocp_handle *h, hprsocp_scalars *o) {
-    o->sigma = h->sigma;
-    o->lambda_max_A = h->lambda_A;
-    o->lambda_max_Q = h->lambda_Q;
-    o->b_scale = h->b_scale;
-    o->c_scale = h->c_scale;
-    o->norm_b = h->norm_b;
-    o->norm_c = h->norm_c;
-    o->norm_b_org = h->norm_b_org;
-    o->norm_c_org = h->norm_c_org;
+    o->sigma = h->I(0).sigma;
+    o->lambda_max_A = h->I(0).lambda_A;
+    o->lambda_max_Q = h->I(0).lambda_Q;
+    o->b_scale = h->I(0).b_scale;
+    o->c_scale = h->I(0).c_scale;
+    o->norm_b = h->ih[0].norm_b;
+    o->norm_c = h->ih[0].norm_c;
+    o->norm_b_org = h->ih[0].norm_b_org;
+    o->norm_c_org = h->ih[0].norm_c_org;
     o->Q_is_diag = h->Q_is_diag;
     o->has_quadratic = h->has_quad;
     return 0;
@@ -1926,9 +2307,12 @@ int hprsocp_get_scalars(hprsocp_handle *h, hprsocp_scalars *o) {
 
 int hprsocp_set_scalars(hprsocp_handle *h, double sigma, double la, double lq) {
     ABI_TRY(h, {
-        if (sigma >= 0) h->sigma = sigma;
-        if (la >= 0) h->lambda_A = la;
-        if (lq >= 0) h->lambda_Q = lq;
+        for (int i = 0; i < h->nb; ++i) {
+            if (sigma >= 0) h->I(i).sigma = sigma;
+            if (la >= 0) h->I(i).lambda_A = la;
+            if (lq >= 0) h->I(i).lambda_Q = lq;
+        }
+        upload_isc(h);
         h->lambda_set = true;
         if (h->Q_is_diag) update_q_factors(h);
         h->fp32_dirty = true;
@@ -1978,8 +2362,10 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
+        for (int i = 0; i < h->nb; ++i) h->I(i).active = 1;
+        set_inner(h, std::vector<int64_t>(h->nb, 0));
         HIPCHK(hipEventRecord(e0, h->st));
-        for (int k = 0; k < iters; ++k) do_update(h, k, 0);
+        for (int k = 0; k < iters; ++k) do_update(h, 0);
         HIPCHK(hipEventRecord(e1, h->st));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0;
@@ -1992,8 +2378,9 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
             }
             h->profile = true;
             double acc[6] = {0, 0, 0, 0, 0, 0};
+            set_inner(h, std::vector<int64_t>(h->nb, 0));
             for (int k = 0; k < iters; ++k) {
-                do_update(h, k, 0);
+                do_update(h, 0);
                 HIPCHK(hipEventSynchronize(h->ev[h->has_quad ? 5 : 2]));
                 for (int j = 0; j + 1 < (h->has_quad ? 6 : 3); ++j) {
                     float t = 0;
@@ -2010,6 +2397,283 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
         (void)hipEventDestroy(e1);
     });
     return 0;
+}
+
+
+// =================================================================== batch
+struct hprsocp_batch {
+    hprsocp_handle *h = nullptr;
+};
+
+}  // extern "C"
+
+namespace {
+// Block-diagonal union of independent problems in the canonical layout (all
+// linear rows, then all SOC rows; all boxed columns, then all SOC columns).
+struct Combined {
+    hprsocp_problem p{};
+    std::vector<int32_t> rp, ci, qrp, qci, con_idx, var_idx;
+    std::vector<double> v, qv, c, AL, AU, l, u, rhs;
+    std::vector<InstRange> rng;
+    std::vector<double> objc;
+};
+
+void concat_problems(const hprsocp_problem *const *probs, int32_t count, Combined &C) {
+    std::string e;
+    int64_t M1 = 0, MC = 0, N1 = 0, NC = 0, KC = 0, KV = 0;
+    const bool has_q = probs[0] && probs[0]->Q_kind == HPRSOCP_Q_CSR && probs[0]->Q_nnz > 0;
+    for (int32_t i = 0; i < count; ++i) {
+        const hprsocp_problem *q = probs[i];
+        char b[200];
+        if (!q) throw Err{HPRSOCP_ERR_INVALID, "null problem in batch"};
+        snprintf(b, sizeof b, "batch instance %d: ", i);
+        const std::string pre(b);
+        if (q->m < 0 || q->n < 0) throw Err{HPRSOCP_ERR_INVALID, pre + "negative dimensions"};
+        if (int rc = validate_csr(e, "A", q->m, q->n, q->A_nnz, q->A_rowptr, q->A_colidx, q->A_val))
+            throw Err{rc, pre + e};
+        if (q->Q_kind != HPRSOCP_Q_CSR) throw Err{HPRSOCP_ERR_UNSUPPORTED, pre + "batches take CSR Q only"};
+        if ((q->Q_nnz > 0) != has_q)
+            throw Err{HPRSOCP_ERR_INVALID, pre + "all instances of a batch must have a quadratic term, or none"};
+        if (q->Q_nnz > 0)
+            if (int rc = validate_csr(e, "Q", q->n, q->n, q->Q_nnz, q->Q_rowptr, q->Q_colidx, q->Q_val))
+                throw Err{rc, pre + e};
+        if (q->number_linear_con < 0 || q->number_linear_con > q->m || q->number_lu_x < 0 || q->number_lu_x > q->n)
+            throw Err{HPRSOCP_ERR_INVALID, pre + "number_linear_con / number_lu_x out of range"};
+        if (!q->c || !q->l || !q->u || (q->m > 0 && (!q->AL || !q->AU)))
+            throw Err{HPRSOCP_ERR_INVALID, pre + "missing vector"};
+        if ((q->n_soc_con > 0 && !q->soc_con_idx) || (q->n_soc_var > 0 && !q->soc_var_idx) || q->n_soc_con < 0 ||
+            q->n_soc_var < 0)
+            throw Err{HPRSOCP_ERR_INVALID, pre + "missing SOC index vector"};
+        if (q->soc_con_idx && (q->soc_con_idx[0] != q->number_linear_con || q->soc_con_idx[q->n_soc_con] != q->m))
+            throw Err{HPRSOCP_ERR_INVALID, pre + "soc_con_idx must start at number_linear_con and end at m"};
+        if (q->soc_var_idx && (q->soc_var_idx[0] != q->number_lu_x || q->soc_var_idx[q->n_soc_var] != q->n))
+            throw Err{HPRSOCP_ERR_INVALID, pre + "soc_var_idx must start at number_lu_x and end at n"};
+        M1 += q->number_linear_con;
+        MC += q->m - q->number_linear_con;
+        N1 += q->number_lu_x;
+        NC += q->n - q->number_lu_x;
+        KC += q->n_soc_con;
+        KV += q->n_soc_var;
+    }
+    if (M1 + MC > INT32_MAX || N1 + NC > INT32_MAX) throw Err{HPRSOCP_ERR_INVALID, "batch too large for 32-bit indices"};
+    const int32_t M = (int32_t)(M1 + MC), N = (int32_t)(N1 + NC);
+    C.rng.resize(count);
+    C.objc.resize(count);
+    {
+        int32_t l0 = 0, c0 = (int32_t)M1, b0 = 0, v0 = (int32_t)N1, k0 = 0, kv0 = 0;
+        for (int32_t i = 0; i < count; ++i) {
+            const hprsocp_problem *q = probs[i];
+            InstRange &R = C.rng[i];
+            R.lin0 = l0;
+            R.lin1 = l0 += q->number_linear_con;
+            R.con0 = c0;
+            R.con1 = c0 += q->m - q->number_linear_con;
+            R.box0 = b0;
+            R.box1 = b0 += q->number_lu_x;
+            R.var0 = v0;
+            R.var1 = v0 += q->n - q->number_lu_x;
+            R.kc0 = k0;
+            R.kc1 = k0 += q->n_soc_con;
+            R.kv0 = kv0;
+            R.kv1 = kv0 += q->n_soc_var;
+            C.objc[i] = q->obj_constant;
+        }
+    }
+    auto gcol = [&](int32_t i, int32_t c) {
+        const int32_t n1 = probs[i]->number_lu_x;
+        return c < n1 ? C.rng[i].box0 + c : C.rng[i].var0 + (c - n1);
+    };
+    // A rows in global order
+    C.rp.assign(1, 0);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int32_t i = 0; i < count; ++i) {
+            const hprsocp_problem *q = probs[i];
+            const int32_t r0 = pass ? q->number_linear_con : 0, r1 = pass ? q->m : q->number_linear_con;
+            for (int32_t r = r0; r < r1; ++r) {
+                if (q->A_rowptr)
+                    for (int32_t k = q->A_rowptr[r]; k < q->A_rowptr[r + 1]; ++k) {
+                        C.ci.push_back(gcol(i, q->A_colidx[k]));
+                        C.v.push_back(q->A_val[k]);
+                    }
+                C.rp.push_back((int32_t)C.ci.size());
+            }
+        }
+    // Q rows (= columns) in global column order
+    if (has_q) {
+        C.qrp.assign(1, 0);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int32_t i = 0; i < count; ++i) {
+                const hprsocp_problem *q = probs[i];
+                const int32_t r0 = pass ? q->number_lu_x : 0, r1 = pass ? q->n : q->number_lu_x;
+                for (int32_t r = r0; r < r1; ++r) {
+                    for (int32_t k = q->Q_rowptr[r]; k < q->Q_rowptr[r + 1]; ++k) {
+                        C.qci.push_back(gcol(i, q->Q_colidx[k]));
+                        C.qv.push_back(q->Q_val[k]);
+                    }
+                    C.qrp.push_back((int32_t)C.qci.size());
+                }
+            }
+    }
+    C.c.resize(N);
+    C.l.resize(N);
+    C.u.resize(N);
+    C.AL.resize(M);
+    C.AU.resize(M);
+    C.rhs.assign((size_t)MC, 0.0);
+    for (int32_t i = 0; i < count; ++i) {
+        const hprsocp_problem *q = probs[i];
+        for (int32_t c = 0; c < q->n; ++c) {
+            const int32_t g = gcol(i, c);
+            C.c[g] = q->c[c];
+            C.l[g] = q->l[c];
+            C.u[g] = q->u[c];
+        }
+        for (int32_t r = 0; r < q->m; ++r) {
+            const int32_t g = r < q->number_linear_con ? C.rng[i].lin0 + r : C.rng[i].con0 + (r - q->number_linear_con);
+            C.AL[g] = q->AL[r];
+            C.AU[g] = q->AU[r];
+        }
+        if (q->soc_rhs)
+            for (int32_t r = q->number_linear_con; r < q->m; ++r)
+                C.rhs[C.rng[i].con0 - M1 + (r - q->number_linear_con)] = q->soc_rhs[r - q->number_linear_con];
+        for (int32_t k = 0; k < q->n_soc_con; ++k)
+            C.con_idx.push_back(C.rng[i].con0 + (q->soc_con_idx[k] - q->number_linear_con));
+        for (int32_t k = 0; k < q->n_soc_var; ++k)
+            C.var_idx.push_back(C.rng[i].var0 + (q->soc_var_idx[k] - q->number_lu_x));
+    }
+    C.con_idx.push_back(M);
+    C.var_idx.push_back(N);
+    hprsocp_problem &P = C.p;
+    P.m = M;
+    P.n = N;
+    P.A_nnz = (int64_t)C.ci.size();
+    P.A_rowptr = C.rp.data();
+    P.A_colidx = C.ci.data();
+    P.A_val = C.v.data();
+    P.Q_kind = HPRSOCP_Q_CSR;
+    P.Q_nnz = (int64_t)C.qci.size();
+    P.Q_rowptr = has_q ? C.qrp.data() : nullptr;
+    P.Q_colidx = C.qci.data();
+    P.Q_val = C.qv.data();
+    P.c = C.c.data();
+    P.AL = C.AL.data();
+    P.AU = C.AU.data();
+    P.l = C.l.data();
+    P.u = C.u.data();
+    P.soc_rhs = C.rhs.data();
+    P.number_linear_con = (int32_t)M1;
+    P.number_lu_x = (int32_t)N1;
+    P.n_soc_con = (int32_t)KC;
+    P.soc_con_idx = C.con_idx.data();
+    P.n_soc_var = (int32_t)KV;
+    P.soc_var_idx = C.var_idx.data();
+    P.obj_constant = 0.0;
+}
+}  // namespace
+
+extern "C" {
+
+int hprsocp_batch_create(const hprsocp_problem *const *probs, int32_t count, const hprsocp_params *params,
+                         hprsocp_batch **out) {
+    *out = nullptr;
+    g_create_error.clear();
+    if (!probs || count <= 0) {
+        g_create_error = "empty batch";
+        return HPRSOCP_ERR_INVALID;
+    }
+    if (params && (params->initial_x || params->initial_y)) {
+        g_create_error = "batches take initial points through hprsocp_batch_set_initial";
+        return HPRSOCP_ERR_INVALID;
+    }
+    hprsocp_batch *b = new (std::nothrow) hprsocp_batch();
+    if (!b) return HPRSOCP_ERR_ALLOC;
+    b->h = new (std::nothrow) hprsocp_handle();
+    if (!b->h) {
+        delete b;
+        return HPRSOCP_ERR_ALLOC;
+    }
+    try {
+        Combined C;
+        concat_problems(probs, count, C);
+        create_impl(&C.p, params, b->h, &C.rng, &C.objc);
+    } catch (const Err &e) {
+        g_create_error = e.msg;
+        delete b->h;
+        delete b;
+        return e.code;
+    }
+    *out = b;
+    return 0;
+}
+
+void hprsocp_batch_destroy(hprsocp_batch *b) {
+    if (!b) return;
+    delete b->h;
+    delete b;
+}
+
+const char *hprsocp_batch_last_error(const hprsocp_batch *b) {
+    return b ? b->h->err.c_str() : g_create_error.c_str();
+}
+
+int32_t hprsocp_batch_size(const hprsocp_batch *b) { return b->h->nb; }
+
+int hprsocp_batch_dims(const hprsocp_batch *b, int32_t i, int32_t *m, int32_t *n) {
+    if (i < 0 || i >= b->h->nb) return HPRSOCP_ERR_INVALID;
+    *m = b->h->rng[i].m();
+    *n = b->h->rng[i].n();
+    return 0;
+}
+
+int hprsocp_batch_estimate_lambda(hprsocp_batch *b) { return hprsocp_estimate_lambda(b->h); }
+
+int hprsocp_batch_set_initial(hprsocp_batch *b, const double *const *x0s, const double *const *y0s) {
+    hprsocp_handle *h = b->h;
+    ABI_TRY(h, {
+        const int nb = h->nb;
+        std::vector<int32_t> fx(nb, 0), fy(nb, 0);
+        std::vector<double> X((size_t)h->n, 0.0), Y((size_t)h->m, 0.0);
+        bool anyx = false, anyy = false;
+        for (int i = 0; i < nb; ++i) {
+            const InstRange &R = h->rng[i];
+            if (x0s && x0s[i]) {
+                const int32_t nbx = R.box1 - R.box0;
+                std::copy(x0s[i], x0s[i] + nbx, X.begin() + R.box0);
+                std::copy(x0s[i] + nbx, x0s[i] + R.n(), X.begin() + R.var0);
+                fx[i] = 1;
+                anyx = true;
+            }
+            if (y0s && y0s[i]) {
+                const int32_t nl = R.lin1 - R.lin0;
+                std::copy(y0s[i], y0s[i] + nl, Y.begin() + R.lin0);
+                std::copy(y0s[i] + nl, y0s[i] + R.m(), Y.begin() + R.con0);
+                fy[i] = 1;
+                anyy = true;
+            }
+        }
+        set_initial_impl(h, anyx ? X.data() : nullptr, anyy ? Y.data() : nullptr, fx, fy);
+    });
+    return 0;
+}
+
+int hprsocp_batch_solve(hprsocp_batch *b, hprsocp_result *results) {
+    hprsocp_handle *h = b->h;
+    ABI_TRY(h, { solve_impl(h, results); });
+    return 0;
+}
+
+int hprsocp_batch_get_solution(hprsocp_batch *b, int32_t i, double *x, double *y, double *z, double *w) {
+    hprsocp_handle *h = b->h;
+    if (i < 0 || i >= h->nb) {
+        h->err = "instance index out of range";
+        return HPRSOCP_ERR_INVALID;
+    }
+    ABI_TRY(h, { get_solution_impl(h, i, x, y, z, w); });
+    return 0;
+}
+
+int hprsocp_batch_time_iterations(hprsocp_batch *b, int32_t iters, double *ms_per_iter) {
+    return hprsocp_time_iterations(b->h, iters, ms_per_iter, nullptr);
 }
 
 }  // extern "C"

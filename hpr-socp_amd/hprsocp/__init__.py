@@ -23,7 +23,7 @@ from . import _abi
 
 __all__ = [
     "HPRSOCP_parameters", "HPRSOCP_results", "QP_info", "build_from_SOCP_data", "build_qp",
-    "optimize", "Solver", "load_library", "HPRSOCPError",
+    "optimize", "Solver", "BatchSolver", "load_library", "HPRSOCPError",
 ]
 
 _LIB = None
@@ -430,3 +430,95 @@ def optimize(model: QP_info, params: Optional[HPRSOCP_parameters] = None) -> HPR
         msg = lib.hprsocp_last_error(None)
         raise HPRSOCPError(rc, msg.decode() if msg else "optimize failed")
     return HPRSOCP_results.from_c(r, x, y, z, w)
+
+
+class BatchSolver:
+    """Many independent problems solved together on one GPU.
+
+    The instances are packed into one block-diagonal layout and iterated in
+    lockstep by the same kernels; each keeps its own scaling, sigma, restarts
+    and termination (hprsocp_batch_* in include/hprsocp.h).
+    """
+
+    def __init__(self, models, params: Optional[HPRSOCP_parameters] = None, lib: Optional[C.CDLL] = None):
+        self.lib = _abi.bind_batch(lib if lib is not None else load_library())
+        self.models = list(models)
+        self.params = params or HPRSOCP_parameters(verbose=False)
+        self._keep = []
+        self._bufs = [m.buffers() for m in self.models]
+        arr = (C.POINTER(_abi.Problem) * len(self._bufs))(*[C.pointer(b.problem) for b in self._bufs])
+        self._keep.append(arr)
+        cp = self.params.to_c(self._keep)
+        h = C.c_void_p()
+        rc = self.lib.hprsocp_batch_create(C.cast(arr, C.c_void_p), len(self._bufs), C.byref(cp), C.byref(h))
+        if rc != 0:
+            msg = self.lib.hprsocp_batch_last_error(None)
+            raise HPRSOCPError(rc, msg.decode() if msg else "batch create failed")
+        self.h = h
+        self.dims = [m.A.shape for m in self.models]
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.hprsocp_batch_last_error(self.h)
+            raise HPRSOCPError(rc, msg.decode() if msg else "error")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.hprsocp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self):
+        return len(self.models)
+
+    def estimate_lambda(self):
+        self._check(self.lib.hprsocp_batch_estimate_lambda(self.h))
+
+    def set_initial(self, xs=None, ys=None):
+        """xs / ys: per-instance initial points (None entries = cold start)."""
+        k = len(self.models)
+        keep = []
+
+        def table(vs):
+            if vs is None:
+                return None
+            t = (C.c_void_p * k)()
+            for i, v in enumerate(vs):
+                if v is not None:
+                    a = _abi.f64(v)
+                    keep.append(a)
+                    t[i] = a.ctypes.data
+            keep.append(t)
+            return C.cast(t, C.c_void_p)
+
+        self._check(self.lib.hprsocp_batch_set_initial(self.h, table(xs), table(ys)))
+
+    def solve(self):
+        """Solves all instances; returns one HPRSOCP_results per instance."""
+        k = len(self.models)
+        res = (_abi.Result * k)()
+        self._check(self.lib.hprsocp_batch_solve(self.h, res))
+        out = []
+        for i, (m, n) in enumerate(self.dims):
+            x, z, w = np.zeros(n), np.zeros(n), np.zeros(n)
+            y = np.zeros(m)
+            self._check(self.lib.hprsocp_batch_get_solution(
+                self.h, i, _abi.ptr_f64(x), _abi.ptr_f64(y) if m else None, _abi.ptr_f64(z), _abi.ptr_f64(w)))
+            out.append(HPRSOCP_results.from_c(res[i], x, y, z, w))
+        return out
+
+    def time_iterations(self, iters: int) -> float:
+        ms = C.c_double()
+        self._check(self.lib.hprsocp_batch_time_iterations(self.h, int(iters), C.byref(ms)))
+        return ms.value

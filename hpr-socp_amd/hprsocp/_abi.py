@@ -138,6 +138,28 @@ _SIGS = {
 # Entry points every library exporting this ABI must provide.
 ABI_FUNCTIONS = sorted(_SIGS)
 
+# Batch entry points (HIP backend only).
+_BATCH_SIGS = {
+    "batch_create": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "batch_destroy": (None, [C.c_void_p]),
+    "batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "batch_size": (C.c_int32, [C.c_void_p]),
+    "batch_dims": (C.c_int, [C.c_void_p, C.c_int32, _i32p, _i32p]),
+    "batch_estimate_lambda": (C.c_int, [C.c_void_p]),
+    "batch_set_initial": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "batch_solve": (C.c_int, [C.c_void_p, C.POINTER(Result)]),
+    "batch_get_solution": (C.c_int, [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _f64p]),
+    "batch_time_iterations": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+}
+
+
+def bind_batch(lib: C.CDLL, prefix: str = "hprsocp_"):
+    for name, (res, args) in _BATCH_SIGS.items():
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
 
 def bind(lib: C.CDLL, prefix: str):
     """Attach argtypes/restype for every ABI function under ``prefix``."""

@@ -286,6 +286,36 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
  * of known traffic.  (No counterpart in the reference; measurement hook.) */
 int hprsocp_probe_copy(hprsocp_handle *h, int64_t bytes, double *ms);
 
+/* ------------------------------------------------------------ batch */
+/* Many independent problems solved together: the instances are packed into one
+ * block-diagonal CSR / cone-offset layout and iterated in lockstep by the same
+ * fused kernels (one launch per step for all instances).  Every instance keeps
+ * its own scaling, lambda, sigma, restart state, Halpern counter and
+ * termination; a finished instance is retired by its per-instance mask and its
+ * tiles skip all work.  Instances must all have a CSR Q or all none (the
+ * operator form is single-problem only), and all a diagonal Q or all a general
+ * one.  (No counterpart in the reference, which solves one model per call; each
+ * instance follows optimize(), algorithm.jl:2538-2581.) */
+typedef struct hprsocp_batch hprsocp_batch;
+int hprsocp_batch_create(const hprsocp_problem *const *probs, int32_t count,
+                         const hprsocp_params *params, hprsocp_batch **out);
+void hprsocp_batch_destroy(hprsocp_batch *b);
+const char *hprsocp_batch_last_error(const hprsocp_batch *b);
+int32_t hprsocp_batch_size(const hprsocp_batch *b);
+/* m and n of instance i */
+int hprsocp_batch_dims(const hprsocp_batch *b, int32_t i, int32_t *m, int32_t *n);
+/* per-instance estimate_eigenvalues (algorithm.jl:2434-2467) */
+int hprsocp_batch_estimate_lambda(hprsocp_batch *b);
+/* per-instance process_initial_points! (algorithm.jl:2612-2677); x0s / y0s hold
+ * `count` pointers (sizes n_i / m_i); a NULL array or entry means a cold start. */
+int hprsocp_batch_set_initial(hprsocp_batch *b, const double *const *x0s, const double *const *y0s);
+/* solves every instance; results has `count` entries */
+int hprsocp_batch_solve(hprsocp_batch *b, hprsocp_result *results);
+/* collect_results! (algorithm.jl:1222-1256) of instance i; sizes n_i, m_i, n_i, n_i */
+int hprsocp_batch_get_solution(hprsocp_batch *b, int32_t i, double *x, double *y, double *z, double *w);
+/* like hprsocp_time_iterations, for the whole batch (measurement hook) */
+int hprsocp_batch_time_iterations(hprsocp_batch *b, int32_t iters, double *ms_per_iter);
+
 #ifdef __cplusplus
 }
 #endif
