@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -200,6 +202,11 @@ struct hprsocp_handle {
     int32_t Frows = 0;
     double *Fdiag = nullptr, *Ftmp = nullptr;
     DevPlan plan_A, plan_AT, plan_Q, plan_F, plan_FT;
+    // batches: host tiles of A, A^T, Q and the compacted (active-instance) plans
+    // the update launches once instances retire
+    HostPlan hp_A, hp_AT, hp_Q;
+    DevPlan act_A, act_AT, act_Q;
+    bool use_act = false;
     int32_t *con_idx = nullptr, *var_idx = nullptr, *row_cone = nullptr;
     std::vector<int32_t> h_con_idx, h_var_idx;
     std::vector<int32_t> h_A_rp;  // for nnz bookkeeping
@@ -234,6 +241,12 @@ struct hprsocp_handle {
     // timing (profiled iterations)
     bool profile = false;
     std::vector<hipEvent_t> ev;
+    // captured runs of partial updates: (first kstep, length, fp32) -> executable graph
+    std::map<std::tuple<int64_t, int32_t, int>, hipGraphExec_t> graphs;
+    void clear_graphs() {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
 
     InstScal &I(int i) { return isc_h[i]; }
     template <class T>
@@ -257,6 +270,7 @@ struct hprsocp_handle {
     void sync() { HIPCHK(hipStreamSynchronize(st)); }
     ~hprsocp_handle() {
         if (st) (void)hipStreamSynchronize(st);
+        clear_graphs();
         for (void *p : allocs) (void)hipFree(p);
         if (hscal) (void)hipHostFree(hscal);
         if (isc_h) (void)hipHostFree(isc_h);
@@ -304,14 +318,42 @@ void upload_plan(hprsocp_handle *h, const HostPlan &hp, DevPlan &dp) {
 }
 
 DevPlan make_plan(hprsocp_handle *h, const std::vector<int32_t> &rp, int32_t nrows, int32_t lin_end,
-                  const std::vector<int32_t> &cidx, int32_t ncones, const std::vector<int32_t> *row_inst = nullptr) {
+                  const std::vector<int32_t> &cidx, int32_t ncones, const std::vector<int32_t> *row_inst = nullptr,
+                  HostPlan *keep = nullptr) {
     PlanConfig cfg;
     const int32_t *ri = (row_inst && h->nb > 1) ? row_inst->data() : nullptr;
     HostPlan hp = build_plan(rp.empty() ? nullptr : rp.data(), nrows, lin_end, cidx.empty() ? nullptr : cidx.data(),
                              ncones, cfg, ri);
     DevPlan dp;
     upload_plan(h, hp, dp);
+    if (keep) *keep = hp;
     return dp;
+}
+
+// A second tile buffer of the plan, filled by compact_plan.
+DevPlan alloc_active(hprsocp_handle *h, const DevPlan &full) {
+    DevPlan a = full;
+    a.tiles = h->dalloc<Tile>(std::max(full.ntiles, 1));
+    return a;
+}
+
+// Keeps the tiles of active instances (in order; HUGE segments stay last).
+void compact_plan(hprsocp_handle *h, const HostPlan &hp, const DevPlan &full, DevPlan &act) {
+    std::vector<Tile> t;
+    t.reserve(hp.tiles.size());
+    int32_t first_huge = 0;
+    for (size_t k = 0; k < hp.tiles.size(); ++k) {
+        const Tile &x = hp.tiles[k];
+        if (!h->I(x.inst).active) continue;
+        t.push_back(x);
+        if ((int32_t)k < hp.first_huge) ++first_huge;
+    }
+    act.ntiles = (int32_t)t.size();
+    act.first_huge = first_huge;
+    if (!t.empty())
+        HIPCHK(hipMemcpyAsync(act.tiles, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, h->st));
+    h->sync();  // t is a host temporary
+    (void)full;
 }
 
 double *slot(hprsocp_handle *h, int k) { return h->dscal + k; }
@@ -1114,7 +1156,7 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
                     break;
                 }
         h->q_struct_diag = sd;
-        h->plan_Q = make_plan(h, Qh.rp, n, n, {}, 0, &h->h_col_inst);
+        h->plan_Q = make_plan(h, Qh.rp, n, n, {}, 0, &h->h_col_inst, &h->hp_Q);
     }
     HostCsr F, FT;
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
@@ -1140,8 +1182,13 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
         h->plan_FT = make_plan(h, FT.rp, n, n, {}, 0);
     }
     // plans: A rows (linear | constraint cones), A^T rows (box | variable cones)
-    h->plan_A = make_plan(h, A.rp, m, h->m1, h->h_con_idx, h->ncon, &h->h_row_inst);
-    h->plan_AT = make_plan(h, AT.rp, n, h->n1, h->h_var_idx, h->nvar, &h->h_col_inst);
+    h->plan_A = make_plan(h, A.rp, m, h->m1, h->h_con_idx, h->ncon, &h->h_row_inst, &h->hp_A);
+    h->plan_AT = make_plan(h, AT.rp, n, h->n1, h->h_var_idx, h->nvar, &h->h_col_inst, &h->hp_AT);
+    if (h->nb > 1) {
+        h->act_A = alloc_active(h, h->plan_A);
+        h->act_AT = alloc_active(h, h->plan_AT);
+        if (h->nnzQ > 0) h->act_Q = alloc_active(h, h->plan_Q);
+    }
     {
         std::vector<int32_t> rc((size_t)m, -1);
         for (int32_t k = 0; k < h->ncon; ++k)
@@ -1176,12 +1223,16 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     do_scaling(h);
     // XCD column slices of every SpMV operator, kept where they measure faster
     // (values are final after scaling)
-    tune_slices(h, h->A, h->plan_A, A, h->A_v, h->m1, h->h_con_idx, h->ncon, "A", &h->h_row_inst);
-    tune_slices(h, h->AT, h->plan_AT, AT, h->AT_v, h->n1, h->h_var_idx, h->nvar, "A^T", &h->h_col_inst);
+    // (batches gather from per-instance slices of the input already; their tile
+    // lists get compacted as instances retire, which the sliced form cannot follow)
+    if (h->nb == 1) {
+        tune_slices(h, h->A, h->plan_A, A, h->A_v, h->m1, h->h_con_idx, h->ncon, "A", &h->h_row_inst);
+        tune_slices(h, h->AT, h->plan_AT, AT, h->AT_v, h->n1, h->h_var_idx, h->nvar, "A^T", &h->h_col_inst);
+    }
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
         tune_slices(h, h->F, h->plan_F, F, h->F.val, F.nrows, {}, 0, "F", nullptr);
         tune_slices(h, h->FT, h->plan_FT, FT, h->FT.val, n, {}, 0, "F^T", nullptr);
-    } else if (h->nnzQ > 0 && !h->q_struct_diag) {
+    } else if (h->nnzQ > 0 && !h->q_struct_diag && h->nb == 1) {
         tune_slices(h, h->Q, h->plan_Q, Qh, h->Q_v, n, {}, 0, "Q", &h->h_col_inst);
     }
     // check_Q_diagonal (scaling.jl:1312-1353), after scaling as in solve(); every
@@ -1361,17 +1412,20 @@ template <typename T>
 static void do_update_t(hprsocp_handle *h, IterView<T> &V, int64_t kstep, int full) {
     hipStream_t st = h->st;
     const int32_t m = h->m;
+    const DevPlan &pA = h->use_act ? h->act_A : h->plan_A;
+    const DevPlan &pAT = h->use_act ? h->act_AT : h->plan_AT;
+    const DevPlan &pQ = h->use_act ? h->act_Q : h->plan_Q;
     auto ev = [&](int k) {
         if (h->profile) HIPCHK(hipEventRecord(h->ev[k], st));
     };
     if (!h->has_quad) {
         ev(0);
         NoQPrimalArgsT<T> a{V.y, V.x, V.x_hat, V.x_bar, V.z_bar, V.dx, V.last_x, V.c, V.l, V.u, h->isc, kstep, full};
-        launch_noq_primal<T>(st, h->plan_AT, V.AT, h->var_idx, a, V.raw_n, h->part_n);
+        launch_noq_primal<T>(st, pAT, V.AT, h->var_idx, a, V.raw_n, h->part_n);
         ev(1);
         if (m > 0) {
             DualArgsT<T> d{V.x_hat, V.y, V.y_bar, V.s, V.dy, V.last_y, V.AL, V.AU, V.rhs, h->isc, kstep, full, 0};
-            launch_dual<T>(st, h->plan_A, V.A, h->con_idx, d, V.raw_m, h->part_m);
+            launch_dual<T>(st, pA, V.A, h->con_idx, d, V.raw_m, h->part_m);
         }
         ev(2);
         return;
@@ -1383,7 +1437,7 @@ static void do_update_t(hprsocp_handle *h, IterView<T> &V, int64_t kstep, int fu
     QPPrimalArgsT<T> a{V.x, V.x_hat, V.x_bar, V.z_bar, V.dx, V.w_bar, V.tempv, V.w, V.last_x, V.c,
                        V.l, V.u, V.aty, V.qw, V.diag_Q, V.fact1, V.fact2, h->isc, kstep,
                        full, sdiag, h->Q_is_diag};
-    launch_qp_primal<T>(st, h->plan_AT, h->var_idx, a, V.raw_n, h->part_n);
+    launch_qp_primal<T>(st, pAT, h->var_idx, a, V.raw_n, h->part_n);
     ev(2);
     if (m > 0) {
         if (!sdiag) {
@@ -1393,19 +1447,19 @@ static void do_update_t(hprsocp_handle *h, IterView<T> &V, int64_t kstep, int fu
                 launch_spmv_tempv<T>(st, h->plan_FT, V.FT, V.Ftmp, t);
             } else {
                 TempvArgsT<T> t{V.x_hat, V.qw, V.tempv, nullptr, nullptr, h->isc};
-                launch_spmv_tempv<T>(st, h->plan_Q, V.Q, V.w_bar, t);
+                launch_spmv_tempv<T>(st, pQ, V.Q, V.w_bar, t);
             }
         }
         ev(3);
         DualArgsT<T> d{V.tempv, V.y, V.y_bar, V.s, V.dy, V.last_y, V.AL, V.AU, V.rhs, h->isc, kstep, full, 1};
-        launch_dual<T>(st, h->plan_A, V.A, h->con_idx, d, V.raw_m, h->part_m);
+        launch_dual<T>(st, pA, V.A, h->con_idx, d, V.raw_m, h->part_m);
     } else {
         ev(3);
     }
     ev(4);
     W2ArgsT<T> w2{V.y_bar, V.w, V.w_bar, V.aty, V.dw, V.atdy, V.aty_bar, V.last_w, V.last_aty, V.fact,
                   h->isc, kstep, full, h->Q_is_diag};
-    launch_w2<T>(st, h->plan_AT, V.AT, w2);
+    launch_w2<T>(st, pAT, V.AT, w2);
     ev(5);
 }
 }  // extern "C++"
@@ -1432,6 +1486,42 @@ static void do_update(hprsocp_handle *h, int full, bool sync64 = false) {
         IterView<double> V = view64(h);
         do_update_t<double>(h, V, k, full);
     }
+}
+
+// L consecutive partial updates.  With params.use_graph the run is captured
+// once per (first step, length, precision) into a HIP graph and replayed: the
+// kernels read sigma, the Halpern counter base and the retirement mask from
+// device memory, so one capture serves every check window that starts at the
+// same step.  Graphs are dropped when the launch shapes change (tile
+// compaction) and captured again on demand.
+static void run_partial_updates(hprsocp_handle *h, int64_t L) {
+    if (!h->params.use_graph || L < 2) {
+        for (int64_t k = 0; k < L; ++k) do_update(h, 0);
+        return;
+    }
+    h->to_check = 0;
+    if (h->fp32_active && h->fp32_dirty) demote_state(h);
+    const int fp = h->fp32_active ? 1 : 0;
+    const auto key = std::make_tuple(h->kstep, (int32_t)L, fp);
+    auto it = h->graphs.find(key);
+    if (it == h->graphs.end()) {
+        if (h->graphs.size() >= 64) h->clear_graphs();
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+        IterView<double> V = view64(h);
+        for (int64_t k = 0; k < L; ++k) {
+            if (fp) do_update_t<float>(h, h->v32, h->kstep + k, 0);
+            else do_update_t<double>(h, V, h->kstep + k, 0);
+        }
+        HIPCHK(hipStreamEndCapture(h->st, &g));
+        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+        it = h->graphs.emplace(key, ge).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, h->st));
+    h->kstep += L;
 }
 
 int hprsocp_update(hprsocp_handle *h, int64_t inner, int32_t to_check) {
@@ -2088,6 +2178,8 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
                 fflush(stdout);
             }
             h->fp32_active = h->fp32;
+            h->use_act = false;
+            h->clear_graphs();
             for (int i = 0; i < nb; ++i) h->I(i).active = 1;  // the handle stays usable after the solve
             upload_isc(h);
             return;
@@ -2098,6 +2190,13 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
                 if (active[i]) worst = jmax(worst, rs[i].kkt);
             printf("iter %lld: %d active, worst KKT %.3e, %.2f s\n", (long long)iter, nactive, worst, now_s() - t_start);
         }
+        if (retired && nb > 1) {  // launch only the tiles of instances still running
+            compact_plan(h, h->hp_A, h->plan_A, h->act_A);
+            compact_plan(h, h->hp_AT, h->plan_AT, h->act_AT);
+            if (h->nnzQ > 0) compact_plan(h, h->hp_Q, h->plan_Q, h->act_Q);
+            h->use_act = true;
+            h->clear_graphs();
+        }
         // Halpern counters: set once per check window (restarts happen only at checks)
         if (check || retired) {
             for (int i = 0; i < nb; ++i) inner[i] = R[i].inner;
@@ -2107,9 +2206,25 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
         bool tc = (nx % check_iter == 0) || restarted;
         if (p.print_frequency == -1) tc = tc || rem_zero(nx, print_step(nx));
         else tc = tc || (nx % p.print_frequency == 0);
-        // perform_iteration_step!, algorithm.jl:2722-2744
-        if (h->fp32_active) ++iter32;
-        do_update(h, tc ? 1 : 0);
+        // perform_iteration_step!, algorithm.jl:2722-2744.  A partial update is
+        // followed by iterations whose bodies are plain (no residual check, no
+        // print step): their partial updates run as one captured graph.
+        if (!tc) {
+            int64_t L = 1;
+            auto plain = [&](int64_t t) {
+                if (t >= p.max_iter || t % check_iter == 0) return false;
+                return p.print_frequency == -1 ? !rem_zero(t, print_step(t)) : (t % p.print_frequency != 0);
+            };
+            while (plain(nx + L) && L < check_iter) ++L;
+            run_partial_updates(h, L);
+            if (h->fp32_active) iter32 += L;
+            for (int i = 0; i < nb; ++i)
+                if (active[i]) R[i].inner += L - 1;
+            iter += L - 1;
+        } else {
+            if (h->fp32_active) ++iter32;
+            do_update(h, 1);
+        }
         const bool need_cur = (nx % check_iter) == 0;
         if (restarted) {
             mnorm_enqueue(h, SL_MN0);
@@ -2672,8 +2787,8 @@ int hprsocp_batch_get_solution(hprsocp_batch *b, int32_t i, double *x, double *y
     return 0;
 }
 
-int hprsocp_batch_time_iterations(hprsocp_batch *b, int32_t iters, double *ms_per_iter) {
-    return hprsocp_time_iterations(b->h, iters, ms_per_iter, nullptr);
+int hprsocp_batch_time_iterations(hprsocp_batch *b, int32_t iters, double *ms_per_iter, double *ms_kernel) {
+    return hprsocp_time_iterations(b->h, iters, ms_per_iter, ms_kernel);
 }
 
 }  // extern "C"

@@ -518,7 +518,10 @@ class BatchSolver:
             out.append(HPRSOCP_results.from_c(res[i], x, y, z, w))
         return out
 
-    def time_iterations(self, iters: int) -> float:
+    def time_iterations(self, iters: int, per_kernel: bool = False):
+        """(ms per iteration, per-launch-group ms or None) of `iters` lockstep iterations."""
         ms = C.c_double()
-        self._check(self.lib.hprsocp_batch_time_iterations(self.h, int(iters), C.byref(ms)))
-        return ms.value
+        kern = np.zeros(6) if per_kernel else None
+        self._check(self.lib.hprsocp_batch_time_iterations(self.h, int(iters), C.byref(ms),
+                                                           _abi.ptr_f64(kern)))
+        return ms.value, (list(kern) if per_kernel else None)

@@ -149,7 +149,7 @@ _BATCH_SIGS = {
     "batch_set_initial": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "batch_solve": (C.c_int, [C.c_void_p, C.POINTER(Result)]),
     "batch_get_solution": (C.c_int, [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _f64p]),
-    "batch_time_iterations": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "batch_time_iterations": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), _f64p]),
 }
 
 

@@ -202,3 +202,72 @@ def make_small_qp(seed: int = 7, n1=60, var_sizes=(5, 9), m1=30, con_sizes=(4, 7
         Q = ("factored", F, d)
         Qx = lambda v: F.T @ (F @ v) + d * v
     return assemble(rng, A1, A2, n1, list(var_sizes), list(con_sizes), Q=Q, Qx_fn=Qx)
+
+
+# ------------------------------------------------------------------ batches
+def _simplex_projection(v):
+    """Euclidean projection of v onto {x >= 0, sum x = 1} (sort-based)."""
+    u = np.sort(v)[::-1]
+    css = np.cumsum(u) - 1.0
+    k = np.arange(1, v.size + 1)
+    rho = np.nonzero(u - css / k > 0)[0][-1]
+    theta = css[rho] / (rho + 1.0)
+    return np.maximum(v - theta, 0.0)
+
+
+def _min_risk_upper(F, steps=60):
+    """Risk ||F^T x|| of an approximate minimum-risk portfolio (accelerated
+    projected gradient on the simplex); an upper bound of the minimum."""
+    n = F.shape[0]
+    L = 2.0 * np.linalg.norm(F, 2) ** 2 + 1e-12
+    x = np.full(n, 1.0 / n)
+    yv, t = x.copy(), 1.0
+    for _ in range(steps):
+        g = 2.0 * (F @ (F.T @ yv))
+        xn = _simplex_projection(yv - g / L)
+        tn = 0.5 * (1.0 + np.sqrt(1.0 + 4.0 * t * t))
+        yv = xn + ((t - 1.0) / tn) * (xn - x)
+        x, t = xn, tn
+    return float(np.linalg.norm(F.T @ x))
+
+
+def make_portfolio(rng, n: int, k: int) -> QP_info:
+    """min -mu'x  s.t.  sum x = 1, x >= 0, ||F'x||_2 <= gamma.
+
+    F is n x k dense N(0, 1/k), mu ~ N(0.05, 0.02).  gamma = 1.5 x max(r_min,
+    r_eq / 4) with r_min the risk of an approximate minimum-risk portfolio (an
+    upper bound of the minimum) and r_eq = ||F' 1/n|| the equal-weight risk:
+    for n > k the minimum risk is 0 (F' has a nonnegative null vector on the
+    simplex), and gamma = 1.5 x 0 would leave only the cone's apex; the r_eq
+    floor keeps the instance strictly feasible with a binding risk limit.
+    Layout: one linear row (sum x = 1), then the cone
+    block (gamma, F'x) in Q^{k+1} as an empty t-row with soc_rhs = -gamma and k
+    rows F'; every variable boxed in [0, inf)."""
+    F = rng.standard_normal((n, k)) / np.sqrt(k)
+    mu = rng.normal(0.05, 0.02, n)
+    r_eq = float(np.linalg.norm(F.T @ np.full(n, 1.0 / n)))
+    gamma = 1.5 * max(_min_risk_upper(F), 0.25 * r_eq)
+    ones = sp.csr_matrix(np.ones((1, n)))
+    rows = sp.vstack([ones, sp.csr_matrix((1, n)), sp.csr_matrix(F.T)], format="csr")
+    rows.sort_indices()
+    m = k + 2
+    AL = np.concatenate([[1.0], np.full(k + 1, -np.inf)])
+    AU = np.concatenate([[1.0], np.full(k + 1, np.inf)])
+    soc_rhs = np.zeros(k + 1)
+    soc_rhs[0] = -gamma
+    model = build_qp(rows, -mu, AL, AU, np.zeros(n), np.full(n, np.inf), soc_rhs, 1, n,
+                     np.array([1, m]), np.array([n]))
+    model.gamma = gamma
+    return model
+
+
+def make_c3(count: int = 512, seed: int = 3000, n_lo: int = 200, n_hi: int = 20000, k_lo: int = 5,
+            k_hi: int = 50):
+    """Batch of `count` portfolio SOCPs, n log-uniform in [n_lo, n_hi], k in U{k_lo..k_hi}."""
+    out = []
+    for i in range(count):
+        rng = _rng(seed + i)
+        n = int(round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
+        k = int(rng.integers(k_lo, k_hi + 1))
+        out.append(make_portfolio(rng, n, k))
+    return out

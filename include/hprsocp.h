@@ -314,7 +314,7 @@ int hprsocp_batch_solve(hprsocp_batch *b, hprsocp_result *results);
 /* collect_results! (algorithm.jl:1222-1256) of instance i; sizes n_i, m_i, n_i, n_i */
 int hprsocp_batch_get_solution(hprsocp_batch *b, int32_t i, double *x, double *y, double *z, double *w);
 /* like hprsocp_time_iterations, for the whole batch (measurement hook) */
-int hprsocp_batch_time_iterations(hprsocp_batch *b, int32_t iters, double *ms_per_iter);
+int hprsocp_batch_time_iterations(hprsocp_batch *b, int32_t iters, double *ms_per_iter, double *ms_kernel);
 
 #ifdef __cplusplus
 }
