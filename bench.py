@@ -144,7 +144,8 @@ def bench_single(args):
     precs = ["fp64", "fp32_refine"] if args.precision == "both" else [args.precision]
     res = {}
     for prec in precs:
-        p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, time_limit=args.solve_time, precision=prec)
+        p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, time_limit=args.solve_time, precision=prec,
+                                       use_graph=not args.no_graph)
         t0 = time.time()
         g = hprsocp.Solver(model, p)
         log(f"[{prec}] setup (transfer + scaling + plans) {time.time() - t0:.2f}s")
@@ -244,7 +245,8 @@ def bench_batch(args):
     out = {"workload": wl, "count": N}
     objs = None
     for prec in (["fp64", "fp32_refine"] if args.precision == "both" else [args.precision]):
-        p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, time_limit=args.solve_time, precision=prec)
+        p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, time_limit=args.solve_time, precision=prec,
+                                       use_graph=not args.no_graph)
         t0 = time.time()
         b = hprsocp.BatchSolver(models, p)
         b.estimate_lambda()
@@ -269,7 +271,8 @@ def bench_batch(args):
     # sequential single-problem FP64 solves on an evenly spaced sample
     K = min(args.seq_sample, N)
     idx = np.linspace(0, N - 1, K).round().astype(int)
-    p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, time_limit=args.solve_time)
+    p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, time_limit=args.solve_time,
+                                   use_graph=not args.no_graph)
     t0 = time.time()
     worst = 0.0
     for i in idx:
@@ -315,6 +318,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched C3 measurement")
+    ap.add_argument("--no-graph", action="store_true", help="launch every update eagerly (no HIP graphs)")
     ap.add_argument("--precision", default="both", choices=["fp64", "fp32_refine", "both"])
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--seq-sample", type=int, default=32)

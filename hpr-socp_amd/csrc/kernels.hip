@@ -377,27 +377,41 @@ void launch_power_check(hipStream_t st, int32_t ninst, InstVal lam, InstVal qq, 
 }
 
 // ------------------------------------------------------------ scaling
-__global__ void k_row_stat(CsrView M, CsrView Q, int has_q, int use_sum, double *out) {
-    GRID_STRIDE(i, M.nrows) {
+// one wavefront per row (rows of a batch can hold tens of thousands of entries)
+__global__ __launch_bounds__(kBlock) void k_row_stat(CsrView M, CsrView Q, int has_q, int use_sum, double *out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
+    for (int64_t i = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; i < M.nrows; i += nw) {
         double a = 0.0, b = 0.0;
-        for (int k = M.rowptr[i]; k < M.rowptr[i + 1]; ++k) {
+        for (int k = M.rowptr[i] + lane; k < M.rowptr[i + 1]; k += 64) {
             const double v = fabs(M.val[k]);
             a = use_sum ? a + v : nmax(a, v);
         }
         if (has_q)
-            for (int k = Q.rowptr[i]; k < Q.rowptr[i + 1]; ++k) {
+            for (int k = Q.rowptr[i] + lane; k < Q.rowptr[i + 1]; k += 64) {
                 const double v = fabs(Q.val[k]);
                 b = use_sum ? b + v : nmax(b, v);
             }
-        double r = sqrt(has_q ? (use_sum ? a + b : nmax(a, b)) : a);
-        if (r == 0.0) r = 1.0;
-        out[i] = r;
+        a = use_sum ? wave_sum(a) : wave_max(a);
+        if (has_q) b = use_sum ? wave_sum(b) : wave_max(b);
+        if (lane == 0) {
+            double r = sqrt(has_q ? (use_sum ? a + b : nmax(a, b)) : a);
+            if (r == 0.0) r = 1.0;
+            out[i] = r;
+        }
     }
+}
+static inline int wave_row_blocks(int64_t rows) {
+    int64_t b = (rows + 3) / 4;
+    if (b < 1) b = 1;
+    if (b > 16384) b = 16384;
+    return (int)b;
 }
 void launch_row_stat(hipStream_t st, const CsrView &M, const CsrView *Q, int use_sum, double *out) {
     if (M.nrows <= 0) return;
     CsrView q = Q ? *Q : CsrView{nullptr, nullptr, nullptr, 0};
-    hipLaunchKernelGGL(k_row_stat, dim3(ew_blocks(M.nrows)), dim3(kBlock), 0, st, M, q, Q ? 1 : 0, use_sum, out);
+    hipLaunchKernelGGL(k_row_stat, dim3(wave_row_blocks(M.nrows)), dim3(kBlock), 0, st, M, q, Q ? 1 : 0, use_sum,
+                       out);
 }
 
 __global__ void k_block_scale(int32_t ncones, const int32_t *idx, const int32_t *cnt, double *t, int32_t strategy,
@@ -429,10 +443,12 @@ void launch_clip(hipStream_t st, int64_t n, double *v) {
 }
 // val = ((1/r[i]) * val) * (1/c[j]) with (i, j) the entry's (row, col) of the
 // operator being scaled, or its transpose when `trans` (keeps A and A^T bit-equal)
-__global__ void k_scale_csr(int32_t nrows, const int32_t *rp, const int32_t *col, double *val, const double *r,
-                            const double *c, int trans) {
-    GRID_STRIDE(i, nrows) {
-        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+__global__ __launch_bounds__(kBlock) void k_scale_csr(int32_t nrows, const int32_t *rp, const int32_t *col,
+                                                     double *val, const double *r, const double *c, int trans) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
+    for (int64_t i = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; i < nrows; i += nw) {
+        for (int k = rp[i] + lane; k < rp[i + 1]; k += 64) {
             const int j = col[k];
             val[k] = trans ? ((1.0 / r[j]) * val[k]) * (1.0 / c[i]) : ((1.0 / r[i]) * val[k]) * (1.0 / c[j]);
         }
@@ -441,12 +457,14 @@ __global__ void k_scale_csr(int32_t nrows, const int32_t *rp, const int32_t *col
 void launch_scale_csr(hipStream_t st, int32_t nrows, const int32_t *rowptr, const int32_t *col,
                       double *val, const double *r, const double *c) {
     if (nrows > 0)
-        hipLaunchKernelGGL(k_scale_csr, dim3(ew_blocks(nrows)), dim3(kBlock), 0, st, nrows, rowptr, col, val, r, c, 0);
+        hipLaunchKernelGGL(k_scale_csr, dim3(wave_row_blocks(nrows)), dim3(kBlock), 0, st, nrows, rowptr, col, val, r,
+                           c, 0);
 }
 void launch_scale_csr_t(hipStream_t st, int32_t nrows, const int32_t *rowptr, const int32_t *col,
                         double *val, const double *r, const double *c) {
     if (nrows > 0)
-        hipLaunchKernelGGL(k_scale_csr, dim3(ew_blocks(nrows)), dim3(kBlock), 0, st, nrows, rowptr, col, val, r, c, 1);
+        hipLaunchKernelGGL(k_scale_csr, dim3(wave_row_blocks(nrows)), dim3(kBlock), 0, st, nrows, rowptr, col, val, r,
+                           c, 1);
 }
 __global__ void k_scale_cols(int32_t n, const double *t, double *c, double *l, double *u, double *cn) {
     GRID_STRIDE(j, n) {
