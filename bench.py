@@ -227,29 +227,43 @@ def bench_single(args):
     }
 
 
-def bench_batch(args):
-    """C3: batched portfolio solves vs sequential single-problem FP64 solves."""
+def batch_workload(name, count):
+    from hprsocp import synthetic
+    if name == "c3":
+        models = synthetic.make_c3(count=count or 512, seed=3000)
+        warms = [None] * len(models)
+        desc = "portfolio SOCPs (n log-uniform [200, 20000], k in [5, 50])"
+    else:
+        models, warms = synthetic.make_c4(count=count or 64, seed=4000)
+        desc = ("robust-regression QP-SOCPs (d ~ U[1e4, 2e5], diagonal Q, one SOC of size d+1; "
+                "even-indexed half warm-started from perturbed solutions)")
+    return models, warms, desc
+
+
+def bench_batch(args, name="c3"):
+    """Batched solves vs sequential single-problem FP64 solves (C3 portfolios, C4 regressions)."""
     import numpy as np
     import hprsocp
-    from hprsocp import synthetic
     hprsocp.load_library()
     t0 = time.time()
-    models = synthetic.make_c3(count=args.batch_count, seed=3000)
+    models, warms, desc = batch_workload(name, args.batch_count if name == "c3" else args.c4_count)
     N = len(models)
     sum_n = sum(mm.A.shape[1] for mm in models)
     sum_m = sum(mm.A.shape[0] for mm in models)
     sum_nnz = sum(mm.A.nnz for mm in models)
-    wl = (f"C3 batch of {N} portfolio SOCPs (n log-uniform [200, 20000], k in [5, 50]; "
-          f"sum n={sum_n}, sum m={sum_m}, sum nnz={sum_nnz})")
+    wl = f"{name.upper()} batch of {N} {desc}; sum n={sum_n}, sum m={sum_m}, sum nnz={sum_nnz}"
     log(f"workload {wl} generated in {time.time() - t0:.1f}s")
+    tol = args.stoptol
     out = {"workload": wl, "count": N}
     objs = None
     for prec in (["fp64", "fp32_refine"] if args.precision == "both" else [args.precision]):
-        p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, time_limit=args.solve_time, precision=prec,
+        p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=tol, time_limit=args.solve_time, precision=prec,
                                        use_graph=not args.no_graph)
         t0 = time.time()
         b = hprsocp.BatchSolver(models, p)
         b.estimate_lambda()
+        if any(w is not None for w in warms):
+            b.set_initial(warms, None)
         t_setup = time.time() - t0
         t1 = time.time()
         rs = b.solve()
@@ -269,15 +283,17 @@ def bench_batch(args):
         if prec == "fp64":
             objs = [r.primal_obj for r in rs]
     # sequential single-problem FP64 solves on an evenly spaced sample
-    K = min(args.seq_sample, N)
+    K = min(args.seq_sample if name == "c3" else args.c4_seq_sample, N)
     idx = np.linspace(0, N - 1, K).round().astype(int)
-    p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, time_limit=args.solve_time,
+    p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=tol, time_limit=args.solve_time,
                                    use_graph=not args.no_graph)
     t0 = time.time()
     worst = 0.0
     for i in idx:
         with hprsocp.Solver(models[i], p) as g:
             g.estimate_lambda()
+            if warms[i] is not None:
+                g.set_initial(warms[i], None)
             r = g.solve()
             g.get("x_bar")
         if objs is not None:
@@ -291,11 +307,13 @@ def bench_batch(args):
     log(f"[sequential fp64] {out['sequential_fp64']}")
     if args.cpu_iters > 0:  # CPU oracle on part of the same sample (1 thread)
         olib = oracle_lib()
-        Kc = min(8, K)
+        Kc = min(8 if name == "c3" else 2, K)
         t0 = time.time()
         for i in idx[:Kc]:
             with hprsocp.Solver(models[i], p, lib=olib, prefix="oracle_") as o:
                 o.estimate_lambda()
+                if warms[i] is not None:
+                    o.set_initial(warms[i], None)
                 o.solve()
         t_cpu = time.time() - t0
         frac_c = sum(models[i].A.nnz for i in idx[:Kc]) / sum_nnz
@@ -322,6 +340,9 @@ def main():
     ap.add_argument("--precision", default="both", choices=["fp64", "fp32_refine", "both"])
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--seq-sample", type=int, default=32)
+    ap.add_argument("--c4-count", type=int, default=64)
+    ap.add_argument("--c4-seq-sample", type=int, default=8)
+    ap.add_argument("--stoptol", type=float, default=1e-6, help="stoptol of the batched configs")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -329,14 +350,14 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the HPR path here is "
               "single-GPU — see the sharding design notes.", file=sys.stderr)
         sys.exit(2)
-    if args.config == "c3":
-        b = bench_batch(args)
+    if args.config in ("c3", "c4"):
+        b = bench_batch(args, args.config)
         first = b.get("fp64") or b.get("fp32_refine")
         best = max(b[k]["solves_per_s"] for k in ("fp64", "fp32_refine") if k in b)
         out = {"metric": METRIC, "value": best, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
                "warmup": args.warmup, "ms_per_step": first["ms_per_iter"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)",
-               "config": {"workload": b["workload"], "config": "c3"}, "batch": b,
+               "config": {"workload": b["workload"], "config": args.config, "stoptol": args.stoptol}, "batch": b,
                "cpu_baseline": {"value": b.get("cpu_oracle", {}).get("solves_per_s"), "unit": "solves/s",
                                 "cores": 1, "kind": "port",
                                 "sample": "sequential oracle solves of an evenly spaced sample, scaled by nnz"}}

@@ -259,6 +259,17 @@ struct hprsocp_handle {
         HIPCHK(hipMemsetAsync(p, 0, bytes, st));
         return (T *)p;
     }
+    // frees an allocation made by dalloc/upload
+    void dfree(const void *p) {
+        if (!p) return;
+        for (size_t k = 0; k < allocs.size(); ++k)
+            if (allocs[k] == p) {
+                (void)hipStreamSynchronize(st);
+                (void)hipFree(allocs[k]);
+                allocs.erase(allocs.begin() + (int64_t)k);
+                return;
+            }
+    }
     // synchronous: the host source may be a temporary
     template <class T>
     T *upload(const T *src, int64_t count) {
@@ -715,6 +726,7 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
     int32_t *dperm = h->upload<int32_t>(perm.data(), nnz);
     double *sval = h->dalloc<double>(nnz);
     launch_gather_perm(h->st, nnz, dval, dperm, sval);
+    h->dfree(dperm);
     M.s_val = sval;
     M.s_tiles = h->upload<SliceTile>(tiles.data(), NT);
     M.s_ntiles = NT;
@@ -728,46 +740,95 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
 // HIP events (median of 5 after a warm-up) and keeps the faster one together
 // with its epilogue plan (nnz-balanced tiles for the plain form, row tiles for
 // the sliced form).  HPRSOCP_XCD_SLICES=<P> forces P slices (tests).
+void free_slices(hprsocp_handle *h, const CsrView &S) {
+    h->dfree(S.s_base);
+    h->dfree(S.s_off);
+    h->dfree(S.s_col);
+    h->dfree(S.s_val);
+    h->dfree(S.s_tiles);
+    h->dfree(S.s_part);
+}
+void free_plan(hprsocp_handle *h, const DevPlan &p) {
+    h->dfree(p.tiles);
+    h->dfree(p.huge_buf);
+}
+
+// Autotuner of the XCD slicing: for an operator whose input vector outgrows
+// ~2 MB it builds the sliced forms with 4 and 8 column slices (8: one slice per
+// XCD; 4: two XCDs share a slice, halving the partial-sum traffic while the
+// slice still fits one L2), times one SpMV of each form and of the plain form
+// with HIP events (median of 5 after a warm-up) and keeps the fastest with its
+// epilogue plan (nnz-balanced tiles for the plain form, row tiles for sliced
+// forms).  HPRSOCP_XCD_SLICES=<P> forces P slices (tests).
 void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm, const double *dval,
                  int32_t lin_end, const std::vector<int32_t> &cidx, int32_t ncones, const char *name,
                  const std::vector<int32_t> *row_inst) {
     const char *env = getenv("HPRSOCP_XCD_SLICES");
-    int32_t P = env ? std::max(1, atoi(env)) : (hm.ncols * (int64_t)sizeof(double) > (2ll << 20) ? 8 : 1);
-    if (P <= 1 || hm.ci.empty()) return;
-    CsrView S = M;
-    if (!build_slices(h, S, hm, dval, P)) return;
-    DevPlan splan = make_plan(h, {}, hm.nrows, lin_end, cidx, ncones, row_inst);
+    if (hm.ci.empty()) return;
+    std::vector<int32_t> cands;
+    if (env) {
+        const int32_t P = std::max(1, atoi(env));
+        if (P > 1) cands.push_back(P);
+    } else if (hm.ncols * (int64_t)sizeof(double) > (2ll << 20)) {
+        cands = {4, 8};
+    }
+    if (cands.empty()) return;
+    double *xin = nullptr, *yout = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto time_one = [&](const CsrView &V, const DevPlan &pl) {
+        std::vector<float> t;
+        for (int rep = 0; rep < 6; ++rep) {
+            HIPCHK(hipEventRecord(e0, h->st));
+            launch_spmv(h->st, pl, V, xin, yout);
+            HIPCHK(hipEventRecord(e1, h->st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep) t.push_back(ms);
+        }
+        std::sort(t.begin(), t.end());
+        return t[t.size() / 2];
+    };
     if (!env) {
-        double *xin = h->dalloc<double>(hm.ncols);
-        double *yout = h->dalloc<double>(hm.nrows);
+        xin = h->dalloc<double>(hm.ncols);
+        yout = h->dalloc<double>(hm.nrows);
         launch_fill(h->st, hm.ncols, 1.0, xin);
-        hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        auto time_one = [&](const CsrView &V, const DevPlan &pl) {
-            std::vector<float> t;
-            for (int rep = 0; rep < 6; ++rep) {
-                HIPCHK(hipEventRecord(e0, h->st));
-                launch_spmv(h->st, pl, V, xin, yout);
-                HIPCHK(hipEventRecord(e1, h->st));
-                HIPCHK(hipEventSynchronize(e1));
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                if (rep) t.push_back(ms);
-            }
-            std::sort(t.begin(), t.end());
-            return t[t.size() / 2];
-        };
-        const float t_plain = time_one(M, plan), t_sliced = time_one(S, splan);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (h->params.verbose)
-            printf("XCD slicing autotune %s: plain %.4f ms, %d slices %.4f ms -> %s\n", name, t_plain, P, t_sliced,
-                   t_sliced < t_plain ? "sliced" : "plain");
-        if (!(t_sliced < t_plain)) return;
     }
-    M = S;
-    plan = splan;
+    float best_t = env ? INFINITY : time_one(M, plan);
+    CsrView bestM = M;
+    DevPlan bestP = plan;
+    bool sliced = false;
+    for (int32_t P : cands) {
+        CsrView S = M;
+        if (!build_slices(h, S, hm, dval, P)) continue;
+        DevPlan splan = make_plan(h, {}, hm.nrows, lin_end, cidx, ncones, row_inst);
+        const float t = env ? 0.0f : time_one(S, splan);
+        if (h->params.verbose) printf("XCD slicing autotune %s: %d slices %.4f ms (best so far %.4f ms)\n", name, P, t, best_t);
+        if (t < best_t) {
+            if (sliced) {
+                free_slices(h, bestM);
+                free_plan(h, bestP);
+            }
+            best_t = t;
+            bestM = S;
+            bestP = splan;
+            sliced = true;
+        } else {
+            free_slices(h, S);
+            free_plan(h, splan);
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    h->dfree(xin);
+    h->dfree(yout);
+    if (sliced) {
+        free_plan(h, plan);  // the plain form's epilogue plan is replaced (its CSR stays for other uses)
+        M = bestM;
+        plan = bestP;
+    }
 }
 
 // ------------------------------------------------------------- validation

@@ -271,3 +271,56 @@ def make_c3(count: int = 512, seed: int = 3000, n_lo: int = 200, n_hi: int = 200
         k = int(rng.integers(k_lo, k_hi + 1))
         out.append(make_portfolio(rng, n, k))
     return out
+
+
+def make_robust_regression(rng, d: int, lam: float = 0.1):
+    """min 1/2 ||r||^2 + lam 1'(x+ + x-)  s.t.  D(x+ - x-) - r = y,  ||W(x+ - x- - xbar)||_2 <= rho.
+
+    D is (d/2) x d sparse with 10 N(0,1) entries per row, W = diag(U(0.5, 2)),
+    x* has 10% N(0,1) nonzeros, y = D x* + 0.01 N(0,1), xbar = 0, rho = 2||x*||.
+    Variables [x+ (d), x- (d), r (d/2)], x+- >= 0 and r free (all boxed);
+    Q = diag(0, 0, 1) (diagonal); rows: d/2 equalities, then the cone block
+    (rho, W(x+ - x-)) in Q^{d+1} as an empty t-row with soc_rhs = -rho.
+    Returns (model, warm_x) with warm_x the point built from x* (x+- from its
+    sign split, r = D x* - y)."""
+    h = d // 2
+    cols = np.concatenate([rng.choice(d, 10, replace=False) for _ in range(h)])
+    rows = np.repeat(np.arange(h), 10)
+    D = sp.csr_matrix((rng.standard_normal(h * 10), (rows, cols)), shape=(h, d))
+    D.sum_duplicates()
+    xs = np.where(rng.uniform(size=d) < 0.1, rng.standard_normal(d), 0.0)
+    y = D @ xs + 0.01 * rng.standard_normal(h)
+    W = rng.uniform(0.5, 2.0, d)
+    rho = 2.0 * float(np.linalg.norm(xs))
+    n = 2 * d + h
+    A1 = sp.hstack([D, -D, -sp.eye(h)], format="csr")
+    Wd = sp.diags(W)
+    A2 = sp.vstack([sp.csr_matrix((1, n)), sp.hstack([Wd, -Wd, sp.csr_matrix((d, h))])], format="csr")
+    A = sp.vstack([A1, A2], format="csr")
+    A.sort_indices()
+    m = h + d + 1
+    AL = np.concatenate([y, np.full(d + 1, -np.inf)])
+    AU = np.concatenate([y, np.full(d + 1, np.inf)])
+    soc_rhs = np.zeros(d + 1)
+    soc_rhs[0] = -rho
+    c = np.concatenate([np.full(2 * d, lam), np.zeros(h)])
+    l = np.concatenate([np.zeros(2 * d), np.full(h, -np.inf)])
+    u = np.full(n, np.inf)
+    Q = sp.diags(np.concatenate([np.zeros(2 * d), np.ones(h)])).tocsr()
+    model = build_qp(A, c, AL, AU, l, u, soc_rhs, h, n, np.array([h, m]), np.array([n]), Q=Q)
+    warm = np.concatenate([np.maximum(xs, 0), np.maximum(-xs, 0), D @ xs - y])
+    return model, warm
+
+
+def make_c4(count: int = 64, seed: int = 4000, d_lo: int = 10000, d_hi: int = 200000):
+    """Batch of `count` robust-regression QP-SOCPs, d ~ U[d_lo, d_hi] (even);
+    returns (models, warm_starts) with warm starts for the even-indexed half,
+    perturbed as w * (1 + 0.01 N(0,1))."""
+    models, warms = [], []
+    for i in range(count):
+        rng = _rng(seed + i)
+        d = 2 * int(rng.integers(d_lo // 2, d_hi // 2 + 1))
+        m, w = make_robust_regression(rng, d)
+        models.append(m)
+        warms.append(w * (1 + 0.01 * rng.standard_normal(w.size)) if i % 2 == 0 else None)
+    return models, warms
