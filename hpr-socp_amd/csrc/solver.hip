@@ -348,14 +348,14 @@ DevPlan alloc_active(hprsocp_handle *h, const DevPlan &full) {
     return a;
 }
 
-// Keeps the tiles of active instances (in order; HUGE segments stay last).
-void compact_plan(hprsocp_handle *h, const HostPlan &hp, const DevPlan &full, DevPlan &act) {
+// Keeps the tiles of the instances with keep[inst] (in order; HUGE segments stay last).
+void compact_plan(hprsocp_handle *h, const HostPlan &hp, DevPlan &act, const std::vector<char> &keep) {
     std::vector<Tile> t;
     t.reserve(hp.tiles.size());
     int32_t first_huge = 0;
     for (size_t k = 0; k < hp.tiles.size(); ++k) {
         const Tile &x = hp.tiles[k];
-        if (!h->I(x.inst).active) continue;
+        if (!keep[x.inst]) continue;
         t.push_back(x);
         if ((int32_t)k < hp.first_huge) ++first_huge;
     }
@@ -364,7 +364,6 @@ void compact_plan(hprsocp_handle *h, const HostPlan &hp, const DevPlan &full, De
     if (!t.empty())
         HIPCHK(hipMemcpyAsync(act.tiles, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, h->st));
     h->sync();  // t is a host temporary
-    (void)full;
 }
 
 double *slot(hprsocp_handle *h, int k) { return h->dscal + k; }
@@ -625,13 +624,17 @@ std::vector<double> power_iteration(hprsocp_handle *h, bool onA) {
     HIPCHK(hipMemcpyAsync(h->pstate, ps.data(), sizeof(PowerState) * nb, hipMemcpyHostToDevice, h->st));
     h->sync();
     const int max_iter = 5000, batch = 10;
+    bool cpt = false;  // compacted plans in use
+    int done_seen = 0;
     for (int it = 0; it < max_iter; it += batch) {
         for (int k = 0; k < batch; ++k) {
             reduce(h, sp, {RedTerm{z, z, nullptr, 0}}, SL_PW);
             launch_scale_by_rnorm(h->st, sp, z, islot(h, SL_PW), q, h->pstate);
             if (onA) {
-                launch_spmv(h->st, h->plan_AT, h->AT, q, h->tempv);
-                launch_spmv(h->st, h->plan_A, h->A, h->tempv, z);
+                launch_spmv(h->st, cpt ? h->act_AT : h->plan_AT, h->AT, q, h->tempv);
+                launch_spmv(h->st, cpt ? h->act_A : h->plan_A, h->A, h->tempv, z);
+            } else if (cpt && h->q_kind == HPRSOCP_Q_CSR && h->nnzQ > 0 && !h->q_struct_diag) {
+                launch_spmv(h->st, h->act_Q, h->Q, q, z);
             } else {
                 qmap(h, q, z);
             }
@@ -644,8 +647,22 @@ std::vector<double> power_iteration(hprsocp_handle *h, bool onA) {
         HIPCHK(hipMemcpyAsync(ps.data(), h->pstate, sizeof(PowerState) * nb, hipMemcpyDeviceToHost, h->st));
         h->sync();
         bool all = true;
-        for (int i = 0; i < nb; ++i) all = all && ps[i].done;
+        int ndone = 0;
+        for (int i = 0; i < nb; ++i) {
+            all = all && ps[i].done;
+            ndone += ps[i].done != 0;
+        }
         if (all) break;
+        // batches: once instances converge, run the SpMVs over the others' tiles only
+        if (nb > 1 && ndone > done_seen) {
+            std::vector<char> keep(nb);
+            for (int i = 0; i < nb; ++i) keep[i] = !ps[i].done;
+            compact_plan(h, h->hp_A, h->act_A, keep);
+            compact_plan(h, h->hp_AT, h->act_AT, keep);
+            if (h->nnzQ > 0) compact_plan(h, h->hp_Q, h->act_Q, keep);
+            cpt = true;
+            done_seen = ndone;
+        }
     }
     std::vector<double> lam((size_t)nb);
     for (int i = 0; i < nb; ++i) lam[i] = ps[i].lambda;
@@ -1129,6 +1146,10 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
     setup_instances(h);
 
+    auto phase = [&](const char *what) {
+        if (P.verbose) printf("setup: %-28s %8.3f s\n", what, now_s() - t0);
+    };
+    phase("validation + instances");
     // ---- host-side matrices: A, A^T, Q (symmetrized, canonical), F, F^T
     HostCsr A;
     A.nrows = m;
@@ -1184,6 +1205,7 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     }
     h->nnzQ = (int64_t)Qh.v.size();
 
+    phase("host transpose / Q");
     // ---- upload
     int32_t *A_rp = h->upload<int32_t>(A.rp.data(), m + 1);
     int32_t *A_ci = h->upload<int32_t>(A.ci.data(), (int64_t)A.ci.size());
@@ -1281,7 +1303,9 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     h->cone_t = h->dalloc<double>(h->ncon);
     h->cone_ns = h->dalloc<double>(h->ncon);
 
+    phase("upload + plans + alloc");
     do_scaling(h);
+    phase("scaling");
     // XCD column slices of every SpMV operator, kept where they measure faster
     // (values are final after scaling)
     // (batches gather from per-instance slices of the input already; their tile
@@ -1296,6 +1320,7 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     } else if (h->nnzQ > 0 && !h->q_struct_diag && h->nb == 1) {
         tune_slices(h, h->Q, h->plan_Q, Qh, h->Q_v, n, {}, 0, "Q", &h->h_col_inst);
     }
+    phase("slice autotune");
     // check_Q_diagonal (scaling.jl:1312-1353), after scaling as in solve(); every
     // instance of a batch must land in the same class
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
@@ -2252,9 +2277,11 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
             printf("iter %lld: %d active, worst KKT %.3e, %.2f s\n", (long long)iter, nactive, worst, now_s() - t_start);
         }
         if (retired && nb > 1) {  // launch only the tiles of instances still running
-            compact_plan(h, h->hp_A, h->plan_A, h->act_A);
-            compact_plan(h, h->hp_AT, h->plan_AT, h->act_AT);
-            if (h->nnzQ > 0) compact_plan(h, h->hp_Q, h->plan_Q, h->act_Q);
+            std::vector<char> keep(nb);
+            for (int i = 0; i < nb; ++i) keep[i] = h->I(i).active != 0;
+            compact_plan(h, h->hp_A, h->act_A, keep);
+            compact_plan(h, h->hp_AT, h->act_AT, keep);
+            if (h->nnzQ > 0) compact_plan(h, h->hp_Q, h->act_Q, keep);
             h->use_act = true;
             h->clear_graphs();
         }

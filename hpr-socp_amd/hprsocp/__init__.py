@@ -23,7 +23,7 @@ from . import _abi
 
 __all__ = [
     "HPRSOCP_parameters", "HPRSOCP_results", "QP_info", "build_from_SOCP_data", "build_qp",
-    "optimize", "Solver", "BatchSolver", "load_library", "HPRSOCPError",
+    "optimize", "Solver", "BatchSolver", "load_library", "HPRSOCPError", "build_from_cbf",
 ]
 
 _LIB = None
@@ -525,3 +525,43 @@ class BatchSolver:
         self._check(self.lib.hprsocp_batch_time_iterations(self.h, int(iters), C.byref(ms),
                                                            _abi.ptr_f64(kern)))
         return ms.value, (list(kern) if per_kernel else None)
+
+
+def build_from_cbf(filename: str, verbose: bool = False) -> QP_info:
+    """Model from a .cbf file, build_from_cbf (reference src/utils/builders.jl:114-130)
+    through the library's reader (hprsocp_cbf_read; host code only)."""
+    lib = load_library()
+    lib.hprsocp_cbf_read.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+    lib.hprsocp_cbf_read.restype = C.c_int
+    lib.hprsocp_cbf_last_error.restype = C.c_char_p
+    lib.hprsocp_cbf_problem.argtypes = [C.c_void_p, C.POINTER(_abi.Problem), C.POINTER(C.c_int32)]
+    lib.hprsocp_cbf_destroy.argtypes = [C.c_void_p]
+    h = C.c_void_p()
+    rc = lib.hprsocp_cbf_read(str(filename).encode(), C.byref(h))
+    if rc != 0:
+        raise HPRSOCPError(rc, lib.hprsocp_cbf_last_error().decode())
+    try:
+        p = _abi.Problem()
+        neq = C.c_int32()
+        lib.hprsocp_cbf_problem(h, C.byref(p), C.byref(neq))
+        m, n, nnz = p.m, p.n, p.A_nnz
+
+        def arr(ptr, k, dt):
+            return np.ctypeslib.as_array(ptr, shape=(k,)).astype(dt, copy=True) if k > 0 else np.zeros(0, dt)
+
+        rp = arr(p.A_rowptr, m + 1, np.int64)
+        ci = arr(p.A_colidx, nnz, np.int64)
+        v = arr(p.A_val, nnz, np.float64)
+        A = sp.csr_matrix((v, ci, rp), shape=(m, n))
+        m1 = p.number_linear_con
+        model = build_qp(A, arr(p.c, n, np.float64), arr(p.AL, m, np.float64), arr(p.AU, m, np.float64),
+                         arr(p.l, n, np.float64), arr(p.u, n, np.float64), arr(p.soc_rhs, m - m1, np.float64),
+                         m1, p.number_lu_x, arr(p.soc_con_idx, p.n_soc_con + 1, np.int64),
+                         arr(p.soc_var_idx, p.n_soc_var + 1, np.int64), obj_constant=p.obj_constant)
+        model.number_eq = int(neq.value)
+        model.number_ineq = int(m1 - neq.value)
+        if verbose:
+            print(f"READ CBF {filename}: m={m} n={n} nnz={nnz}")
+        return model
+    finally:
+        lib.hprsocp_cbf_destroy(h)

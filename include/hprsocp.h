@@ -286,6 +286,20 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
  * of known traffic.  (No counterpart in the reference; measurement hook.) */
 int hprsocp_probe_copy(hprsocp_handle *h, int64_t bytes, double *ms);
 
+/* ------------------------------------------------------------ CBF input */
+/* Reads a Conic Benchmark Format file into the canonical layout, as read_cbf
+ * + build_from_cbf do (reference src/utils/cbf_io.jl:493-750, 899-1131,
+ * src/utils/builders.jl:24-130): rows L= | L+,L- (L- negated) | Q, F rows
+ * dropped; variables linear | Q; objective sense max negates c and the
+ * constant.  Host code only (no device is touched). */
+typedef struct hprsocp_cbf hprsocp_cbf;
+int hprsocp_cbf_read(const char *path, hprsocp_cbf **out);
+const char *hprsocp_cbf_last_error(void);
+/* Fills `prob` with pointers into the reader's storage (valid until destroy);
+ * number_eq (may be NULL) receives the count of equality rows. */
+int hprsocp_cbf_problem(const hprsocp_cbf *m, hprsocp_problem *prob, int32_t *number_eq);
+void hprsocp_cbf_destroy(hprsocp_cbf *m);
+
 /* ------------------------------------------------------------ batch */
 /* Many independent problems solved together: the instances are packed into one
  * block-diagonal CSR / cone-offset layout and iterated in lockstep by the same
