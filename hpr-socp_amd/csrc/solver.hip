@@ -14,6 +14,7 @@
 #include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
+#include <time.h>
 
 #include <algorithm>
 #include <array>
@@ -28,10 +29,13 @@
 #include "plan.h"
 #include "golden.h"
 #include "scale_rules.h"
+#include "npz.h"
 
 using namespace hpr;
 
 namespace {
+
+const char *const kDefaultSaveName = "HPRSOCP_autosave.npz";
 
 double now_s() {
     using namespace std::chrono;
@@ -228,6 +232,16 @@ struct hprsocp_handle {
     double *ivtmp = nullptr, *ivtmp_h = nullptr;  // per-instance scratch values (2 x nb)
     PowerState *pstate = nullptr;
     double *tmp_m = nullptr, *tmp_n = nullptr, *tmp_n2 = nullptr;
+    // auto_save: checkpoint path, the initial points given in the parameters,
+    // and the best iterate of the solve (scaled, device) with its residuals
+    std::string save_path;
+    std::vector<double> init_x, init_y;
+    double *sv_x = nullptr, *sv_y = nullptr, *sv_z = nullptr, *sv_w = nullptr;
+    struct Saved {
+        double sigma = 0, err_Rp = INFINITY, err_Rd = INFINITY, primal_obj = INFINITY, dual_obj = INFINITY,
+               rel_gap = INFINITY;
+        int64_t iter = 0;
+    } saved;
     bool lambda_set = false;
     double power_time = 0, setup_time = 0;
     int to_check = 1;
@@ -1047,6 +1061,8 @@ void hprsocp_default_params(hprsocp_params *p) {
     p->use_l2_scaling = 0;
     p->use_Pock_Chambolle_scaling = 1;
     p->soc_block_scaling_strategy = HPRSOCP_SOC_PHASE_TAPER;
+    p->auto_save = 0;
+    p->save_filename = kDefaultSaveName;
     p->verbose = 1;
     p->use_gpu = 1;
     p->precision = HPRSOCP_PREC_FP64;
@@ -1068,7 +1084,11 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     else hprsocp_default_params(&h->params);
     hprsocp_params &P = h->params;
     if (!P.use_gpu) throw Err{HPRSOCP_ERR_UNSUPPORTED, "use_gpu = false: this backend runs the HPR iteration on the GPU only"};
-    if (P.auto_save) throw Err{HPRSOCP_ERR_UNSUPPORTED, "auto_save (HDF5) is not provided by this backend"};
+    // the handle keeps its own copies of what auto_save writes later: the
+    // caller's strings and initial points need only live through this call
+    h->save_path = P.save_filename && P.save_filename[0] ? P.save_filename : kDefaultSaveName;
+    P.save_filename = h->save_path.c_str();
+    if (P.auto_save && ranges) throw Err{HPRSOCP_ERR_UNSUPPORTED, "auto_save applies to single-problem handles, not batches"};
     if (P.precision != HPRSOCP_PREC_FP64 && P.precision != HPRSOCP_PREC_FP32_REFINE)
         throw Err{HPRSOCP_ERR_INVALID, "unknown precision mode"};
     if (P.precision == HPRSOCP_PREC_FP32_REFINE && (!(P.refine_tol >= 0.0) || P.fp32_stall_checks <= 0))
@@ -1083,6 +1103,11 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
         throw Err{HPRSOCP_ERR_INVALID, "Invalid print_frequency: it should be a positive integer or -1"};
     const int32_t m = p->m, n = p->n;
     if (m < 0 || n < 0) throw Err{HPRSOCP_ERR_INVALID, "negative dimensions"};
+    if (P.auto_save) {
+        if (P.initial_x) h->init_x.assign(P.initial_x, P.initial_x + n);
+        if (P.initial_y) h->init_y.assign(P.initial_y, P.initial_y + m);
+    }
+    P.initial_x = P.initial_y = nullptr;
     std::string e;
     if (int rc = validate_csr(e, "A", m, n, p->A_nnz, p->A_rowptr, p->A_colidx, p->A_val)) throw Err{rc, e};
     if (p->number_linear_con < 0 || p->number_linear_con > m) throw Err{HPRSOCP_ERR_INVALID, "number_linear_con out of range"};
@@ -1298,6 +1323,12 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     h->row_norm = h->dalloc<double>(m);
     double **vm[] = {&h->y, &h->y_bar, &h->dy, &h->last_y, &h->s, &h->ax, &h->rp, &h->raw_m, &h->tmp_m};
     for (double **pp : vm) *pp = h->dalloc<double>(m);
+    if (P.auto_save) {  // saved_state of allocate_workspace (algorithm.jl:1574-1585)
+        h->sv_x = h->dalloc<double>(n);
+        h->sv_z = h->dalloc<double>(n);
+        h->sv_w = h->dalloc<double>(n);
+        h->sv_y = h->dalloc<double>(m);
+    }
     h->part_n = h->dalloc<double>(h->plan_AT.nslots);
     h->part_m = h->dalloc<double>(h->plan_A.nslots);
     h->cone_t = h->dalloc<double>(h->ncon);
@@ -2093,6 +2124,165 @@ bool do_restart(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<ch
     return true;
 }
 
+// collect_results! (algorithm.jl:1222-1256) for instance i from the scaled
+// device vectors sx, sw, sz (n) and sy (m): the maps x = b_scale D_C xbar,
+// w = b_scale D_C wbar, z = c_scale D_C^{-1} zbar, y = c_scale D_R ybar,
+// gathered from the instance's column / row ranges.  Any output may be null.
+void fetch_unscaled(hprsocp_handle *h, int i, const double *sx, const double *sw, const double *sz, const double *sy,
+                    double *x, double *y, double *z, double *w) {
+    hipStream_t st = h->st;
+    const InstRange &R = h->rng[i];
+    const InstVal bsv = isc_field(h, &h->isc->b_scale), csv = isc_field(h, &h->isc->c_scale);
+    auto fetch_n = [&](double *dst) {
+        const int32_t nbx = R.box1 - R.box0, nvr = R.var1 - R.var0;
+        if (nbx > 0) HIPCHK(hipMemcpyAsync(dst, h->tmp_n + R.box0, sizeof(double) * nbx, hipMemcpyDeviceToHost, st));
+        if (nvr > 0)
+            HIPCHK(hipMemcpyAsync(dst + nbx, h->tmp_n + R.var0, sizeof(double) * nvr, hipMemcpyDeviceToHost, st));
+        h->sync();
+    };
+    if (x) {
+        launch_unscale(st, h->seg_n, sx, h->col_norm, bsv, 0, h->tmp_n);
+        fetch_n(x);
+    }
+    if (w) {
+        launch_unscale(st, h->seg_n, sw, h->col_norm, bsv, 0, h->tmp_n);
+        fetch_n(w);
+    }
+    if (z) {
+        launch_unscale(st, h->seg_n, sz, h->col_norm, csv, 1, h->tmp_n);
+        fetch_n(z);
+    }
+    if (y && R.m() > 0) {
+        launch_unscale(st, h->seg_m, sy, h->row_norm, csv, 0, h->tmp_m);
+        const int32_t nl = R.lin1 - R.lin0, nc = R.con1 - R.con0;
+        if (nl > 0) HIPCHK(hipMemcpyAsync(y, h->tmp_m + R.lin0, sizeof(double) * nl, hipMemcpyDeviceToHost, st));
+        if (nc > 0) HIPCHK(hipMemcpyAsync(y + nl, h->tmp_m + R.con0, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
+        h->sync();
+    }
+}
+
+// The best-iterate record of compute_residuals! (algorithm.jl:176-195): at
+// iteration 0, or when the KKT value beats max(err_Rp, err_Rd, rel_gap) of the
+// saved state, the bar iterate (scaled) and its residuals become the saved
+// state.  w_bar is recorded only for problems with a quadratic term.
+void track_best(hprsocp_handle *h, int64_t iter, const hprsocp_residuals &R) {
+    hprsocp_handle::Saved &S = h->saved;
+    if (!(iter == 0 || R.kkt < jmax(jmax(S.err_Rp, S.err_Rd), S.rel_gap))) return;
+    hipStream_t st = h->st;
+    const size_t bn = sizeof(double) * (size_t)h->n, bm = sizeof(double) * (size_t)h->m;
+    if (bn > 0) {
+        HIPCHK(hipMemcpyAsync(h->sv_x, h->x_bar, bn, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(h->sv_z, h->z_bar, bn, hipMemcpyDeviceToDevice, st));
+        if (h->has_quad) HIPCHK(hipMemcpyAsync(h->sv_w, h->w_bar, bn, hipMemcpyDeviceToDevice, st));
+    }
+    if (bm > 0) HIPCHK(hipMemcpyAsync(h->sv_y, h->y_bar, bm, hipMemcpyDeviceToDevice, st));
+    S.sigma = h->I(0).sigma;
+    S.iter = iter;
+    S.err_Rp = R.err_Rp;
+    S.err_Rd = R.err_Rd;
+    S.primal_obj = R.primal_obj;
+    S.dual_obj = R.dual_obj;
+    S.rel_gap = R.rel_gap;
+}
+
+// Local time as Julia's string(Dates.now()): yyyy-mm-ddTHH:MM:SS.sss
+std::string timestamp_now() {
+    using namespace std::chrono;
+    const auto t = system_clock::now();
+    const time_t tt = system_clock::to_time_t(t);
+    const int ms = (int)(duration_cast<milliseconds>(t.time_since_epoch()).count() % 1000);
+    struct tm tm_;
+    localtime_r(&tt, &tm_);
+    char b[40];
+    snprintf(b, sizeof b, "%04d-%02d-%02dT%02d:%02d:%02d.%03d", tm_.tm_year + 1900, tm_.tm_mon + 1, tm_.tm_mday,
+             tm_.tm_hour, tm_.tm_min, tm_.tm_sec, ms);
+    return b;
+}
+
+// Names of the bc_scaling_norm_type and soc_block_scaling_strategy codes
+// (the reference's Symbols, scaling.jl:181 and :625-632).
+const char *bc_norm_name(int32_t k) {
+    static const char *const names[] = {"l2", "rms", "linf"};
+    return k >= 0 && k < 3 ? names[k] : "unknown";
+}
+const char *soc_strategy_name(int32_t k) {
+    static const char *const names[] = {
+        "max", "rms", "geom", "adaptive", "hybrid", "ruiz_geom", "role_split", "pock_safe_role_split",
+        "size_split_role_split", "pre_pock_size_split", "pre_pock_huge_var_taper",
+        "pre_pock_huge_var_taper_pock_geom", "pre_pock_band_split", "pre_pock_constraint_rms",
+        "pre_pock_constraint_max", "pre_pock_constraint_taper", "small_cone_constraint_max",
+        "count_aware_small_cone_bias", "phase_blend", "phase_taper"};
+    return k >= 1 && k <= 20 ? names[k - 1] : "unknown";
+}
+
+// save_state_to_hdf5! (algorithm.jl:306-423): the current and the best
+// iterate in original units, their residuals and the parameters, under the
+// reference's dataset paths, in an .npz archive at params.save_filename.  A
+// failed write is reported (verbose) and the solve goes on, as the
+// reference's try/catch does (algorithm.jl:2872-2879).
+void autosave(hprsocp_handle *h, int64_t iter, const hprsocp_residuals &R, double elapsed) {
+    const hprsocp_params &p = h->params;
+    const int32_t n = h->n, m = h->m;
+    const bool q = h->has_quad != 0;
+    std::vector<double> cx(n), cw(n), cz(n), cy(m), bx(n), bw(n), bz(n), by(m);
+    fetch_unscaled(h, 0, h->x_bar, q ? h->w_bar : h->x_bar, h->z_bar, h->y_bar, cx.data(), cy.data(), cz.data(),
+                   cw.data());
+    fetch_unscaled(h, 0, h->sv_x, q ? h->sv_w : h->sv_x, h->sv_z, h->sv_y, bx.data(), by.data(), bz.data(),
+                   bw.data());
+    const hprsocp_handle::Saved &S = h->saved;
+    hprsocp::NpzWriter f;
+    f.add_i64("current/iteration", iter);
+    f.add_f64("current/time_elapsed", elapsed);
+    f.add_str("current/timestamp", timestamp_now());
+    f.add_f64("current/x_org", cx.data(), n);
+    f.add_f64("current/w_org", cw.data(), n);
+    if (m > 0) f.add_f64("current/y_org", cy.data(), m);
+    f.add_f64("current/z_org", cz.data(), n);
+    f.add_f64("current/sigma", h->I(0).sigma);
+    f.add_f64("current/err_Rp", R.err_Rp);
+    f.add_f64("current/err_Rd", R.err_Rd);
+    f.add_f64("current/primal_obj", R.primal_obj);
+    f.add_f64("current/dual_obj", R.dual_obj);
+    f.add_f64("current/rel_gap", R.rel_gap);
+    f.add_f64("current/KKTx_and_gap", R.kkt);
+    f.add_f64("best/x_org", bx.data(), n);
+    f.add_f64("best/w_org", bw.data(), n);
+    if (m > 0) f.add_f64("best/y_org", by.data(), m);
+    f.add_f64("best/z_org", bz.data(), n);
+    f.add_f64("best/sigma", S.sigma);
+    f.add_i64("best/iteration", S.iter);
+    f.add_f64("best/err_Rp", S.err_Rp);
+    f.add_f64("best/err_Rd", S.err_Rd);
+    f.add_f64("best/primal_obj", S.primal_obj);
+    f.add_f64("best/dual_obj", S.dual_obj);
+    f.add_f64("best/rel_gap", S.rel_gap);
+    f.add_f64("best/KKTx_and_gap", jmax(jmax(S.err_Rp, S.err_Rd), S.rel_gap));
+    f.add_f64("parameters/stoptol", p.stoptol);
+    f.add_f64("parameters/sigma", p.sigma);
+    f.add_i64("parameters/max_iter", p.max_iter);
+    f.add_f64("parameters/time_limit", p.time_limit);
+    f.add_i64("parameters/check_iter", p.check_iter);
+    f.add_bool("parameters/warm_up", p.warm_up != 0);
+    f.add_i64("parameters/print_frequency", p.print_frequency);
+    f.add_i32("parameters/device_number", p.device_number);
+    f.add_bool("parameters/use_Ruiz_scaling", p.use_Ruiz_scaling != 0);
+    f.add_i64("parameters/ruiz_iterations", p.ruiz_iterations);
+    f.add_bool("parameters/use_bc_scaling", p.use_bc_scaling != 0);
+    f.add_str("parameters/bc_scaling_norm_type", bc_norm_name(p.bc_scaling_norm_type));
+    f.add_bool("parameters/use_l2_scaling", p.use_l2_scaling != 0);
+    f.add_bool("parameters/use_Pock_Chambolle_scaling", p.use_Pock_Chambolle_scaling != 0);
+    f.add_str("parameters/soc_block_scaling_strategy", soc_strategy_name(p.soc_block_scaling_strategy));
+    f.add_bool("parameters/auto_save", p.auto_save != 0);
+    if (!h->init_x.empty()) f.add_f64("parameters/initial_x", h->init_x.data(), (int64_t)h->init_x.size());
+    if (!h->init_y.empty()) f.add_f64("parameters/initial_y", h->init_y.data(), (int64_t)h->init_y.size());
+    std::string err;
+    if (!f.write(h->save_path, &err)) {
+        if (p.verbose) printf("Warning: Failed to save the auto-save file: %s\n", err.c_str());
+    } else if (p.verbose) {
+        printf("State saved to %s at iteration %lld\n", h->save_path.c_str(), (long long)iter);
+    }
+}
+
 double print_step(int64_t iter) {
     const double e = floor(log10((double)iter));
     return jmax(pow(10.0, e) / 10.0, 10.0);
@@ -2120,6 +2310,7 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
         R[i].best_sigma = h->I(i).sigma;
         h->I(i).active = 1;
     }
+    h->saved = hprsocp_handle::Saved{};
     upload_isc(h);
     if (h->Q_is_diag) update_q_factors(h);
     const bool single = nb == 1;
@@ -2149,7 +2340,10 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
     for (int64_t iter = 0; iter <= p.max_iter; ++iter) {
         const bool print_yes = should_print(p, iter, t_start);
         const bool check = iter % check_iter == 0;
-        if (check || print_yes) residuals_impl(h, iter, rs.data());  // synchronises the stream
+        if (check || print_yes) {
+            residuals_impl(h, iter, rs.data());  // synchronises the stream
+            if (p.auto_save) track_best(h, iter, rs[0]);
+        }
         if (check && (mn[0].pending || mn[1].pending)) {
             // deferred M-norms (perform_iteration_step!, algorithm.jl:2731-2741); the gaps are
             // only consumed by check_restart at multiples of check_iter
@@ -2208,6 +2402,8 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
             printf("%5lld    %3.2e    %3.2e    %+7.6e    %+7.6e    %3.2e    %3.2e    %6.2f\n", (long long)iter,
                    rs[0].err_Rp, rs[0].err_Rd, rs[0].primal_obj, rs[0].dual_obj, rs[0].rel_gap, h->I(0).sigma,
                    now_s() - t_start);
+        if (p.auto_save && (print_yes || status[0] != HPRSOCP_STATUS_CONTINUE))
+            autosave(h, iter, rs[0], now_s() - t_start);
         bool retired = false;
         for (int i = 0; i < nb; ++i) {
             if (!active[i] || !(check || print_yes)) continue;
@@ -2353,37 +2549,9 @@ int hprsocp_solve(hprsocp_handle *h, hprsocp_result *res) {
 
 namespace {
 // collect_results! (algorithm.jl:1222-1256) for instance i: unscaled x_bar,
-// y_bar, z_bar, w_bar gathered from the instance's column / row ranges.
+// y_bar, z_bar and w_bar (x_bar for problems without a quadratic term).
 void get_solution_impl(hprsocp_handle *h, int i, double *x, double *y, double *z, double *w) {
-    hipStream_t st = h->st;
-    const InstRange &R = h->rng[i];
-    const InstVal bsv = isc_field(h, &h->isc->b_scale), csv = isc_field(h, &h->isc->c_scale);
-    auto fetch_n = [&](double *dst) {
-        const int32_t nbx = R.box1 - R.box0, nvr = R.var1 - R.var0;
-        if (nbx > 0) HIPCHK(hipMemcpyAsync(dst, h->tmp_n + R.box0, sizeof(double) * nbx, hipMemcpyDeviceToHost, st));
-        if (nvr > 0)
-            HIPCHK(hipMemcpyAsync(dst + nbx, h->tmp_n + R.var0, sizeof(double) * nvr, hipMemcpyDeviceToHost, st));
-        h->sync();
-    };
-    if (x) {
-        launch_unscale(st, h->seg_n, h->x_bar, h->col_norm, bsv, 0, h->tmp_n);
-        fetch_n(x);
-    }
-    if (w) {
-        launch_unscale(st, h->seg_n, h->has_quad ? h->w_bar : h->x_bar, h->col_norm, bsv, 0, h->tmp_n);
-        fetch_n(w);
-    }
-    if (z) {
-        launch_unscale(st, h->seg_n, h->z_bar, h->col_norm, csv, 1, h->tmp_n);
-        fetch_n(z);
-    }
-    if (y && R.m() > 0) {
-        launch_unscale(st, h->seg_m, h->y_bar, h->row_norm, csv, 0, h->tmp_m);
-        const int32_t nl = R.lin1 - R.lin0, nc = R.con1 - R.con0;
-        if (nl > 0) HIPCHK(hipMemcpyAsync(y, h->tmp_m + R.lin0, sizeof(double) * nl, hipMemcpyDeviceToHost, st));
-        if (nc > 0) HIPCHK(hipMemcpyAsync(y + nl, h->tmp_m + R.con0, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
-        h->sync();
-    }
+    fetch_unscaled(h, i, h->x_bar, h->has_quad ? h->w_bar : h->x_bar, h->z_bar, h->y_bar, x, y, z, w);
 }
 }  // namespace
 

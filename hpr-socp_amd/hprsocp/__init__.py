@@ -12,6 +12,7 @@ there is no CPU fallback: if the library or a GPU is missing the call raises.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import os
 from dataclasses import dataclass, field
 from typing import Optional
@@ -24,6 +25,7 @@ from . import _abi
 __all__ = [
     "HPRSOCP_parameters", "HPRSOCP_results", "QP_info", "build_from_SOCP_data", "build_qp",
     "optimize", "Solver", "BatchSolver", "load_library", "HPRSOCPError", "build_from_cbf",
+    "load_autosave", "resume_parameters",
 ]
 
 _LIB = None
@@ -75,7 +77,7 @@ class HPRSOCP_parameters:
     initial_x: Optional[np.ndarray] = None
     initial_y: Optional[np.ndarray] = None
     auto_save: bool = False
-    save_filename: str = "HPRSOCP_autosave.h5"
+    save_filename: str = "HPRSOCP_autosave.npz"
     verbose: bool = True
     use_gpu: bool = True
     # extensions of this backend
@@ -117,6 +119,9 @@ class HPRSOCP_parameters:
             keep.append(a)
             p.initial_y = _abi.ptr_f64(a)
         p.auto_save = int(bool(self.auto_save))
+        fname = os.fsencode(self.save_filename)
+        keep.append(fname)
+        p.save_filename = fname
         p.verbose = int(bool(self.verbose))
         p.use_gpu = int(bool(self.use_gpu))
         p.precision = {"fp64": _abi.PREC_FP64, "fp32_refine": _abi.PREC_FP32_REFINE}[self.precision]
@@ -430,6 +435,40 @@ def optimize(model: QP_info, params: Optional[HPRSOCP_parameters] = None) -> HPR
         msg = lib.hprsocp_last_error(None)
         raise HPRSOCPError(rc, msg.decode() if msg else "optimize failed")
     return HPRSOCP_results.from_c(r, x, y, z, w)
+
+
+def load_autosave(filename: str) -> dict:
+    """Reads an auto-save checkpoint written by a solve with ``auto_save=True``:
+    the current / best / parameters groups of the reference's
+    save_state_to_hdf5! (src/algorithm.jl:306-423), keyed by the same dataset
+    paths ("best/x_org", "current/iteration", ...).  Vectors come back as
+    float64 arrays, scalars as Python float / int / bool / str.  The file is
+    an .npz archive, read with allow_pickle=False."""
+    out = {}
+    with np.load(filename, allow_pickle=False) as d:
+        for k in d.files:
+            a = d[k]
+            out[k] = a.item() if a.ndim == 0 else a
+    return out
+
+
+def resume_parameters(filename: str, params: Optional[HPRSOCP_parameters] = None, which: str = "best",
+                      use_sigma: bool = True) -> HPRSOCP_parameters:
+    """Warm-start parameters from an auto-save checkpoint: initial_x and
+    initial_y from the checkpoint's ``which`` group ("best" or "current"),
+    sigma from the same group when ``use_sigma``, every other field from
+    ``params`` (the defaults when None).  optimize() applies the initial points
+    as process_initial_points! does (reference src/algorithm.jl:2612-2677)."""
+    if which not in ("best", "current"):
+        raise ValueError("which must be 'best' or 'current'")
+    d = load_autosave(filename)
+    p = dataclasses.replace(params) if params is not None else HPRSOCP_parameters()
+    p.initial_x = np.array(d[f"{which}/x_org"], dtype=np.float64)
+    y = d.get(f"{which}/y_org")
+    p.initial_y = None if y is None else np.array(y, dtype=np.float64)
+    if use_sigma:
+        p.sigma = float(d[f"{which}/sigma"])
+    return p
 
 
 class BatchSolver:

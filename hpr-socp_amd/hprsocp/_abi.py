@@ -74,7 +74,8 @@ class Params(C.Structure):
         ("use_l2_scaling", C.c_int32), ("use_Pock_Chambolle_scaling", C.c_int32),
         ("soc_block_scaling_strategy", C.c_int32),
         ("initial_x", _f64p), ("initial_y", _f64p),
-        ("auto_save", C.c_int32), ("verbose", C.c_int32), ("use_gpu", C.c_int32),
+        ("auto_save", C.c_int32), ("save_filename", C.c_char_p),
+        ("verbose", C.c_int32), ("use_gpu", C.c_int32),
         ("precision", C.c_int32), ("use_graph", C.c_int32), ("refine_tol", C.c_double),
         ("fp32_stall_checks", C.c_int32), ("reserved", C.c_int32 * 7),
     ]

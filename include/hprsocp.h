@@ -145,7 +145,12 @@ typedef struct {
     int32_t soc_block_scaling_strategy; /* HPRSOCP_SOC_PHASE_TAPER            */
     const double *initial_x;   /* NULL or n                                   */
     const double *initial_y;   /* NULL or m                                   */
-    int32_t auto_save;         /* 0 (HDF5 auto-save is not provided)          */
+    int32_t auto_save;         /* 0; 1 writes the current / best / parameters
+                                  checkpoint of algorithm.jl:306-423 at every
+                                  print step and at termination (single-
+                                  problem handles; an .npz archive whose
+                                  member names are the HDF5 dataset paths)    */
+    const char *save_filename; /* "HPRSOCP_autosave.npz" (NULL => that name)  */
     int32_t verbose;           /* 1                                           */
     int32_t use_gpu;           /* 1 (0 is rejected: this backend is the GPU)  */
     /* extensions */

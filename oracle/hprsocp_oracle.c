@@ -209,6 +209,7 @@ void oracle_default_params(hprsocp_params *p) {
     p->use_l2_scaling = 0;
     p->use_Pock_Chambolle_scaling = 1;
     p->soc_block_scaling_strategy = HPRSOCP_SOC_PHASE_TAPER;
+    p->save_filename = "HPRSOCP_autosave.npz"; /* auto_save is ignored by the oracle */
     p->verbose = 1;
     p->use_gpu = 1;
     p->precision = HPRSOCP_PREC_FP64;
