@@ -13,6 +13,15 @@ constexpr int kTileRows = 256;    // max rows of one tile (one row per thread in
 constexpr int kStreamNnz = 2048;  // LDS product buffer of the CSR-stream SpMV paths
 constexpr int kSliceRows = 1024;  // rows of one slice tile
 constexpr int kSliceTileNnz = 60000;  // cap of one slice tile (per-slice offsets fit uint16)
+// Column-sorted slice tiles: a chunk of <= kSortChunk consecutive (row-major)
+// entries of one slice tile is stored sorted by column, each entry packed as
+// (column - slice start) << kSortPosBits | (row-major position in the chunk),
+// so that the lanes of one gather instruction hit few distinct cache lines.
+constexpr int kSortChunk = 8192;   // LDS products of one chunk (power of two)
+constexpr int kSortPosBits = 13;   // log2(kSortChunk)
+constexpr int kSortColBits = 32 - kSortPosBits;  // slice width limit 2^19
+constexpr int kSortBlock = 512;    // threads of the sorted-slice kernel
+constexpr int kSortRows = 4096;    // rows of one sorted slice tile
 
 enum TileKind : int32_t { TILE_LINEAR = 0, TILE_CONES = 1, TILE_HUGE = 2 };
 
@@ -60,7 +69,12 @@ struct CsrT {
     // stored per slice as uint16 relative to the slice tile's base:
     //   entries of (slice p, row r) start at s_base[p*(s_ntiles+1) + t] + s_off[p*nrows + r]
     // with t the slice tile holding r; the tile's end is s_base[... + t + 1].
+    // With s_sorted = 1 the entries of each chunk of kSortChunk are in column
+    // order and s_col holds packed words (see kSortChunk); s_ncols gives the
+    // slice starts (ncols * p / nslices).
     int32_t nslices = 1;
+    int32_t s_sorted = 0;
+    int32_t s_ncols = 0;
     const int32_t *s_base = nullptr;
     const uint16_t *s_off = nullptr;
     const int32_t *s_col = nullptr;

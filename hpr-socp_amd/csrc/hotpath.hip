@@ -324,6 +324,67 @@ __global__ __launch_bounds__(kBlock) void k_spmv_slices(CsrT<T> M, const T *__re
     }
 }
 
+// Column-sorted form of k_spmv_slices.  Each chunk of the slice tile's entries
+// is stored in column order, so consecutive lanes gather neighbouring columns
+// and one gather instruction touches far fewer cache lines than with the
+// row-major order (the gathers, not the stream, bound the row-major kernel on
+// random sparsity).  A product is written to LDS at its row-major position,
+// and every row sums its products in the row-major order, so the partial sums
+// are bit-identical to k_spmv_slices.  Row ranges live in registers: thread t
+// owns rows t, t + kSortBlock, ...
+template <typename T>
+__global__ __launch_bounds__(kSortBlock) void k_spmv_sorted(CsrT<T> M, const T *__restrict__ x) {
+    __shared__ T prod[kSortChunk];
+    const int P = M.nslices;
+    const int p = blockIdx.x % P;
+    const int tix = blockIdx.x / P;
+    const SliceTile S = M.s_tiles[tix];
+    const int tid = threadIdx.x;
+    const int nr = S.re - S.rb;
+    const int32_t *base = M.s_base + (int64_t)p * (M.s_ntiles + 1) + tix;
+    const int32_t k0 = base[0];
+    const int32_t tnnz = base[1] - k0;
+    const uint16_t *off = M.s_off + (int64_t)p * M.nrows + S.rb;
+    const T *__restrict__ xs = x + (int64_t)M.s_ncols * p / P;
+    constexpr int RPT = kSortRows / kSortBlock;
+    T acc[RPT];
+    int32_t ra[RPT], re[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int r = tid + j * kSortBlock;
+        acc[j] = T(0);
+        ra[j] = r < nr ? (int32_t)off[r] : 0;
+        re[j] = r < nr ? (r + 1 < nr ? (int32_t)off[r + 1] : tnnz) : 0;
+    }
+    const T *__restrict__ val = M.s_val + k0;
+    const uint32_t *__restrict__ pk = (const uint32_t *)M.s_col + k0;
+    constexpr uint32_t kPosMask = (1u << kSortPosBits) - 1;
+    for (int c0 = 0; c0 < tnnz; c0 += kSortChunk) {
+        const int c1 = min(c0 + kSortChunk, tnnz);
+#pragma unroll 4
+        for (int k = c0 + tid; k < c1; k += kSortBlock) {
+            const uint32_t w = __builtin_nontemporal_load(pk + k);
+            const T v = __builtin_nontemporal_load(val + k);
+            prod[w & kPosMask] = v * xs[w >> kSortPosBits];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int a = max(ra[j], c0), e = min(re[j], c1);
+            T s = acc[j];
+            for (int k = a; k < e; ++k) s += prod[k - c0];
+            acc[j] = s;
+        }
+        __syncthreads();
+    }
+    T *out = M.s_part + (int64_t)p * M.nrows + S.rb;
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int r = tid + j * kSortBlock;
+        if (r < nr) out[r] = acc[j];
+    }
+}
+
 // ------------------------------------------------------------ tile kernel
 // SRC: 0 = no SpMV, 1 = CSR SpMV inside the tile, 2 = sum of the XCD-slice partials
 enum : int { SRC_NONE = 0, SRC_CSR = 1, SRC_SLICES = 2 };
@@ -428,7 +489,9 @@ static void run_tiles(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, con
                       const Op &op, T *raw_buf, double *partials) {
     if (plan.ntiles == 0) return;
     if (SRC == SRC_CSR && M.nslices > 1) {
-        if (M.s_ntiles > 0)
+        if (M.s_ntiles > 0 && M.s_sorted)
+            hipLaunchKernelGGL((k_spmv_sorted<T>), dim3(M.s_ntiles * M.nslices), dim3(kSortBlock), 0, st, M, v);
+        else if (M.s_ntiles > 0)
             hipLaunchKernelGGL((k_spmv_slices<T>), dim3(M.s_ntiles * M.nslices), dim3(kBlock), 0, st, M, v);
         hipLaunchKernelGGL((k_tile<T, Op, SRC_SLICES, CONES>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles, M,
                            v, cone_idx, plan.huge, op, raw_buf, partials);

@@ -20,6 +20,7 @@
 #include <array>
 #include <chrono>
 #include <map>
+#include <thread>
 #include <tuple>
 #include <string>
 #include <vector>
@@ -689,8 +690,11 @@ std::vector<double> power_iteration(hprsocp_handle *h, bool onA) {
 
 // Regroups the entries of M slice-major (slice p = columns [p*C/P, (p+1)*C/P)),
 // fills the device copy of the (already scaled) values through a permutation,
-// and builds the slice row tiles.
-bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double *dval, int32_t P) {
+// and builds the slice row tiles.  `sorted` selects the column-sorted layout of
+// k_spmv_sorted: tiles of <= kSortRows rows whose entries in each slice fill
+// about one chunk of kSortChunk, each chunk stored in column order as packed
+// (column offset, row-major position) words.
+bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double *dval, int32_t P, bool sorted) {
     if (P <= 1 || hm.ci.empty()) return false;
     const int32_t R = hm.nrows;
     const int64_t nnz = (int64_t)hm.ci.size();
@@ -698,15 +702,21 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
         if (hm.rp[i + 1] - hm.rp[i] > kSliceTileNnz) return false;  // per-slice offsets must fit uint16
     std::vector<int64_t> bound(P + 1);
     for (int p = 0; p <= P; ++p) bound[p] = (int64_t)hm.ncols * p / P;
+    if (sorted)
+        for (int p = 0; p < P; ++p)
+            if (bound[p + 1] - bound[p] > (1ll << kSortColBits)) return false;  // column offsets must fit
     auto slice_of = [&](int32_t c) {
         int p = (int)((int64_t)c * P / std::max<int64_t>(hm.ncols, 1));
         while (p + 1 < P && c >= bound[p + 1]) ++p;
         while (p > 0 && c < bound[p]) --p;
         return p;
     };
-    // slice tiles: <= kSliceRows rows, about P * kStreamNnz nonzeros, <= kSliceTileNnz
+    std::vector<int32_t> cnt((size_t)P * R, 0);
+    for (int32_t i = 0; i < R; ++i)
+        for (int32_t k = hm.rp[i]; k < hm.rp[i + 1]; ++k) cnt[(size_t)slice_of(hm.ci[k]) * R + i]++;
     std::vector<SliceTile> tiles;
-    {
+    if (!sorted) {
+        // <= kSliceRows rows, about P * kStreamNnz nonzeros, <= kSliceTileNnz
         const int64_t cap = std::min<int64_t>((int64_t)P * kStreamNnz, kSliceTileNnz);
         int32_t r = 0;
         while (r < R) {
@@ -719,11 +729,26 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
             tiles.push_back(SliceTile{r, e});
             r = e;
         }
+    } else {
+        // <= kSortRows rows, <= kSortChunk entries in every slice (a single
+        // longer row forms a tile of its own; the kernel then runs several chunks)
+        std::vector<int64_t> tc((size_t)P);
+        int32_t r = 0;
+        while (r < R) {
+            for (int p = 0; p < P; ++p) tc[p] = cnt[(size_t)p * R + r];
+            int32_t e = r + 1;
+            while (e < R && e - r < kSortRows) {
+                bool fits = true;
+                for (int p = 0; p < P && fits; ++p) fits = tc[p] + cnt[(size_t)p * R + e] <= kSortChunk;
+                if (!fits) break;
+                for (int p = 0; p < P; ++p) tc[p] += cnt[(size_t)p * R + e];
+                ++e;
+            }
+            tiles.push_back(SliceTile{r, e});
+            r = e;
+        }
     }
     const int32_t NT = (int32_t)tiles.size();
-    std::vector<int32_t> cnt((size_t)P * R, 0);
-    for (int32_t i = 0; i < R; ++i)
-        for (int32_t k = hm.rp[i]; k < hm.rp[i + 1]; ++k) cnt[(size_t)slice_of(hm.ci[k]) * R + i]++;
     // slice-major entry order: slice p, then tile, then row
     std::vector<int32_t> base((size_t)P * (NT + 1));
     std::vector<uint16_t> off((size_t)P * R);
@@ -750,7 +775,42 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
             scol[d] = hm.ci[k];
             perm[d] = k;
         }
+    if (sorted) {
+        // each chunk: packed words sorted ascending (= by column, then by
+        // row-major position), the value permutation reordered alongside
+        const int64_t items = (int64_t)P * NT;
+        const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(16, items),
+                                                                    std::thread::hardware_concurrency()));
+        auto work = [&](int w) {
+            std::vector<uint32_t> key;
+            std::vector<int32_t> pv;
+            for (int64_t it = w; it < items; it += nth) {
+                const int p = (int)(it / NT);
+                const int32_t t = (int32_t)(it % NT);
+                const int64_t b0 = base[(size_t)p * (NT + 1) + t], b1 = base[(size_t)p * (NT + 1) + t + 1];
+                for (int64_t c0 = b0; c0 < b1; c0 += kSortChunk) {
+                    const int64_t c1 = std::min<int64_t>(c0 + kSortChunk, b1);
+                    const int32_t len = (int32_t)(c1 - c0);
+                    key.resize(len);
+                    pv.assign(perm.begin() + c0, perm.begin() + c1);
+                    for (int32_t j = 0; j < len; ++j)
+                        key[j] = ((uint32_t)(scol[c0 + j] - bound[p]) << kSortPosBits) | (uint32_t)j;
+                    std::sort(key.begin(), key.end());
+                    for (int32_t j = 0; j < len; ++j) {
+                        scol[c0 + j] = (int32_t)key[j];
+                        perm[c0 + j] = pv[key[j] & ((1u << kSortPosBits) - 1)];
+                    }
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int w = 1; w < nth; ++w) pool.emplace_back(work, w);
+        work(0);
+        for (auto &th : pool) th.join();
+    }
     M.nslices = P;
+    M.s_sorted = sorted ? 1 : 0;
+    M.s_ncols = hm.ncols;
     M.s_base = h->upload<int32_t>(base.data(), (int64_t)base.size());
     M.s_off = h->upload<uint16_t>(off.data(), (int64_t)off.size());
     M.s_col = h->upload<int32_t>(scol.data(), nnz);
@@ -787,21 +847,24 @@ void free_plan(hprsocp_handle *h, const DevPlan &p) {
 // Autotuner of the XCD slicing: for an operator whose input vector outgrows
 // ~2 MB it builds the sliced forms with 4 and 8 column slices (8: one slice per
 // XCD; 4: two XCDs share a slice, halving the partial-sum traffic while the
-// slice still fits one L2), times one SpMV of each form and of the plain form
-// with HIP events (median of 5 after a warm-up) and keeps the fastest with its
-// epilogue plan (nnz-balanced tiles for the plain form, row tiles for sliced
-// forms).  HPRSOCP_XCD_SLICES=<P> forces P slices (tests).
+// slice still fits one L2), each in row-major and in column-sorted layout,
+// times one SpMV of each form and of the plain form with HIP events (median of
+// 5 after a warm-up) and keeps the fastest with its epilogue plan
+// (nnz-balanced tiles for the plain form, row tiles for sliced forms).
+// HPRSOCP_XCD_SLICES=<P> forces P slices and HPRSOCP_XCD_SORTED=<0|1> the
+// layout (tests).
 void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm, const double *dval,
                  int32_t lin_end, const std::vector<int32_t> &cidx, int32_t ncones, const char *name,
                  const std::vector<int32_t> *row_inst) {
     const char *env = getenv("HPRSOCP_XCD_SLICES");
+    const char *env_sorted = getenv("HPRSOCP_XCD_SORTED");
     if (hm.ci.empty()) return;
-    std::vector<int32_t> cands;
+    std::vector<std::pair<int32_t, bool>> cands;
     if (env) {
         const int32_t P = std::max(1, atoi(env));
-        if (P > 1) cands.push_back(P);
+        if (P > 1) cands.push_back({P, env_sorted && atoi(env_sorted) != 0});
     } else if (hm.ncols * (int64_t)sizeof(double) > (2ll << 20)) {
-        cands = {4, 8};
+        cands = {{4, false}, {8, false}, {4, true}, {8, true}};
     }
     if (cands.empty()) return;
     double *xin = nullptr, *yout = nullptr;
@@ -831,12 +894,15 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
     CsrView bestM = M;
     DevPlan bestP = plan;
     bool sliced = false;
-    for (int32_t P : cands) {
+    for (const auto &cand : cands) {
+        const int32_t P = cand.first;
         CsrView S = M;
-        if (!build_slices(h, S, hm, dval, P)) continue;
+        if (!build_slices(h, S, hm, dval, P, cand.second)) continue;
         DevPlan splan = make_plan(h, {}, hm.nrows, lin_end, cidx, ncones, row_inst);
         const float t = env ? 0.0f : time_one(S, splan);
-        if (h->params.verbose) printf("XCD slicing autotune %s: %d slices %.4f ms (best so far %.4f ms)\n", name, P, t, best_t);
+        if (h->params.verbose)
+            printf("XCD slicing autotune %s: %d slices%s %.4f ms (best so far %.4f ms)\n", name, P,
+                   cand.second ? " column-sorted" : "", t, best_t);
         if (t < best_t) {
             if (sliced) {
                 free_slices(h, bestM);
@@ -976,6 +1042,8 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz) {
     launch_demote(h->st, nnz, M.val, v);
     f.val = v;
     f.nslices = M.nslices;
+    f.s_sorted = M.s_sorted;
+    f.s_ncols = M.s_ncols;
     if (M.nslices > 1) {
         f.s_base = M.s_base;
         f.s_off = M.s_off;

@@ -239,18 +239,22 @@ def test_gpu_rejects_bad_input(gpu_lib):
 # ------------------------------------------------------------ XCD-sliced SpMV path
 @pytest.fixture
 def forced_slices(request, monkeypatch):
-    monkeypatch.setenv("HPRSOCP_XCD_SLICES", str(request.param))
+    """(slices, column-sorted layout) forced on every operator of the next create."""
+    P, srt = request.param
+    monkeypatch.setenv("HPRSOCP_XCD_SLICES", str(P))
+    monkeypatch.setenv("HPRSOCP_XCD_SORTED", str(int(srt)))
     return request.param
 
 
 SLICED = ["c0", "qp_csr", "qp_factored", "huge_cones", "c1_tiny", "c2_tiny"]
 
 
-@pytest.mark.parametrize("forced_slices", [8, 3], indirect=True)
+@pytest.mark.parametrize("forced_slices", [(8, False), (3, False), (8, True), (3, True)], indirect=True)
 @pytest.mark.parametrize("name", SLICED)
 def test_sliced_spmv_and_iterations(name, forced_slices, oracle_lib, gpu_lib):
-    """Column-sliced SpMV (forced on small models): SpMV vs scipy, then 30 HPR
-    steps, residuals and M-norm against the oracle."""
+    """Column-sliced SpMV (forced on small models), row-major and column-sorted
+    layouts: SpMV vs scipy, then 30 HPR steps, residuals and M-norm against the
+    oracle."""
     m, g, o = pair(name, oracle_lib, gpu_lib)
     with g, o:
         A = m.A.tocsr()
@@ -275,3 +279,73 @@ def test_sliced_spmv_and_iterations(name, forced_slices, oracle_lib, gpu_lib):
         for k in ro:
             assert rg[k] == pytest.approx(ro[k], rel=1e-9, abs=1e-13), k
         assert g.mnorm() == pytest.approx(o.mnorm(), rel=1e-9, abs=1e-14)
+
+
+_extra = {}
+
+
+def _slice_model(name):
+    """c1 at 2% scale: several slice tiles per slice; dense_rows: rows with more
+    than one sorted chunk of entries in a slice (SpMV only, not a solvable model)."""
+    if name not in _extra:
+        if name == "c1_small":
+            _extra[name] = synthetic.make_c1(scale=0.02)
+        elif name == "dense_rows":
+            rng = np.random.default_rng(4)
+            n = 60000
+            A = sp.random(300, n, density=0.002, format="lil", random_state=rng, data_rvs=rng.standard_normal)
+            for r in (17, 150):
+                cols = rng.choice(n, 55000, replace=False)
+                A[r, cols] = rng.standard_normal(55000)
+            A = A.tocsr()
+            m = A.shape[0]
+            _extra[name] = hprsocp.build_qp(A, rng.standard_normal(n), -np.ones(m), np.ones(m),
+                                            -np.ones(n), np.ones(n), [], m, n, [m], [n])
+        else:
+            _extra[name] = model(name)
+    return _extra[name]
+
+
+def _sliced_run(name, P, srt, monkeypatch, precision="fp64", solve=True):
+    monkeypatch.setenv("HPRSOCP_XCD_SLICES", str(P))
+    monkeypatch.setenv("HPRSOCP_XCD_SORTED", str(int(srt)))
+    m = _slice_model(name)
+    with hprsocp.Solver(m, params(stoptol=1e-6, max_iter=3000, precision=precision)) as g:
+        rng = np.random.default_rng(2)
+        v, w = rng.standard_normal(m.A.shape[1]), rng.standard_normal(m.A.shape[0])
+        ax, atw = g.spmv(v), g.spmv(w, trans=True)
+        As = sp.csr_matrix((g.scaled_A(), m.A.indices, m.A.indptr), shape=m.A.shape)
+        close(ax, As @ v, 1e-13)
+        close(atw, As.T @ w, 1e-13)
+        r = None
+        if solve:
+            g.estimate_lambda()
+            r = g.solve()
+    return ax, atw, r
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+@pytest.mark.parametrize("name", ["c1_small", "c2_tiny", "qp_factored"])
+def test_sorted_slices_bit_identical(name, precision, monkeypatch, gpu_lib):
+    """The column-sorted layout only reorders the gathers: every row still sums
+    its products in row-major order, so SpMV results and (up to 3000 iterations
+    of) whole solves equal the row-major sliced form bit for bit."""
+    print(f"[case] sorted vs row-major slices {name} {precision}", flush=True)
+    a = _sliced_run(name, 8, False, monkeypatch, precision)
+    b = _sliced_run(name, 8, True, monkeypatch, precision)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    print(f"[result] {name} {precision}: iter {a[2].iter} / {b[2].iter} status {a[2].status}", flush=True)
+    assert a[2].iter == b[2].iter and a[2].primal_obj == b[2].primal_obj
+    np.testing.assert_array_equal(a[2].x, b[2].x)
+    np.testing.assert_array_equal(a[2].y, b[2].y)
+
+
+def test_sorted_slices_multi_chunk_rows(monkeypatch, gpu_lib):
+    """Rows with more entries in a slice than one sorted chunk holds (55000
+    entries over 3 slices): SpMV vs scipy and bit-identical to row-major."""
+    print("[case] sorted slices, rows longer than a chunk", flush=True)
+    a = _sliced_run("dense_rows", 3, False, monkeypatch, solve=False)
+    b = _sliced_run("dense_rows", 3, True, monkeypatch, solve=False)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
