@@ -39,7 +39,7 @@ PEAK_HBM_GBS = 8000.0
 # write factor 1.0).  Source: profiles/c1_fp64_iteration_hbm_traffic.txt.
 # Keyed by (config, precision, group); only full-size runs use it.
 MEASURED_TRAFFIC = {
-    ("c1", "fp64", "dual"): 549.1e6 + 172.3e6,   # k_spmv_slices(A) + k_tile<OpDual, slices>
+    ("c1", "fp64", "dual"): 537.4e6 + 172.4e6,   # k_spmv_sorted(A, 8 slices) + k_tile<OpDual, slices>
 }
 
 
@@ -134,6 +134,27 @@ def solve_record(r):
             "primal_obj": r.primal_obj, "restarts": r.restarts, "iter_fp32": r.iter_fp32}
 
 
+def gather_measure(g, model, slices=8):
+    """Gathers per second of the iteration's two big SpMVs (A x and A^T y, as
+    the autotuner left them) next to the gather ceilings the probe kernel
+    measures for the same entry count and slice width: random column order
+    (the access pattern of the row-major layouts) and column-sorted chunks."""
+    m, n = model.A.shape
+    nnz = int(model.A.nnz)
+    out = {}
+    for name, trans, width_total in (("A", 0, n), ("AT", 1, m)):
+        t = g.time_spmv(trans, 20)
+        width = -(-width_total // slices)
+        rec = {"entries": nnz, "spmv_ms": round(t, 5), "G_per_s": round(nnz / t / 1e6, 1)}
+        if nnz >= 8192 and width <= (1 << 19):
+            pr = g.probe_gather(nnz, width, slices, False)
+            ps = g.probe_gather(nnz, width, slices, True)
+            rec["ceiling_random_G_per_s"] = round(nnz / pr / 1e6, 1)
+            rec["ceiling_sorted_G_per_s"] = round(nnz / ps / 1e6, 1)
+        out[name] = rec
+    return out
+
+
 def bench_single(args):
     import hprsocp
     hprsocp.load_library()
@@ -159,8 +180,11 @@ def bench_single(args):
         ms, _ = g.time_iterations(args.steps)
         _, kern = g.time_iterations(max(args.steps // 4, 10), per_kernel=True)
         log(f"[{prec}] {ms:.4f} ms/iter -> {1000.0 / ms:.1f} iters/s; per-group ms {kern}")
+        gather = gather_measure(g, model) if prec == precs[0] and not args.no_gather else None
+        if gather:
+            log(f"[{prec}] SpMV gathers vs probe ceilings: {gather}")
         g.close()
-        rec = {"ms_per_iter": ms, "kern": kern, "sc": sc, "solve": None}
+        rec = {"ms_per_iter": ms, "kern": kern, "sc": sc, "solve": None, "gather": gather}
         if not args.no_solve:
             t0 = time.time()
             with hprsocp.Solver(model, p) as g2:
@@ -217,7 +241,8 @@ def bench_single(args):
                    "iteration_GBps": round(it_bytes / (ms * 1e-3) / 1e9, 1)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": traffic, "algorithmic_bytes": int(dom_bytes), "ms": round(dom_ms, 5)},
+                     "traffic": traffic, "algorithmic_bytes": int(dom_bytes), "ms": round(dom_ms, 5),
+                     "gather": base["gather"]},
         "kernel_ms": {name: round(t, 5) for name, t in groups},
         "forms": forms,
         "cpu_baseline": {"value": cpu_rate and round(cpu_rate, 4), "unit": "HPR iters/s", "cores": 1,
@@ -337,6 +362,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched C3 measurement")
     ap.add_argument("--no-graph", action="store_true", help="launch every update eagerly (no HIP graphs)")
+    ap.add_argument("--no-gather", action="store_true", help="skip the SpMV gather-ceiling measurement")
     ap.add_argument("--precision", default="both", choices=["fp64", "fp32_refine", "both"])
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--seq-sample", type=int, default=32)

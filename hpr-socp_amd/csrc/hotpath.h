@@ -170,4 +170,15 @@ template <typename T>
 void launch_spmv_tempv(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, const T *wbar,
                        const TempvArgsT<T> &a);
 
+// Gather-ceiling probe: synthetic entries shaped like the sliced SpMV's
+// (8-byte value + packed column/position word, one gather each from the slice
+// of block b % slices).  probe_fill writes nch chunks of kStreamNnz (random
+// columns) or kSortChunk (column-sorted chunks) entries; probe_gather runs
+// the gather phase of k_spmv_slices (random) or k_spmv_sorted (sorted) on them
+// and writes per-thread sums to out (nch x threads per block).
+void launch_probe_fill(hipStream_t st, int64_t nch, int32_t width, int sorted, uint32_t *pk, double *val);
+void launch_probe_gather(hipStream_t st, int64_t nch, int32_t slices, int32_t width, int sorted, const uint32_t *pk,
+                         const double *val, const double *x, double *out);
+constexpr int kProbePosBits = 11;  // log2(kStreamNnz): position bits of the random-order probe
+
 }  // namespace hpr

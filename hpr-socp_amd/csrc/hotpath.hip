@@ -10,6 +10,8 @@
 // owns one cone, reduces |xi|^2 in FP64 with shuffles and applies the SOC
 // projection in registers; HUGE tiles store raw values and a partial |xi|^2,
 // and k_huge_finish projects once all partials of the cone exist.
+#include <algorithm>
+
 #include "devutil.h"
 #include "hotpath.h"
 
@@ -537,6 +539,89 @@ template <typename T>
 void launch_spmv_tempv(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, const T *wbar,
                        const TempvArgsT<T> &a) {
     run_tiles<T, OpTempv<T>, SRC_CSR, false>(st, plan, M, wbar, nullptr, OpTempv<T>{a}, nullptr, nullptr);
+}
+
+// ------------------------------------------------------ gather ceiling probe
+__device__ __forceinline__ uint64_t probe_hash(uint64_t z) {  // SplitMix64 finaliser
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Entry k of chunk c = k / C, j = k % C.  Random order: a uniform column,
+// position j.  Sorted order: the columns of a sorted uniform sample of C
+// points of the slice — each wavefront's 64 entries start at j0 * width / C
+// and advance by exponential gaps of mean width / C (an inclusive shuffle scan
+// over the wavefront; every wavefront of the launch is whole because n and the
+// grid stride are multiples of 64) — with positions an odd-multiplier
+// permutation of j.
+__global__ void k_probe_fill(int64_t n, int32_t width, int sorted, uint32_t *pk, double *val) {
+    const int C = sorted ? kSortChunk : kStreamNnz;
+    const int bits = sorted ? kSortPosBits : kProbePosBits;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t j = (uint32_t)(k % C);
+        const uint64_t r = probe_hash((uint64_t)k);
+        uint32_t col, pos;
+        if (sorted) {
+            const double step = (double)width / C;
+            const double u = (double)((r >> 11) + 1) * 0x1.0p-53;  // (0, 1]
+            const double gap = -log(u) * step;
+            double sc = gap;
+            const int lane = (int)(threadIdx.x & 63);
+            for (int o = 1; o < 64; o <<= 1) {
+                const double t = __shfl_up(sc, o, 64);
+                if (lane >= o) sc += t;
+            }
+            const int64_t c = (int64_t)((double)(j & ~63u) * step + (sc - gap));
+            col = (uint32_t)(c < width ? c : width - 1);
+            pos = (j * 40503u) & (uint32_t)(C - 1);
+        } else {
+            col = (uint32_t)(r % (uint64_t)width);
+            pos = j;
+        }
+        pk[k] = (col << bits) | pos;
+        val[k] = 1.0;
+    }
+}
+
+template <int C, int NT, int BITS>
+__global__ __launch_bounds__(NT) void k_probe_gather(int32_t slices, int32_t width, const uint32_t *__restrict__ pk,
+                                                     const double *__restrict__ val, const double *__restrict__ x,
+                                                     double *__restrict__ out) {
+    __shared__ double prod[C];
+    const int64_t b = blockIdx.x;
+    const double *__restrict__ xs = x + (int64_t)(b % slices) * width;
+    const int64_t base = b * (int64_t)C;
+    constexpr uint32_t mask = (1u << BITS) - 1;
+#pragma unroll 4
+    for (int k = threadIdx.x; k < C; k += NT) {
+        const uint32_t w = __builtin_nontemporal_load(pk + base + k);
+        const double v = __builtin_nontemporal_load(val + base + k);
+        prod[w & mask] = v * xs[w >> BITS];
+    }
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < C / NT; ++k) s += prod[threadIdx.x * (C / NT) + k];
+    out[b * NT + threadIdx.x] = s;
+}
+
+void launch_probe_fill(hipStream_t st, int64_t nch, int32_t width, int sorted, uint32_t *pk, double *val) {
+    const int64_t n = nch * (sorted ? kSortChunk : kStreamNnz);
+    if (n <= 0) return;
+    const int64_t blocks = std::min<int64_t>((n + kBlock - 1) / kBlock, 65536);
+    hipLaunchKernelGGL(k_probe_fill, dim3((unsigned)blocks), dim3(kBlock), 0, st, n, width, sorted, pk, val);
+}
+
+void launch_probe_gather(hipStream_t st, int64_t nch, int32_t slices, int32_t width, int sorted, const uint32_t *pk,
+                         const double *val, const double *x, double *out) {
+    if (nch <= 0) return;
+    if (sorted)
+        hipLaunchKernelGGL((k_probe_gather<kSortChunk, kSortBlock, kSortPosBits>), dim3((unsigned)nch),
+                           dim3(kSortBlock), 0, st, slices, width, pk, val, x, out);
+    else
+        hipLaunchKernelGGL((k_probe_gather<kStreamNnz, kBlock, kProbePosBits>), dim3((unsigned)nch), dim3(kBlock),
+                           0, st, slices, width, pk, val, x, out);
 }
 
 #define HPR_INSTANTIATE(T)                                                                                        \

@@ -890,10 +890,18 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
     }
-    float best_t = env ? INFINITY : time_one(M, plan);
-    CsrView bestM = M;
-    DevPlan bestP = plan;
-    bool sliced = false;
+    // every candidate is timed; the fastest wins unless a form earlier in the
+    // preference order (8 sorted, 8, 4 sorted, 4, plain) is within 2% of it,
+    // which keeps the choice stable between runs when forms time alike
+    struct Form {
+        CsrView M;
+        DevPlan plan;
+        float t;
+        int rank;
+        bool sliced;
+    };
+    std::vector<Form> forms;
+    forms.push_back(Form{M, plan, env ? INFINITY : time_one(M, plan), 4, false});
     for (const auto &cand : cands) {
         const int32_t P = cand.first;
         CsrView S = M;
@@ -901,22 +909,23 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
         DevPlan splan = make_plan(h, {}, hm.nrows, lin_end, cidx, ncones, row_inst);
         const float t = env ? 0.0f : time_one(S, splan);
         if (h->params.verbose)
-            printf("XCD slicing autotune %s: %d slices%s %.4f ms (best so far %.4f ms)\n", name, P,
-                   cand.second ? " column-sorted" : "", t, best_t);
-        if (t < best_t) {
-            if (sliced) {
-                free_slices(h, bestM);
-                free_plan(h, bestP);
-            }
-            best_t = t;
-            bestM = S;
-            bestP = splan;
-            sliced = true;
-        } else {
-            free_slices(h, S);
-            free_plan(h, splan);
-        }
+            printf("XCD slicing autotune %s: %d slices%s %.4f ms (plain %.4f ms)\n", name, P,
+                   cand.second ? " column-sorted" : "", t, forms[0].t);
+        forms.push_back(Form{S, splan, t, (P >= 8 ? 0 : 2) + (cand.second ? 0 : 1), true});
     }
+    float tmin = INFINITY;
+    for (const Form &f : forms) tmin = std::min(tmin, f.t);
+    size_t pick = 0;
+    for (size_t k = 0; k < forms.size(); ++k)
+        if (forms[k].t <= 1.02f * tmin && (forms[pick].t > 1.02f * tmin || forms[k].rank < forms[pick].rank)) pick = k;
+    for (size_t k = 1; k < forms.size(); ++k)
+        if (k != pick) {
+            free_slices(h, forms[k].M);
+            free_plan(h, forms[k].plan);
+        }
+    const bool sliced = forms[pick].sliced;
+    const CsrView bestM = forms[pick].M;
+    const DevPlan bestP = forms[pick].plan;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     h->dfree(xin);
@@ -1805,6 +1814,89 @@ int hprsocp_probe_copy(hprsocp_handle *h, int64_t bytes, double *ms) {
         (void)hipEventDestroy(e1);
         (void)hipFree(a);
         (void)hipFree(b);
+    });
+    return 0;
+}
+
+// Median HIP-event time of `reps` launches of one operator's SpMV as the
+// iteration runs it (sliced or plain form, stored-product epilogue) on
+// device-resident operands: A x (trans = 0) or A^T y (trans = 1).
+int hprsocp_time_spmv(hprsocp_handle *h, int32_t trans, int32_t reps, double *ms) {
+    ABI_TRY(h, {
+        if (reps < 1 || !ms) throw Err{HPRSOCP_ERR_INVALID, "time_spmv: reps >= 1 and an output are required"};
+        if (h->nb != 1) throw Err{HPRSOCP_ERR_UNSUPPORTED, "time_spmv applies to single-problem handles"};
+        const CsrView &M = trans ? h->AT : h->A;
+        const DevPlan &pl = trans ? h->plan_AT : h->plan_A;
+        double *din = trans ? h->tmp_m : h->tmp_n;
+        double *dout = trans ? h->tmp_n2 : h->ax;
+        launch_fill(h->st, trans ? h->m : h->n, 1.0, din);
+        launch_spmv(h->st, pl, M, din, dout);  // warm-up
+        h->sync();
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        std::vector<float> t((size_t)reps);
+        for (int r = 0; r < reps; ++r) {
+            HIPCHK(hipEventRecord(e0, h->st));
+            launch_spmv(h->st, pl, M, din, dout);
+            HIPCHK(hipEventRecord(e1, h->st));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&t[r], e0, e1));
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        std::sort(t.begin(), t.end());
+        *ms = t[t.size() / 2];
+    });
+    return 0;
+}
+
+// Gather-ceiling probe: nnz synthetic entries (rounded down to whole chunks)
+// gathered from `slices` slices of `width` doubles, in random or column-sorted
+// chunk order; median time of 5 launches after a warm-up.
+int hprsocp_probe_gather(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t slices, int32_t sorted, double *ms) {
+    ABI_TRY(h, {
+        const int C = sorted ? kSortChunk : kStreamNnz;
+        const int bits = sorted ? kSortPosBits : kProbePosBits;
+        if (!ms || slices < 1 || width < 1 || nnz < C || (int64_t)width > (1ll << (32 - bits)))
+            throw Err{HPRSOCP_ERR_INVALID, "probe_gather: need nnz >= one chunk, 1 <= width <= 2^(32-position bits), slices >= 1"};
+        const int NT = sorted ? kSortBlock : kBlock;
+        const int64_t nch = nnz / C;
+        if (nch > (int64_t)INT32_MAX) throw Err{HPRSOCP_ERR_INVALID, "probe_gather: too many chunks"};
+        void *pk = nullptr, *val = nullptr, *x = nullptr, *out = nullptr;
+        auto release = [&]() {
+            (void)hipFree(pk);
+            (void)hipFree(val);
+            (void)hipFree(x);
+            (void)hipFree(out);
+        };
+        if (hipMalloc(&pk, (size_t)(nch * C) * 4) != hipSuccess || hipMalloc(&val, (size_t)(nch * C) * 8) != hipSuccess ||
+            hipMalloc(&x, (size_t)slices * width * 8) != hipSuccess || hipMalloc(&out, (size_t)(nch * NT) * 8) != hipSuccess) {
+            release();
+            throw Err{HPRSOCP_ERR_ALLOC, "probe_gather alloc"};
+        }
+        launch_fill(h->st, (int64_t)slices * width, 1.0, (double *)x);
+        launch_probe_fill(h->st, nch, width, sorted, (uint32_t *)pk, (double *)val);
+        launch_probe_gather(h->st, nch, slices, width, sorted, (const uint32_t *)pk, (const double *)val,
+                            (const double *)x, (double *)out);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        std::vector<float> t(5);
+        for (int r = 0; r < 5; ++r) {
+            HIPCHK(hipEventRecord(e0, h->st));
+            launch_probe_gather(h->st, nch, slices, width, sorted, (const uint32_t *)pk, (const double *)val,
+                                (const double *)x, (double *)out);
+            HIPCHK(hipEventRecord(e1, h->st));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&t[r], e0, e1));
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        h->sync();
+        release();
+        std::sort(t.begin(), t.end());
+        *ms = t[2];
     });
     return 0;
 }

@@ -49,7 +49,7 @@ def load_library(path: Optional[str] = None) -> C.CDLL:
         raise RuntimeError(
             f"HIP backend library not found at {p}; build it with "
             "`python -c 'import __graft_entry__ as g; g.build()'` (make -C hpr-socp_amd/csrc)")
-    lib = _abi.bind(C.CDLL(p), "hprsocp_")
+    lib = _abi.bind_hooks(_abi.bind(C.CDLL(p), "hprsocp_"))
     if path is None:
         _LIB = lib
     return lib
@@ -416,6 +416,19 @@ class Solver:
         self._check(self.lib.hprsocp_time_iterations(self.h, int(iters), C.byref(ms),
                                                       kern if per_kernel else None))
         return ms.value, list(kern) if per_kernel else None
+
+    def time_spmv(self, trans: bool = False, reps: int = 20) -> float:
+        """Median ms of the iteration's A x (or A^T y) SpMV on resident operands."""
+        ms = C.c_double()
+        self._check(self.lib.hprsocp_time_spmv(self.h, int(bool(trans)), int(reps), C.byref(ms)))
+        return ms.value
+
+    def probe_gather(self, nnz: int, width: int, slices: int = 8, sorted_chunks: bool = False) -> float:
+        """Median ms of the synthetic gather-ceiling kernel (see include/hprsocp.h)."""
+        ms = C.c_double()
+        self._check(self.lib.hprsocp_probe_gather(self.h, int(nnz), int(width), int(slices),
+                                                   int(bool(sorted_chunks)), C.byref(ms)))
+        return ms.value
 
 
 def optimize(model: QP_info, params: Optional[HPRSOCP_parameters] = None) -> HPRSOCP_results:

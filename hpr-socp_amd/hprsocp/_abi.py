@@ -154,6 +154,23 @@ _BATCH_SIGS = {
 }
 
 
+# Measurement hooks (HIP backend only).
+_HOOK_SIGS = {
+    "time_iterations": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), _f64p]),
+    "probe_copy": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double)]),
+    "time_spmv": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    "probe_gather": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+}
+
+
+def bind_hooks(lib: C.CDLL, prefix: str = "hprsocp_"):
+    for name, (res, args) in _HOOK_SIGS.items():
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
 def bind_batch(lib: C.CDLL, prefix: str = "hprsocp_"):
     for name, (res, args) in _BATCH_SIGS.items():
         fn = getattr(lib, prefix + name)

@@ -291,6 +291,23 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
  * of known traffic.  (No counterpart in the reference; measurement hook.) */
 int hprsocp_probe_copy(hprsocp_handle *h, int64_t bytes, double *ms);
 
+/* Median HIP-event time (ms) of `reps` launches of the SpMV A x (trans = 0)
+ * or A^T y (trans = 1) exactly as the iteration runs it (sliced or plain form
+ * chosen by the autotuner), on device-resident operands.  Single-problem
+ * handles.  (No counterpart in the reference; measurement hook.) */
+int hprsocp_time_spmv(hprsocp_handle *h, int32_t trans, int32_t reps, double *ms);
+
+/* Gather ceiling of the sliced SpMV: times a synthetic kernel that streams
+ * nnz entries of 12 bytes (8-byte value, 4-byte packed column/position word)
+ * and gathers one double per entry from `slices` column slices of `width`
+ * doubles (block b reads slice b % slices, as the sliced SpMV does), in random
+ * column order (sorted = 0, the row-major layout's access pattern) or in
+ * column-sorted chunks (sorted = 1).  The data is generated on the device.
+ * Returns the median of 5 launches in ms.  (No counterpart in the reference;
+ * measurement hook.) */
+int hprsocp_probe_gather(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t slices, int32_t sorted,
+                         double *ms);
+
 /* ------------------------------------------------------------ CBF input */
 /* Reads a Conic Benchmark Format file into the canonical layout, as read_cbf
  * + build_from_cbf do (reference src/utils/cbf_io.jl:493-750, 899-1131,
