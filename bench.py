@@ -111,11 +111,24 @@ def oracle_lib():
     return olib
 
 
-def cpu_baseline(model, sigma, lam_a, lam_q, iters):
-    """The oracle (FP64 C restatement of the reference CPU path), 1 thread, on a
-    bounded sample: `iters` HPR iterations of the same instance after setup."""
+def host_cores():
+    """Host cores this process may use, capped at the 16-core share of one GPU."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(model, sigma, lam_a, lam_q, iters, threads=1):
+    """The oracle (FP64 C restatement of the reference CPU path) on a bounded
+    sample: `iters` HPR iterations of the same instance after setup.  `threads`
+    > 1 runs its sparse products with OpenMP (the rest stays serial, as in the
+    reference's single-threaded CPU path)."""
     import hprsocp
     olib = oracle_lib()
+    olib.oracle_set_threads.argtypes = [C.c_int32]
+    olib.oracle_set_threads(int(threads))
     p = hprsocp.HPRSOCP_parameters(verbose=False)
     with hprsocp.Solver(model, p, lib=olib, prefix="oracle_") as o:
         o.set_scalars(sigma=sigma, lambda_A=lam_a, lambda_Q=lam_q)
@@ -124,14 +137,43 @@ def cpu_baseline(model, sigma, lam_a, lam_q, iters):
         for k in range(iters):
             o.update(k + 1, False)
         dt = time.perf_counter() - t0
+    olib.oracle_set_threads(1)
     return iters / dt, dt
 
 
-def solve_record(r):
+def device_info():
+    """hipDeviceProp figures of device 0 (name, CUs, memory size, memory clock
+    and bus width), read through the HIP runtime, to set beside the 8 TB/s
+    published peak."""
+    info = {}
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        return info
+    name = C.create_string_buffer(256)
+    if hip.hipDeviceGetName(name, 256, 0) == 0:
+        info["name"] = name.value.decode(errors="replace")
+    total = C.c_size_t()
+    if hip.hipDeviceTotalMem(C.byref(total), 0) == 0:
+        info["mem_GiB"] = round(total.value / 2**30, 1)
+    v = C.c_int()
+    # hipDeviceAttribute_t: MultiprocessorCount = 63, MemoryClockRate = 60 (kHz), MemoryBusWidth = 59 (bits)
+    for key, attr in (("cus", 63), ("mem_clock_kHz", 60), ("mem_bus_bits", 59)):
+        if hip.hipDeviceGetAttribute(C.byref(v), attr, 0) == 0:
+            info[key] = v.value
+    return info
+
+
+def solve_record(r, stoptol=1e-6):
     hit6 = r.residuals < 1e-6 or r.iter_6 < r.iter
-    return {"status": r.status, "iter": r.iter, "time_s": round(r.time, 3), "kkt": r.residuals,
-            "iter_to_1e-6": r.iter_6 if hit6 else None, "time_to_1e-6_s": round(r.time_6, 3) if hit6 else None,
-            "primal_obj": r.primal_obj, "restarts": r.restarts, "iter_fp32": r.iter_fp32}
+    rec = {"status": r.status, "iter": r.iter, "time_s": round(r.time, 3), "kkt": r.residuals,
+           "iter_to_1e-6": r.iter_6 if hit6 else None, "time_to_1e-6_s": round(r.time_6, 3) if hit6 else None,
+           "primal_obj": r.primal_obj, "restarts": r.restarts, "iter_fp32": r.iter_fp32}
+    if stoptol < 1e-6:
+        ok = r.status == "OPTIMAL"
+        rec[f"time_to_{stoptol:g}_s"] = round(r.time, 3) if ok else None
+        rec[f"iter_to_{stoptol:g}"] = r.iter if ok else None
+    return rec
 
 
 def gather_measure(g, model, slices=8):
@@ -165,8 +207,8 @@ def bench_single(args):
     precs = ["fp64", "fp32_refine"] if args.precision == "both" else [args.precision]
     res = {}
     for prec in precs:
-        p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, time_limit=args.solve_time, precision=prec,
-                                       use_graph=not args.no_graph)
+        p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=args.single_stoptol, time_limit=args.solve_time,
+                                       precision=prec, use_graph=not args.no_graph)
         t0 = time.time()
         g = hprsocp.Solver(model, p)
         log(f"[{prec}] setup (transfer + scaling + plans) {time.time() - t0:.2f}s")
@@ -190,7 +232,7 @@ def bench_single(args):
             with hprsocp.Solver(model, p) as g2:
                 g2.estimate_lambda()
                 r = g2.solve()
-            rec["solve"] = solve_record(r)
+            rec["solve"] = solve_record(r, args.single_stoptol)
             log(f"[{prec}] solve: {rec['solve']} (wall {time.time() - t0:.1f}s)")
         res[prec] = rec
 
@@ -205,12 +247,16 @@ def bench_single(args):
     it_bytes = sum(bytes_by.values())
     traffic = MEASURED_TRAFFIC.get((args.config, precs[0], dom)) if args.scale == 1.0 else None
 
+    ncores = host_cores()
     if args.cpu_iters > 0:
         sc = base["sc"]
         cpu_rate, cpu_dt = cpu_baseline(model, sc["sigma"], sc["lambda_max_A"], sc["lambda_max_Q"], args.cpu_iters)
         log(f"cpu oracle: {cpu_rate:.3f} iters/s over {args.cpu_iters} iterations ({cpu_dt:.1f}s)")
+        mt_rate, mt_dt = cpu_baseline(model, sc["sigma"], sc["lambda_max_A"], sc["lambda_max_Q"], args.cpu_iters,
+                                      threads=ncores)
+        log(f"cpu oracle, {ncores} threads: {mt_rate:.3f} iters/s ({mt_dt:.1f}s)")
     else:
-        cpu_rate, cpu_dt = None, 0.0
+        cpu_rate, cpu_dt, mt_rate, mt_dt = None, 0.0, None, 0.0
 
     forms = {}
     for prec, rec in res.items():
@@ -235,6 +281,7 @@ def bench_single(args):
         "vs_baseline": None,
         "dtype": "f32" if precs[0] == "fp32_refine" else "f64",
         "data": "synthetic (seeded feasibility construction)",
+        "device": device_info(),
         "config": {"workload": wl, "config": args.config, "scale": args.scale, "m": m, "n": n,
                    "nnz_A": int(model.A.nnz), "precision": precs[0],
                    "algorithmic_MB_per_iter": round(it_bytes / 1e6, 2),
@@ -248,7 +295,10 @@ def bench_single(args):
         "cpu_baseline": {"value": cpu_rate and round(cpu_rate, 4), "unit": "HPR iters/s", "cores": 1,
                          "kind": "port",
                          "sample": f"{args.cpu_iters} partial HPR iterations of the same instance "
-                                   f"after setup (FP64 C oracle, 1 thread, {cpu_dt:.1f}s)"},
+                                   f"after setup (FP64 C oracle, 1 thread, {cpu_dt:.1f}s)",
+                         "all_cores": {"value": mt_rate and round(mt_rate, 4), "cores": ncores,
+                                       "sample": f"the same iterations with OpenMP sparse products "
+                                                 f"({mt_dt:.1f}s)"}},
     }
 
 
@@ -369,6 +419,7 @@ def main():
     ap.add_argument("--c4-count", type=int, default=64)
     ap.add_argument("--c4-seq-sample", type=int, default=8)
     ap.add_argument("--stoptol", type=float, default=1e-6, help="stoptol of the batched configs")
+    ap.add_argument("--single-stoptol", type=float, default=1e-6, help="stoptol of the single-problem solves")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()

@@ -41,10 +41,19 @@ struct Tile {
 
 // Per-instance scalars of the iteration, read by every tile of the instance.
 // h1 = 1/(inner0 + kstep + 2) with kstep the iterations since inner0 was set.
+// active = 0 retires the instance; fp32 selects the precision its updates run
+// in (the float kernels update only fp32 instances, the double kernels only
+// the others).
 struct InstScal {
     double sigma, lambda_A, lambda_Q, b_scale, c_scale, inner0;
-    int32_t active, pad;
+    int32_t active, fp32;
 };
+
+// Whether the update kernels of value type T work on this instance.
+template <typename T>
+__host__ __device__ inline bool inst_runs(const InstScal &s) {
+    return s.active != 0 && (s.fp32 != 0) == (sizeof(T) == sizeof(float));
+}
 
 // Long cones split over several tiles (two-phase projection).
 struct HugeInfo {

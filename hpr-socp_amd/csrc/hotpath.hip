@@ -129,7 +129,7 @@ struct OpNoQPrimal {
         sigma = (T)s.sigma;
         h1 = (T)d1;
         h2 = (T)d2;
-        return s.active != 0;
+        return inst_runs<T>(s);
     }
     __device__ T raw(int i, T acc, bool) const { return a.x[i] + sigma * (acc - a.c[i]); }
     __device__ T lin_proj(int i, T r) const { return jclamp(r, a.l[i], a.u[i]); }
@@ -159,7 +159,7 @@ struct OpDual {
         f2 = (T)(1.0 / g);
         h1 = (T)d1;
         h2 = (T)d2;
-        return s.active != 0;
+        return inst_runs<T>(s);
     }
     __device__ T raw(int i, T acc, bool cone) const {
         return cone ? (acc - a.rhs[i]) - f1 * a.y[i] : acc - f1 * a.y[i];
@@ -191,7 +191,7 @@ struct OpQPPrimal {
         h2 = (T)d2;
         f2s = (T)q2;
         f1s = (T)(1.0 - q2);
-        return s.active != 0;
+        return inst_runs<T>(s);
     }
     __device__ T qw(int i) const { return a.diag ? a.dq[i] * a.w[i] : a.qw[i]; }
     __device__ T raw(int i, T, bool) const { return a.x[i] + sigma * ((-qw(i) + a.aty[i]) - a.c[i]); }
@@ -223,7 +223,7 @@ struct OpW2 {
         fs = (T)(s.sigma / (1.0 + s.sigma * s.lambda_Q));
         h1 = (T)d1;
         h2 = (T)d2;
-        return s.active != 0;
+        return inst_runs<T>(s);
     }
     __device__ T raw(int, T acc, bool) const { return acc; }
     __device__ T lin_proj(int, T r) const { return r; }
@@ -260,7 +260,7 @@ struct OpTempv {
     __device__ bool bind(int inst) {
         const InstScal s = a.isc[inst];
         sigma = (T)s.sigma;
-        return s.active != 0;
+        return inst_runs<T>(s);
     }
     __device__ T raw(int, T acc, bool) const { return acc; }
     __device__ T lin_proj(int, T r) const { return r; }

@@ -53,6 +53,8 @@ void launch_mul(hipStream_t st, int64_t n, const double *a, const double *b, dou
 // precision conversion of the FP32 mirrors (round to nearest / exact widening)
 void launch_demote(hipStream_t st, int64_t n, const double *src, float *dst);
 void launch_promote(hipStream_t st, int64_t n, const float *src, double *dst);
+// dst = (double)src on the chunks of the active instances whose updates run in FP32
+void launch_promote_prec(hipStream_t st, const SegPlan &sp, const float *src, double *dst, const InstScal *isc);
 
 // ---- residual kernels (compute_Rd!/compute_Rp!, kernels.jl:118-292) ---------
 struct ResidNArgs {

@@ -646,6 +646,16 @@ void launch_unscale(hipStream_t st, const SegPlan &sp, const double *v, const do
                     double *out) {
     if (sp.nchunks > 0) hipLaunchKernelGGL(k_unscale, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, v, d, a, mul, out);
 }
+__global__ void k_promote_prec(SegPlan sp, const float *src, double *dst, const InstScal *isc) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const InstScal s = isc[ch.inst];
+    if (!s.active || !s.fp32) return;
+    CHUNK_LOOP(i, ch) dst[i] = (double)src[i];
+}
+void launch_promote_prec(hipStream_t st, const SegPlan &sp, const float *src, double *dst, const InstScal *isc) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_promote_prec, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, src, dst, isc);
+}
+
 __global__ void k_copy_flagged(SegPlan sp, const double *src, double *dst, const int32_t *flag) {
     const SegChunk ch = sp.chunks[blockIdx.x];
     if (!flag[ch.inst]) return;

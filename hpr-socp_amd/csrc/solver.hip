@@ -211,6 +211,7 @@ struct hprsocp_handle {
     // the update launches once instances retire
     HostPlan hp_A, hp_AT, hp_Q;
     DevPlan act_A, act_AT, act_Q;
+    DevPlan act32_A, act32_AT, act32_Q;  // the same for the instances running in FP32
     bool use_act = false;
     int32_t *con_idx = nullptr, *var_idx = nullptr, *row_cone = nullptr;
     std::vector<int32_t> h_con_idx, h_var_idx;
@@ -249,7 +250,7 @@ struct hprsocp_handle {
     int64_t kstep = 0;  // updates since InstScal::inner0 was set
     // FP32 mirrors of the iteration (precision FP32_REFINE)
     bool fp32 = false;         // mirrors allocated
-    bool fp32_active = false;  // updates run on the mirrors
+    int32_t n32 = 0, n64 = 0;  // active instances whose updates run in FP32 / FP64
     bool fp32_dirty = true;    // FP64 state changed since the last demotion
     int64_t nnzF = 0;
     IterView<float> v32{};
@@ -395,6 +396,9 @@ void read_slots(hprsocp_handle *h) {
 void upload_isc(hprsocp_handle *h) {
     HIPCHK(hipMemcpyAsync(h->isc, h->isc_h, sizeof(InstScal) * h->nb, hipMemcpyHostToDevice, h->st));
     h->sync();
+    h->n32 = h->n64 = 0;
+    for (int i = 0; i < h->nb; ++i)
+        if (h->isc_h[i].active) ++(h->isc_h[i].fp32 ? h->n32 : h->n64);
 }
 // per-instance scratch values: ivtmp[k*nb + i]
 InstVal ivals(hprsocp_handle *h, int k, const std::vector<double> &v) {
@@ -1104,13 +1108,14 @@ void demote_state(hprsocp_handle *h) {
     h->fp32_dirty = false;
 }
 
-// FP32 state -> FP64 (exact widening), for the FP64 checks, restarts and reads.
+// FP32 state -> FP64 (exact widening) of the active instances running in FP32,
+// for the FP64 checks, restarts and reads.  The FP64 state of instances
+// running in FP64 is left alone (their mirrors are stale).
 void promote_state(hprsocp_handle *h) {
     IterView<float> &v = h->v32;
-    const int64_t n = h->n, m = h->m;
     hipStream_t st = h->st;
-#define HPR_PRO_N(f) launch_promote(st, n, v.f, h->f);
-#define HPR_PRO_M(f) launch_promote(st, m, v.f, h->f);
+#define HPR_PRO_N(f) launch_promote_prec(st, h->seg_n, v.f, h->f, h->isc);
+#define HPR_PRO_M(f) launch_promote_prec(st, h->seg_m, v.f, h->f, h->isc);
     HPR_STATE_N(HPR_PRO_N) HPR_STATE_M(HPR_PRO_M)
 #undef HPR_PRO_N
 #undef HPR_PRO_M
@@ -1373,6 +1378,11 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
         h->act_A = alloc_active(h, h->plan_A);
         h->act_AT = alloc_active(h, h->plan_AT);
         if (h->nnzQ > 0) h->act_Q = alloc_active(h, h->plan_Q);
+        if (P.precision == HPRSOCP_PREC_FP32_REFINE) {
+            h->act32_A = alloc_active(h, h->plan_A);
+            h->act32_AT = alloc_active(h, h->plan_AT);
+            if (h->nnzQ > 0) h->act32_Q = alloc_active(h, h->plan_Q);
+        }
     }
     {
         std::vector<int32_t> rc((size_t)m, -1);
@@ -1457,12 +1467,12 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
             h->I(i).sigma = P.sigma;
         }
     }
-    upload_isc(h);
-    if (h->Q_is_diag) update_q_factors(h);
     if (P.precision == HPRSOCP_PREC_FP32_REFINE) {
         setup_fp32(h);
-        h->fp32_active = true;
+        for (int i = 0; i < h->nb; ++i) h->I(i).fp32 = 1;
     }
+    upload_isc(h);
+    if (h->Q_is_diag) update_q_factors(h);
     h->to_check = 1;
     h->sync();
     h->setup_time = now_s() - t0;
@@ -1606,9 +1616,10 @@ template <typename T>
 static void do_update_t(hprsocp_handle *h, IterView<T> &V, int64_t kstep, int full) {
     hipStream_t st = h->st;
     const int32_t m = h->m;
-    const DevPlan &pA = h->use_act ? h->act_A : h->plan_A;
-    const DevPlan &pAT = h->use_act ? h->act_AT : h->plan_AT;
-    const DevPlan &pQ = h->use_act ? h->act_Q : h->plan_Q;
+    const bool f32 = sizeof(T) == sizeof(float);
+    const DevPlan &pA = h->use_act ? (f32 ? h->act32_A : h->act_A) : h->plan_A;
+    const DevPlan &pAT = h->use_act ? (f32 ? h->act32_AT : h->act_AT) : h->plan_AT;
+    const DevPlan &pQ = h->use_act ? (f32 ? h->act32_Q : h->act_Q) : h->plan_Q;
     auto ev = [&](int k) {
         if (h->profile) HIPCHK(hipEventRecord(h->ev[k], st));
     };
@@ -1666,20 +1677,43 @@ static void set_inner(hprsocp_handle *h, const std::vector<int64_t> &inner) {
     h->kstep = 0;
 }
 
-// One iteration in the handle's current precision.  In FP32 the mirrors are
-// refreshed from FP64 first if the FP64 state changed, and the FP64 state is
+// One iteration: the float kernels update the instances running in FP32, the
+// double kernels the others.  The FP32 mirrors are refreshed from FP64 first if
+// the FP64 state changed, and the FP64 state of the FP32 instances is
 // refreshed from the mirrors after a full (check) iteration or when `sync64`.
 static void do_update(hprsocp_handle *h, int full, bool sync64 = false) {
     h->to_check = full;
     const int64_t k = h->kstep++;
-    if (h->fp32_active) {
+    if (h->n32 > 0) {
         if (h->fp32_dirty) demote_state(h);
         do_update_t<float>(h, h->v32, k, full);
         if (full || sync64) promote_state(h);
-    } else {
+    }
+    if (h->n64 > 0) {
         IterView<double> V = view64(h);
         do_update_t<double>(h, V, k, full);
     }
+}
+
+// Tile lists of the running instances, one set per precision, after
+// retirements or precision switches; the captured graphs no longer fit.
+static void compact_active(hprsocp_handle *h) {
+    const int nb = h->nb;
+    std::vector<char> k64(nb), k32(nb);
+    for (int i = 0; i < nb; ++i) {
+        k64[i] = h->I(i).active && !h->I(i).fp32;
+        k32[i] = h->I(i).active && h->I(i).fp32;
+    }
+    compact_plan(h, h->hp_A, h->act_A, k64);
+    compact_plan(h, h->hp_AT, h->act_AT, k64);
+    if (h->nnzQ > 0) compact_plan(h, h->hp_Q, h->act_Q, k64);
+    if (h->fp32) {
+        compact_plan(h, h->hp_A, h->act32_A, k32);
+        compact_plan(h, h->hp_AT, h->act32_AT, k32);
+        if (h->nnzQ > 0) compact_plan(h, h->hp_Q, h->act32_Q, k32);
+    }
+    h->use_act = true;
+    h->clear_graphs();
 }
 
 // L consecutive partial updates.  With params.use_graph the run is captured
@@ -1694,8 +1728,8 @@ static void run_partial_updates(hprsocp_handle *h, int64_t L) {
         return;
     }
     h->to_check = 0;
-    if (h->fp32_active && h->fp32_dirty) demote_state(h);
-    const int fp = h->fp32_active ? 1 : 0;
+    if (h->n32 > 0 && h->fp32_dirty) demote_state(h);
+    const int fp = (h->n32 > 0 ? 1 : 0) | (h->n64 > 0 ? 2 : 0);
     const auto key = std::make_tuple(h->kstep, (int32_t)L, fp);
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
@@ -1705,8 +1739,8 @@ static void run_partial_updates(hprsocp_handle *h, int64_t L) {
         HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
         IterView<double> V = view64(h);
         for (int64_t k = 0; k < L; ++k) {
-            if (fp) do_update_t<float>(h, h->v32, h->kstep + k, 0);
-            else do_update_t<double>(h, V, h->kstep + k, 0);
+            if (fp & 1) do_update_t<float>(h, h->v32, h->kstep + k, 0);
+            if (fp & 2) do_update_t<double>(h, V, h->kstep + k, 0);
         }
         HIPCHK(hipStreamEndCapture(h->st, &g));
         const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -2489,14 +2523,15 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
     MnormCtx mn[2];
     std::vector<char> pend0(nb, 0), pend1(nb, 0);
     h->to_check = 1;
-    // FP32 bulk: windows of FP32 iterations, FP64 checks; FP64 iterations take
-    // over once every active instance reaches refine_tol or stops improving
-    h->fp32_active = h->fp32;
+    // FP32 bulk: windows of FP32 iterations, FP64 checks; an instance continues
+    // in FP64 once it reaches refine_tol or stops improving
+    for (int i = 0; i < nb; ++i) h->I(i).fp32 = h->fp32 ? 1 : 0;
+    upload_isc(h);
     h->fp32_dirty = true;
     std::vector<double> best32(nb, INFINITY);
     std::vector<int> stall32(nb, 0);
     std::vector<int64_t> restarts32(nb, 0);
-    int64_t iter32 = 0;
+    std::vector<int64_t> iter32(nb, 0);
     for (int64_t iter = 0; iter <= p.max_iter; ++iter) {
         const bool print_yes = should_print(p, iter, t_start);
         const bool check = iter % check_iter == 0;
@@ -2526,15 +2561,13 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
             else if (iter == p.max_iter) status[i] = HPRSOCP_STATUS_MAX_ITER;
             else if (tnow > p.time_limit) status[i] = HPRSOCP_STATUS_TIME_LIMIT;
         }
-        if (h->fp32_active && iter > 0 && check) {
+        if (h->n32 > 0 && iter > 0 && check) {
             // the FP64 state holds this check's iterate (the last update was a full one).
             // Stalled: no 1% improvement of the best KKT over fp32_stall_checks checks
             // that also saw two restarts (a long restart cycle alone is not a stall).
-            bool all_ready = true, any = false;
-            double worst = 0.0;
+            int switched = 0;
             for (int i = 0; i < nb; ++i) {
-                if (!active[i] || status[i] != HPRSOCP_STATUS_CONTINUE) continue;
-                any = true;
+                if (!active[i] || !h->I(i).fp32 || status[i] != HPRSOCP_STATUS_CONTINUE) continue;
                 if (rs[i].kkt < 0.99 * best32[i]) {
                     best32[i] = rs[i].kkt;
                     stall32[i] = 0;
@@ -2543,12 +2576,19 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
                     ++stall32[i];
                 }
                 const bool stalled = stall32[i] >= p.fp32_stall_checks && R[i].times - restarts32[i] >= 2;
-                all_ready = all_ready && (rs[i].kkt < p.refine_tol || stalled);
-                worst = jmax(worst, rs[i].kkt);
+                if (rs[i].kkt < p.refine_tol || stalled) {
+                    h->I(i).fp32 = 0;
+                    ++switched;
+                    if (p.verbose && single)
+                        printf("precision: FP64 iterations from iteration %lld (KKT %.3e)\n", (long long)iter, rs[i].kkt);
+                }
             }
-            if (any && all_ready) {
-                h->fp32_active = false;
-                if (p.verbose) printf("precision: FP64 iterations from iteration %lld (KKT %.3e)\n", (long long)iter, worst);
+            if (switched) {
+                upload_isc(h);
+                if (nb > 1) compact_active(h);
+                if (p.verbose && !single)
+                    printf("precision: %d instance(s) continue in FP64 from iteration %lld; %d run in FP32\n", switched,
+                           (long long)iter, h->n32);
             }
         }
         // as in the reference loop, restart logic runs also at an instance's last iteration
@@ -2602,7 +2642,7 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
             o.lambda_max_A = h->I(i).lambda_A;
             o.lambda_max_Q = h->I(i).lambda_Q;
             o.restarts = (int32_t)R[i].times;
-            o.iter_fp32 = iter32;
+            o.iter_fp32 = iter32[i];
             active[i] = 0;
             h->I(i).active = 0;
             --nactive;
@@ -2619,10 +2659,12 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
                 printf("batch done: %d instances, %lld iterations, %.3f s\n", nb, (long long)iter, now_s() - t_start);
                 fflush(stdout);
             }
-            h->fp32_active = h->fp32;
             h->use_act = false;
             h->clear_graphs();
-            for (int i = 0; i < nb; ++i) h->I(i).active = 1;  // the handle stays usable after the solve
+            for (int i = 0; i < nb; ++i) {  // the handle stays usable after the solve
+                h->I(i).active = 1;
+                h->I(i).fp32 = h->fp32 ? 1 : 0;
+            }
             upload_isc(h);
             return;
         }
@@ -2632,15 +2674,8 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
                 if (active[i]) worst = jmax(worst, rs[i].kkt);
             printf("iter %lld: %d active, worst KKT %.3e, %.2f s\n", (long long)iter, nactive, worst, now_s() - t_start);
         }
-        if (retired && nb > 1) {  // launch only the tiles of instances still running
-            std::vector<char> keep(nb);
-            for (int i = 0; i < nb; ++i) keep[i] = h->I(i).active != 0;
-            compact_plan(h, h->hp_A, h->act_A, keep);
-            compact_plan(h, h->hp_AT, h->act_AT, keep);
-            if (h->nnzQ > 0) compact_plan(h, h->hp_Q, h->act_Q, keep);
-            h->use_act = true;
-            h->clear_graphs();
-        }
+        if (retired) upload_isc(h);
+        if (retired && nb > 1) compact_active(h);  // launch only the tiles of instances still running
         // Halpern counters: set once per check window (restarts happen only at checks)
         if (check || retired) {
             for (int i = 0; i < nb; ++i) inner[i] = R[i].inner;
@@ -2660,13 +2695,15 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
                 return p.print_frequency == -1 ? !rem_zero(t, print_step(t)) : (t % p.print_frequency != 0);
             };
             while (plain(nx + L) && L < check_iter) ++L;
+            for (int i = 0; i < nb; ++i)
+                if (active[i] && h->I(i).fp32) iter32[i] += L;
             run_partial_updates(h, L);
-            if (h->fp32_active) iter32 += L;
             for (int i = 0; i < nb; ++i)
                 if (active[i]) R[i].inner += L - 1;
             iter += L - 1;
         } else {
-            if (h->fp32_active) ++iter32;
+            for (int i = 0; i < nb; ++i)
+                if (active[i] && h->I(i).fp32) ++iter32[i];
             do_update(h, 1);
         }
         const bool need_cur = (nx % check_iter) == 0;
@@ -2922,7 +2959,7 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
             h->profile = false;
             for (int j = 0; j < 6; ++j) ms_kernel[j] = iters > 0 ? acc[j] / iters : 0.0;
         }
-        if (h->fp32_active) promote_state(h);
+        if (h->n32 > 0) promote_state(h);
         h->sync();
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

@@ -178,3 +178,20 @@ def test_autosave_write_failure_keeps_solving(tmp_path, gpu_lib):
 def test_autosave_rejected_for_batches(gpu_lib):
     with pytest.raises(hprsocp.HPRSOCPError):
         hprsocp.BatchSolver([model("c0")], params(auto_save=True))
+
+
+def test_autosave_without_rows(tmp_path, gpu_lib):
+    """m = 0 (the reference's "SOC variables only" solve, runtests.jl:544-562):
+    no y_org datasets, as save_state_to_hdf5! writes them only when m > 0."""
+    from kat_models import kats, model_from
+    s = kats()["solve_soc_variables_only"]
+    m = model_from(s)
+    path = tmp_path / "norows.npz"
+    r = run(m, path, stoptol=s["stoptol"], max_iter=s["max_iter"])
+    assert r.status == "OPTIMAL"
+    d = hprsocp.load_autosave(str(path))
+    assert "current/y_org" not in d and "best/y_org" not in d
+    assert d["best/x_org"].shape == (3,)
+    np.testing.assert_array_equal(d["current/x_org"], r.x)
+    p = hprsocp.resume_parameters(str(path))
+    assert p.initial_y is None and p.initial_x.shape == (3,)

@@ -122,6 +122,25 @@ def test_batch_fp32(gpu_lib):
         assert cert["err_Rp"] < 1e-5 and cert["err_Rd"] < 1e-5
 
 
+@pytest.mark.parametrize("kind", ["noq", "diag"])
+def test_batch_fp32_instances_follow_single_solves(kind, gpu_lib):
+    """Each instance switches from FP32 to FP64 on its own (refine_tol or
+    stall), so in FP32 mode too an instance of a batch follows its
+    single-problem solve bit for bit: same iterations, same FP32 iterations,
+    same objective."""
+    models = noq_models() if kind == "noq" else diag_models()
+    print(f"[case] fp32 batch {kind} vs single fp32 solves", flush=True)
+    rb = batch_solve(models, stoptol=1e-8, precision="fp32_refine")
+    for i, (m, r) in enumerate(zip(models, rb)):
+        rs = single_solve(m, stoptol=1e-8, precision="fp32_refine")
+        print(f"[result] {kind}[{i}] batch iter {r.iter} (fp32 {r.iter_fp32}) single {rs.iter} (fp32 {rs.iter_fp32})",
+              flush=True)
+        assert r.status == rs.status == "OPTIMAL"
+        assert (r.iter, r.iter_fp32) == (rs.iter, rs.iter_fp32)
+        assert r.primal_obj == rs.primal_obj
+        np.testing.assert_array_equal(r.x, rs.x)
+
+
 def test_batch_rejects_mixed_q(gpu_lib):
     a = synthetic.make_small_qp(seed=1, q="none")
     b = synthetic.make_small_qp(seed=2, q="diag")
