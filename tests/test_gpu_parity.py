@@ -78,6 +78,30 @@ def test_scaling_parity(name, oracle_lib, gpu_lib):
         close(g.scaled_A(), o.scaled_A(), 1e-12)
 
 
+SCALING_OPTIONS = [dict(soc_block_scaling_strategy=k) for k in sorted(hprsocp._abi.SOC_STRATEGIES)] + [
+    dict(bc_scaling_norm_type="rms"), dict(bc_scaling_norm_type="linf"),
+    dict(use_Ruiz_scaling=False), dict(use_Pock_Chambolle_scaling=False), dict(use_bc_scaling=False),
+    dict(ruiz_iterations=3, use_bc_scaling=False, use_Pock_Chambolle_scaling=False),
+]
+
+
+@pytest.mark.parametrize("opts", SCALING_OPTIONS, ids=lambda o: "-".join(f"{k}={v}" for k, v in o.items()))
+@pytest.mark.parametrize("name", ["huge_cones", "c2_tiny"])
+def test_scaling_options_parity(name, opts, oracle_lib, gpu_lib):
+    """Every SOC block rule (scaling.jl:253-477), every bc norm and every switch
+    of the scaling pipeline: the GPU's scaled problem equals the oracle's
+    (two independent restatements) to 1e-12, on a model with long and short
+    cones of both kinds and on one with 20k size-3 cones."""
+    m, g, o = pair(name, oracle_lib, gpu_lib, **opts)
+    with g, o:
+        sg, so = g.scalars(), o.scalars()
+        for k in ("b_scale", "c_scale", "norm_b", "norm_c", "sigma"):
+            assert sg[k] == pytest.approx(so[k], rel=1e-12, abs=1e-300), k
+        for v in ("row_norm", "col_norm", "c", "AL", "AU", "l", "u", "soc_rhs_full"):
+            close(g.get(v), o.get(v), 1e-12)
+        close(g.scaled_A(), o.scaled_A(), 1e-12)
+
+
 @pytest.mark.parametrize("name", list(MODELS))
 def test_lambda_parity(name, oracle_lib, gpu_lib):
     m, g, o = pair(name, oracle_lib, gpu_lib)

@@ -306,3 +306,39 @@ def test_oracle_c0_certificate(oracle_lib):
     assert r.status == "OPTIMAL"
     cert = certificate(model, r.x, r.y, r.z)
     assert cert["err_Rp"] < 1e-5 and cert["err_Rd"] < 1e-5 and cert["rel_gap"] < 1e-5
+
+
+@pytest.mark.parametrize("strategy", sorted(hprsocp._abi.SOC_STRATEGIES))
+def test_oracle_soc_block_rules(oracle_lib, strategy):
+    """Every SOC block rule after one Ruiz pass (no bc, no Pock-Chambolle): each
+    constraint cone gets one row factor.  Without the two extra max passes of
+    the pre-Pock strategies (scaling.jl:136-146, :1008-1040) that factor lies
+    between the smallest and the largest factor its rows would get alone (sqrt
+    of the row inf-norm) — the rules (scaling.jl:253-477) all aggregate
+    max/rms-type statistics of the block — and equals the largest for the max
+    rule.  Then the oracle solves the model with the full pipeline to a
+    certified 1e-6."""
+    model = synthetic.make_small_qp(seed=5, q="diag", var_sizes=(3, 12, 40), m1=10, con_sizes=(4, 30, 9),
+                                    density=0.5)
+    A = model.A.tocsr()
+    rmax = np.abs(A).max(axis=1).toarray().ravel()
+    assert np.all(rmax > 0)
+    own = np.sqrt(rmax)
+    p = params(soc_block_scaling_strategy=strategy, ruiz_iterations=1, use_bc_scaling=False,
+               use_Pock_Chambolle_scaling=False)
+    with osolver(oracle_lib, model, p) as o:
+        rn = o.get("row_norm")
+    ci = np.asarray(model.SOC_con_idx) - 1
+    for k in range(len(ci) - 1):
+        blk = slice(ci[k], ci[k + 1])
+        assert np.ptp(rn[blk]) == 0.0
+        f = rn[ci[k]]
+        if 10 <= hprsocp._abi.SOC_STRATEGIES[strategy] <= 18:  # pre-Pock stage: extra passes
+            continue
+        assert own[blk].min() * (1 - 1e-12) <= f <= own[blk].max() * (1 + 1e-12), (k, f)
+        if strategy == "max":
+            assert f == pytest.approx(own[blk].max(), rel=1e-14)
+    r = _solve(oracle_lib, model, soc_block_scaling_strategy=strategy, stoptol=1e-6, max_iter=200000)
+    assert r.status == "OPTIMAL"
+    cert = certificate(model, r.x, r.y, r.z)
+    assert cert["err_Rp"] < 1e-5 and cert["err_Rd"] < 1e-5
