@@ -70,6 +70,10 @@ struct CsrT {
     const int32_t *rowptr = nullptr, *col = nullptr;
     const T *val = nullptr;
     int32_t nrows = 0;
+    // When every row's columns are consecutive (dense row blocks), c0[r] is the
+    // first column of row r and the lane-group SpMV reads v[c0[r] + (k - rowptr[r])]
+    // without loading col[k]; null otherwise.
+    const int32_t *c0 = nullptr;
     // XCD column slicing (nslices > 1): the columns are cut into nslices equal
     // ranges and the entries regrouped slice-major.  Workgroup b of the slice
     // kernel works on slice b % nslices — the XCD it is dispatched to — so each

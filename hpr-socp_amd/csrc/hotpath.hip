@@ -43,11 +43,26 @@ __device__ void tile_spmv_stream(const CsrT<T> &M, const T *__restrict__ v, cons
 // Lane-group SpMV: g lanes per row (1..64), or the whole workgroup per row
 // (256).  Each lane keeps four independent partial sums so that four gathers
 // are in flight per lane (long rows are otherwise bound by the latency of one
-// dependent load per step); the partials are combined in a fixed order.
+// dependent load per step); the partials are combined in a fixed order.  Rows
+// with consecutive columns (M.c0) read the input without column indices, in
+// the same order, so the sums are the same.
 template <typename T>
-__device__ __forceinline__ T row_dot(const CsrT<T> &M, const T *__restrict__ v, int s, int e, int lane, int G) {
+__device__ __forceinline__ T row_dot(const CsrT<T> &M, const T *__restrict__ v, int row, int s, int e, int lane,
+                                     int G) {
     T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
     int k = s + lane;
+    if (M.c0) {
+        const T *__restrict__ vr = v + (M.c0[row] - s);
+        for (; k + 3 * G < e; k += 4 * G) {
+            const T w0 = M.val[k], w1 = M.val[k + G], w2 = M.val[k + 2 * G], w3 = M.val[k + 3 * G];
+            a0 += w0 * vr[k];
+            a1 += w1 * vr[k + G];
+            a2 += w2 * vr[k + 2 * G];
+            a3 += w3 * vr[k + 3 * G];
+        }
+        for (; k < e; k += G) a0 += M.val[k] * vr[k];
+        return (a0 + a1) + (a2 + a3);
+    }
     for (; k + 3 * G < e; k += 4 * G) {
         const int c0 = M.col[k], c1 = M.col[k + G], c2 = M.col[k + 2 * G], c3 = M.col[k + 3 * G];
         const T w0 = M.val[k], w1 = M.val[k + G], w2 = M.val[k + 2 * G], w3 = M.val[k + 3 * G];
@@ -69,14 +84,14 @@ __device__ void tile_spmv(const CsrT<T> &M, const T *__restrict__ v, const Tile 
         const int lane = tid & (G - 1), grp = tid / G, ngrp = kBlock / G;
         for (int r = grp; r < nrows; r += ngrp) {
             const int row = t.rb + r;
-            T sum = row_dot(M, v, M.rowptr[row], M.rowptr[row + 1], lane, G);
+            T sum = row_dot(M, v, row, M.rowptr[row], M.rowptr[row + 1], lane, G);
             for (int o = G >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, G);
             if (lane == 0) acc[r] = sum;
         }
     } else {
         for (int r = 0; r < nrows; ++r) {
             const int row = t.rb + r;
-            T sum = row_dot(M, v, M.rowptr[row], M.rowptr[row + 1], tid, kBlock);
+            T sum = row_dot(M, v, row, M.rowptr[row], M.rowptr[row + 1], tid, kBlock);
             // workgroup sum in T (wave butterflies, fixed order)
             for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
             __syncthreads();

@@ -93,6 +93,10 @@ struct HostCsr {
     std::vector<double> v;
 };
 
+// First column of every row of `a` on the device when each row's columns are
+// consecutive (CsrT::c0), null otherwise.
+int32_t *contig_c0(hprsocp_handle *h, const HostCsr &a);
+
 HostCsr transpose(const HostCsr &a) {
     HostCsr t;
     t.nrows = a.ncols;
@@ -941,6 +945,19 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
     }
 }
 
+int32_t *contig_c0(hprsocp_handle *h, const HostCsr &a) {
+    if (a.ci.empty()) return nullptr;
+    std::vector<int32_t> c0((size_t)a.nrows, 0);
+    for (int32_t r = 0; r < a.nrows; ++r) {
+        const int32_t s = a.rp[r], e = a.rp[r + 1];
+        if (s == e) continue;
+        c0[r] = a.ci[s];
+        for (int32_t k = s + 1; k < e; ++k)
+            if (a.ci[k] != a.ci[s] + (k - s)) return nullptr;
+    }
+    return h->upload<int32_t>(c0.data(), a.nrows);
+}
+
 // ------------------------------------------------------------- validation
 int validate_csr(std::string &err, const char *name, int32_t nr, int32_t nc, int64_t nnz, const int32_t *rp,
                  const int32_t *ci, const double *v) {
@@ -1051,6 +1068,7 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz) {
     f.rowptr = M.rowptr;
     f.col = M.col;
     f.nrows = M.nrows;
+    f.c0 = M.c0;
     float *v = h->dalloc<float>(nnz);
     launch_demote(h->st, nnz, M.val, v);
     f.val = v;
@@ -1322,6 +1340,8 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     int32_t *AT_ci = h->upload<int32_t>(AT.ci.data(), (int64_t)AT.ci.size());
     h->AT_v = h->upload<double>(AT.v.data(), (int64_t)AT.v.size());
     h->AT = CsrView{AT_rp, AT_ci, h->AT_v, n};
+    h->A.c0 = contig_c0(h, A);
+    h->AT.c0 = contig_c0(h, AT);
     h->h_A_rp = A.rp;
     h->con_idx = h->upload<int32_t>(h->h_con_idx.data(), h->ncon + 1);
     h->var_idx = h->upload<int32_t>(h->h_var_idx.data(), h->nvar + 1);
@@ -1331,6 +1351,7 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
         h->Q_v = h->upload<double>(Qh.v.data(), h->nnzQ);
         h->Q_v2 = h->dalloc<double>(h->nnzQ);
         h->Q = CsrView{Q_rp, Q_ci, h->Q_v, n};
+        h->Q.c0 = contig_c0(h, Qh);
         // transpose partner of every entry (Qh is symmetric in pattern, rows sorted)
         HostCsr idx = Qh;
         for (int64_t k = 0; k < h->nnzQ; ++k) idx.v[k] = (double)k;
@@ -1362,10 +1383,12 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
         int32_t *F_ci = h->upload<int32_t>(F.ci.data(), (int64_t)F.ci.size());
         double *F_v = h->upload<double>(F.v.data(), (int64_t)F.v.size());
         h->F = CsrView{F_rp, F_ci, F_v, F.nrows};
+        h->F.c0 = contig_c0(h, F);
         int32_t *FT_rp = h->upload<int32_t>(FT.rp.data(), n + 1);
         int32_t *FT_ci = h->upload<int32_t>(FT.ci.data(), (int64_t)FT.ci.size());
         double *FT_v = h->upload<double>(FT.v.data(), (int64_t)FT.v.size());
         h->FT = CsrView{FT_rp, FT_ci, FT_v, n};
+        h->FT.c0 = contig_c0(h, FT);
         h->Fdiag = p->Q_diag ? h->upload<double>(p->Q_diag, n) : nullptr;
         h->Ftmp = h->dalloc<double>(F.nrows);
         h->plan_F = make_plan(h, F.rp, F.nrows, F.nrows, {}, 0);

@@ -149,3 +149,38 @@ def test_batch_rejects_mixed_q(gpu_lib):
     c = synthetic.make_small_qp(seed=3, q="factored")
     with pytest.raises(hprsocp.HPRSOCPError):
         hprsocp.BatchSolver([c], params())
+
+
+def test_portfolio_contiguous_rows(oracle_lib, gpu_lib):
+    """Portfolio instances (the C3 family, small): every row of A is one run of
+    consecutive columns (the budget row and the dense F^T rows), which the
+    lane-group SpMV reads without column indices (CsrT::c0).  SpMV against
+    scipy, batch = single solves bit for bit, and objectives against the
+    oracle's solve."""
+    import scipy.sparse as sp
+    models = synthetic.make_c3(count=4, n_lo=200, n_hi=3000)
+    for m in models:
+        A = m.A.tocsr()
+        for r in range(A.shape[0]):
+            idx = A.indices[A.indptr[r]:A.indptr[r + 1]]
+            assert np.array_equal(idx, np.arange(idx[0], idx[0] + len(idx)) if len(idx) else idx)
+    m = models[0]
+    with hprsocp.Solver(m, params()) as g:
+        A = m.A.tocsr()
+        As = sp.csr_matrix((g.scaled_A(), A.indices, A.indptr), shape=A.shape)
+        rng = np.random.default_rng(3)
+        v, w = rng.standard_normal(A.shape[1]), rng.standard_normal(A.shape[0])
+        np.testing.assert_allclose(g.spmv(v), As @ v, rtol=1e-12, atol=1e-12 * np.abs(As @ v).max())
+        np.testing.assert_allclose(g.spmv(w, trans=True), As.T @ w, rtol=1e-12,
+                                   atol=1e-12 * np.abs(As.T @ w).max())
+    rb = batch_solve(models, stoptol=1e-6)
+    for i, (m, r) in enumerate(zip(models, rb)):
+        rs = single_solve(m, stoptol=1e-6)
+        with hprsocp.Solver(m, params(stoptol=1e-6), lib=oracle_lib, prefix="oracle_") as o:
+            o.estimate_lambda()
+            ro = o.solve()
+        print(f"[result] portfolio[{i}] n={m.A.shape[1]}: batch iter {r.iter} single {rs.iter} oracle {ro.iter} "
+              f"obj {r.primal_obj:.10e} / {ro.primal_obj:.10e}", flush=True)
+        assert r.status == rs.status == ro.status == "OPTIMAL"
+        assert r.iter == rs.iter and r.primal_obj == rs.primal_obj
+        assert abs(r.primal_obj - ro.primal_obj) <= 1e-6 * (1 + abs(ro.primal_obj))
