@@ -8,6 +8,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 namespace hpr {
 
 // device helpers (nmax, jclamp, wave/block reductions) live in devutil.h
@@ -382,11 +384,23 @@ __global__ __launch_bounds__(kBlock) void k_row_stat(CsrView M, CsrView Q, int h
     const int lane = threadIdx.x & 63;
     const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
     for (int64_t i = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; i < M.nrows; i += nw) {
-        double a = 0.0, b = 0.0;
-        for (int k = M.rowptr[i] + lane; k < M.rowptr[i + 1]; k += 64) {
-            const double v = fabs(M.val[k]);
-            a = use_sum ? a + v : nmax(a, v);
+        // four independent accumulators per lane keep four loads in flight on long rows
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, b = 0.0;
+        const int e = M.rowptr[i + 1];
+        int k = M.rowptr[i] + lane;
+        for (; k + 192 < e; k += 256) {
+            const double v0 = fabs(M.val[k]), v1 = fabs(M.val[k + 64]), v2 = fabs(M.val[k + 128]),
+                         v3 = fabs(M.val[k + 192]);
+            a0 = use_sum ? a0 + v0 : nmax(a0, v0);
+            a1 = use_sum ? a1 + v1 : nmax(a1, v1);
+            a2 = use_sum ? a2 + v2 : nmax(a2, v2);
+            a3 = use_sum ? a3 + v3 : nmax(a3, v3);
         }
+        for (; k < e; k += 64) {
+            const double v = fabs(M.val[k]);
+            a0 = use_sum ? a0 + v : nmax(a0, v);
+        }
+        double a = use_sum ? (a0 + a1) + (a2 + a3) : nmax(nmax(a0, a1), nmax(a2, a3));
         if (has_q)
             for (int k = Q.rowptr[i] + lane; k < Q.rowptr[i + 1]; k += 64) {
                 const double v = fabs(Q.val[k]);
@@ -414,26 +428,51 @@ void launch_row_stat(hipStream_t st, const CsrView &M, const CsrView *Q, int use
                        out);
 }
 
-__global__ void k_block_scale(int32_t ncones, const int32_t *idx, const int32_t *cnt, double *t, int32_t strategy,
-                              int32_t phase, int32_t loc) {
-    GRID_STRIDE(c, ncones) {
+// One workgroup per cone: block max and sum of squares of the cone's per-row
+// norms (lane-strided, then wave butterflies, then the waves in fixed order),
+// the block rule, and the shared factor written back to every row.  Cones of
+// 10^5 rows and more (one per robust-regression instance) would otherwise be
+// reduced by a single thread.
+__global__ __launch_bounds__(kBlock) void k_block_scale(int32_t ncones, const int32_t *idx, const int32_t *cnt,
+                                                      double *t, int32_t strategy, int32_t phase, int32_t loc) {
+    __shared__ double sh_m[kBlock / 64], sh_s[kBlock / 64];
+    __shared__ double sv;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int64_t c = blockIdx.x; c < ncones; c += gridDim.x) {
         const int s = idx[c], e = idx[c + 1];
-        if (s >= e) continue;
+        if (s >= e) continue;  // uniform over the workgroup
         double bm = t[s], ss = 0.0;
-        for (int j = s; j < e; ++j) {
+        for (int j = s + (int)threadIdx.x; j < e; j += kBlock) {
             bm = nmax(bm, t[j]);
             ss += t[j] * t[j];
         }
-        const double rms = sqrt(ss / (double)(e - s));
-        const double v = soc_block_value(e - s, bm, rms, strategy, phase, loc, cnt ? cnt[c] : ncones);
-        for (int j = s; j < e; ++j) t[j] = v;
+        bm = wave_max(bm);
+        ss = wave_sum(ss);
+        if (lane == 0) {
+            sh_m[wid] = bm;
+            sh_s[wid] = ss;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = sh_m[0], q = sh_s[0];
+            for (int w = 1; w < kBlock / 64; ++w) {
+                m = nmax(m, sh_m[w]);
+                q += sh_s[w];
+            }
+            const double rms = sqrt(q / (double)(e - s));
+            sv = soc_block_value(e - s, m, rms, strategy, phase, loc, cnt ? cnt[c] : ncones);
+        }
+        __syncthreads();
+        const double v = sv;
+        for (int j = s + (int)threadIdx.x; j < e; j += kBlock) t[j] = v;
+        __syncthreads();  // sh_m, sh_s and sv are reused by the next cone
     }
 }
 void launch_block_scale(hipStream_t st, int32_t ncones, const int32_t *idx, const int32_t *cone_cnt, double *t,
                         int32_t strategy, int32_t phase, int32_t loc) {
     if (ncones > 0)
-        hipLaunchKernelGGL(k_block_scale, dim3(ew_blocks(ncones)), dim3(kBlock), 0, st, ncones, idx, cone_cnt, t,
-                           strategy, phase, loc);
+        hipLaunchKernelGGL(k_block_scale, dim3((unsigned)std::min<int64_t>(ncones, 1 << 18)), dim3(kBlock), 0, st,
+                           ncones, idx, cone_cnt, t, strategy, phase, loc);
 }
 __global__ void k_clip(int64_t n, double *v) {
     GRID_STRIDE(i, n) { const double x = v[i]; v[i] = x > 1e4 ? 1e4 : (x < 1e-4 ? 1e-4 : x); }
@@ -448,10 +487,20 @@ __global__ __launch_bounds__(kBlock) void k_scale_csr(int32_t nrows, const int32
     const int lane = threadIdx.x & 63;
     const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
     for (int64_t i = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; i < nrows; i += nw) {
-        for (int k = rp[i] + lane; k < rp[i + 1]; k += 64) {
-            const int j = col[k];
-            val[k] = trans ? ((1.0 / r[j]) * val[k]) * (1.0 / c[i]) : ((1.0 / r[i]) * val[k]) * (1.0 / c[j]);
+        auto f = [&](int k, int j) {
+            return trans ? ((1.0 / r[j]) * val[k]) * (1.0 / c[i]) : ((1.0 / r[i]) * val[k]) * (1.0 / c[j]);
+        };
+        const int e = rp[i + 1];
+        int k = rp[i] + lane;
+        for (; k + 192 < e; k += 256) {  // four entries in flight per lane
+            const int j0 = col[k], j1 = col[k + 64], j2 = col[k + 128], j3 = col[k + 192];
+            const double w0 = f(k, j0), w1 = f(k + 64, j1), w2 = f(k + 128, j2), w3 = f(k + 192, j3);
+            val[k] = w0;
+            val[k + 64] = w1;
+            val[k + 128] = w2;
+            val[k + 192] = w3;
         }
+        for (; k < e; k += 64) val[k] = f(k, col[k]);
     }
 }
 void launch_scale_csr(hipStream_t st, int32_t nrows, const int32_t *rowptr, const int32_t *col,

@@ -97,22 +97,43 @@ struct HostCsr {
 // consecutive (CsrT::c0), null otherwise.
 int32_t *contig_c0(hprsocp_handle *h, const HostCsr &a);
 
+// Counting-sort transpose.  Large matrices fill in parallel: each thread owns
+// a range of output rows (input columns) holding about the same number of
+// entries and scans the input rows in order, so every output row comes out
+// sorted by column, exactly as in the sequential fill.
 HostCsr transpose(const HostCsr &a) {
     HostCsr t;
     t.nrows = a.ncols;
     t.ncols = a.nrows;
     t.rp.assign((size_t)a.ncols + 1, 0);
-    t.ci.resize(a.ci.size());
-    t.v.resize(a.v.size());
+    const int64_t nnz = (int64_t)a.ci.size();
+    t.ci.resize((size_t)nnz);
+    t.v.resize((size_t)nnz);
     for (int32_t c : a.ci) t.rp[c + 1]++;
     for (int32_t j = 0; j < a.ncols; ++j) t.rp[j + 1] += t.rp[j];
-    std::vector<int32_t> next(t.rp.begin(), t.rp.end() - 1);
-    for (int32_t i = 0; i < a.nrows; ++i)
-        for (int32_t k = a.rp[i]; k < a.rp[i + 1]; ++k) {
-            const int32_t d = next[a.ci[k]]++;
-            t.ci[d] = i;
-            t.v[d] = a.v[k];
-        }
+    const int nth =
+        nnz < (1 << 16) ? 1 : (int)std::max<unsigned>(1, std::min<unsigned>(16, std::thread::hardware_concurrency()));
+    std::vector<int32_t> cut((size_t)nth + 1, a.ncols);
+    cut[0] = 0;
+    for (int w = 1; w < nth; ++w)  // first column whose entries start at or after w/nth of nnz
+        cut[w] = (int32_t)(std::lower_bound(t.rp.begin(), t.rp.end() - 1, (int32_t)(nnz * w / nth)) - t.rp.begin());
+    auto fill = [&](int w) {
+        const int32_t lo = cut[w], hi = std::max(cut[w], cut[w + 1]);
+        if (lo >= hi) return;
+        std::vector<int32_t> next(t.rp.begin() + lo, t.rp.begin() + hi);
+        for (int32_t i = 0; i < a.nrows; ++i)
+            for (int32_t k = a.rp[i]; k < a.rp[i + 1]; ++k) {
+                const int32_t c = a.ci[k];
+                if (c < lo || c >= hi) continue;
+                const int32_t d = next[c - lo]++;
+                t.ci[d] = i;
+                t.v[d] = a.v[k];
+            }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nth; ++w) pool.emplace_back(fill, w);
+    fill(0);
+    for (auto &th : pool) th.join();
     return t;
 }
 
