@@ -52,15 +52,15 @@ __device__ __forceinline__ T row_dot(const CsrT<T> &M, const T *__restrict__ v, 
     T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
     int k = s + lane;
     if (M.c0) {
-        const T *__restrict__ vr = v + (M.c0[row] - s);
+        const int off = M.c0[row] - s;  // column of entry k is off + k
         for (; k + 3 * G < e; k += 4 * G) {
             const T w0 = M.val[k], w1 = M.val[k + G], w2 = M.val[k + 2 * G], w3 = M.val[k + 3 * G];
-            a0 += w0 * vr[k];
-            a1 += w1 * vr[k + G];
-            a2 += w2 * vr[k + 2 * G];
-            a3 += w3 * vr[k + 3 * G];
+            a0 += w0 * v[off + k];
+            a1 += w1 * v[off + k + G];
+            a2 += w2 * v[off + k + 2 * G];
+            a3 += w3 * v[off + k + 3 * G];
         }
-        for (; k < e; k += G) a0 += M.val[k] * vr[k];
+        for (; k < e; k += G) a0 += M.val[k] * v[off + k];
         return (a0 + a1) + (a2 + a3);
     }
     for (; k + 3 * G < e; k += 4 * G) {
