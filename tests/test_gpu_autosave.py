@@ -74,12 +74,12 @@ def test_autosave_reference_checks(tmp_path, gpu_lib):
     assert "parameters/initial_x" not in d
 
 
-@pytest.mark.parametrize("name", ["qp_csr", "c0"])
-def test_autosave_last_checkpoint_is_the_result(name, tmp_path, gpu_lib):
+@pytest.mark.parametrize("name,precision", [("qp_csr", "fp64"), ("c0", "fp64"), ("qp_csr", "fp32_refine")])
+def test_autosave_last_checkpoint_is_the_result(name, precision, tmp_path, gpu_lib):
     m = model(name)
     path = tmp_path / "state.npz"
-    print(f"[case] autosave final state {name}", flush=True)
-    r = run(m, path, stoptol=1e-6)
+    print(f"[case] autosave final state {name} {precision}", flush=True)
+    r = run(m, path, stoptol=1e-6, precision=precision)
     d = hprsocp.load_autosave(str(path))
     assert d["current/iteration"] == r.iter
     np.testing.assert_array_equal(d["current/x_org"], r.x)
