@@ -62,11 +62,14 @@ HostPlan build_plan(const int32_t *rowptr, int32_t nrows, int32_t lin_end, const
             while (a < e) {
                 int32_t b = a + 1;
                 int64_t nz = nnz_of(a, b);
-                while (b < e && b - a < kTileRows && nz + nnz_of(b, b + 1) <= target) {
+                const int64_t htarget = rowptr ? std::max<int64_t>(target, cfg.huge_target) : target;
+                while (b < e && b - a < kTileRows && nz + nnz_of(b, b + 1) <= htarget) {
                     nz += nnz_of(b, b + 1);
                     ++b;
                 }
-                Tile t{a, b, TILE_HUGE, choose_g(nz, b - a, cfg), h, P.nslots + k, 1, inst_of(s)};
+                int32_t g = choose_g(nz, b - a, cfg);
+                if (g == kBlock && nz < (int64_t)cfg.huge_whole * (b - a)) g = 64;
+                Tile t{a, b, TILE_HUGE, g, h, P.nslots + k, 1, inst_of(s)};
                 huge.push_back(t);
                 ++k;
                 a = b;

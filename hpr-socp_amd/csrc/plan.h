@@ -26,6 +26,12 @@ struct HostPlan {
 struct PlanConfig {
     int32_t nnz_target = 2048;     // nonzeros per tile (SpMV work balance)
     int32_t huge_nnz_factor = 16;  // cones above factor*nnz_target nonzeros are split
+    // HUGE segments (rows of a split cone): up to huge_target nonzeros per tile;
+    // segments whose rows average between 1024 and huge_whole nonzeros take one
+    // wavefront per row (several rows per tile) instead of the whole workgroup
+    // per row, which leaves a one-row tile latency-bound
+    int32_t huge_target = 16384;
+    int32_t huge_whole = 8192;
     int32_t g_divisor = 2;         // g = pow2ceil(mean_row_nnz / g_divisor), clamped to 1..64
     bool stream = true;            // CSR-stream tiles when the tile fits kStreamNnz
 };
