@@ -855,11 +855,7 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
     return true;
 }
 
-// Autotuner of the XCD slicing: builds the sliced form (8 slices) of an
-// operator whose input vector outgrows ~2 MB, times one SpMV of each form with
-// HIP events (median of 5 after a warm-up) and keeps the faster one together
-// with its epilogue plan (nnz-balanced tiles for the plain form, row tiles for
-// the sliced form).  HPRSOCP_XCD_SLICES=<P> forces P slices (tests).
+// Releases the device arrays of a sliced form built by build_slices.
 void free_slices(hprsocp_handle *h, const CsrView &S) {
     h->dfree(S.s_base);
     h->dfree(S.s_off);
