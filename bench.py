@@ -150,14 +150,16 @@ def device_info():
         hip = C.CDLL("libamdhip64.so")
     except OSError:
         return info
+    dev = C.c_int(0)
     name = C.create_string_buffer(256)
-    if hip.hipDeviceGetName(name, 256, 0) == 0:
+    if hip.hipDeviceGet(C.byref(dev), 0) == 0 and hip.hipDeviceGetName(name, 256, dev) == 0 and name.value:
         info["name"] = name.value.decode(errors="replace")
     total = C.c_size_t()
     if hip.hipDeviceTotalMem(C.byref(total), 0) == 0:
         info["mem_GiB"] = round(total.value / 2**30, 1)
     v = C.c_int()
-    # hipDeviceAttribute_t: MultiprocessorCount = 63, MemoryClockRate = 60 (kHz), MemoryBusWidth = 59 (bits)
+    # hipDeviceAttribute_t: MultiprocessorCount = 63, MemoryClockRate = 60 (kHz), MemoryBusWidth = 59 (bits).
+    # HBM3E moves 4 bits per pin per reported clock: 2.0 GHz x 4 x 8192 bits = 8.19 TB/s.
     for key, attr in (("cus", 63), ("mem_clock_kHz", 60), ("mem_bus_bits", 59)):
         if hip.hipDeviceGetAttribute(C.byref(v), attr, 0) == 0:
             info[key] = v.value
