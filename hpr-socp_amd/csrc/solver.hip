@@ -2651,11 +2651,13 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
                 t4[i] = now_s() - t_start;
                 it4[i] = iter;
                 first4[i] = 0;
+                if (p.verbose && single) printf("KKT < 1e-4 at iter = %lld\n", (long long)iter);
             }
             if (rs[i].kkt < 1e-6 && first6[i]) {
                 t6[i] = now_s() - t_start;
                 it6[i] = iter;
                 first6[i] = 0;
+                if (p.verbose && single) printf("KKT < 1e-6 at iter = %lld\n", (long long)iter);
             }
             if (status[i] == HPRSOCP_STATUS_CONTINUE) continue;
             const double tt = now_s() - t_start;
