@@ -157,6 +157,33 @@ struct TempvArgsT {  // tempv = x_hat + sigma*(Qw - Q w_bar) for non-diagonal Q
     const T *wbar;           // needed with dq
     const InstScal *isc;
 };
+// Delta form (precision FP32_DELTA): the state is the FP64 Halpern anchor of
+// the restart cycle plus an FP32 delta, x = last_x + dx, y = last_y + dy,
+// w = last_w + dw, aty = last_aty + daty.  The SpMV inputs are deltas too
+// (x_hat - last_x, tempv - last_x, y_bar - last_y, w_bar - last_w), so every
+// SpMV runs in FP32 on vectors that shrink as the cycle converges; the images
+// of the anchors are folded into two FP64 vectors computed once per cycle:
+//   ra  = last_x + sigma (last_aty - Q last_w - c)        (QP)
+//       = last_x + sigma (A^T last_y - c)                 (noQ)
+//   rya = A last_x - rhs (SOC rows) - lambda_A sigma last_y
+// so that the raw values the projections see are ra + (FP32 terms) and
+// rya + (FP32 terms), formed in FP64.
+struct DPrimalArgs {
+    float *dx, *dxh, *dwb, *dtv;   // dx in/out; x_hat - last_x; w_bar - last_w; tempv - last_x (sdiag)
+    const float *dy;               // noQ: SpMV input (A^T dy)
+    const float *dw, *daty, *dqw;  // QP: w, aty deltas; Q dw (unless sdiag)
+    const double *ra, *xa, *wa, *l, *u, *dq, *fact1, *fact2;
+    const InstScal *isc;
+    int64_t kstep;
+    int32_t sdiag, vec_fact;
+};
+struct DDualArgs {
+    float *dy, *dyb;               // dy in/out; y_bar - last_y
+    const double *rya, *ya, *AL, *AU;
+    const InstScal *isc;
+    int64_t kstep;
+};
+
 using NoQPrimalArgs = NoQPrimalArgsT<double>;
 using DualArgs = DualArgsT<double>;
 using QPPrimalArgs = QPPrimalArgsT<double>;
@@ -182,6 +209,12 @@ void launch_spmv(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, const T 
 template <typename T>
 void launch_spmv_tempv(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, const T *wbar,
                        const TempvArgsT<T> &a);
+// Delta form: primal (QP: no SpMV; noQ: A^T dy) and dual (A v, v = the FP32
+// delta of x_hat or tempv); raw values and projections in FP64 (raw_buf FP64).
+void launch_dprimal(hipStream_t st, const DevPlan &plan, const CsrT<float> &AT, const int32_t *var_idx,
+                    const DPrimalArgs &a, bool qp, double *raw_buf, double *partials);
+void launch_ddual(hipStream_t st, const DevPlan &plan, const CsrT<float> &A, const int32_t *con_idx,
+                  const float *v, const DDualArgs &a, double *raw_buf, double *partials);
 
 // Gather-ceiling probe: synthetic entries shaped like the sliced SpMV's
 // (8-byte value + packed column/position word, one gather each from the slice

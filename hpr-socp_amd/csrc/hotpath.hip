@@ -18,8 +18,9 @@
 namespace hpr {
 
 // ---------------------------------------------------------------- SpMV phase
-template <typename T>
-__device__ void tile_spmv_stream(const CsrT<T> &M, const T *__restrict__ v, const Tile &t, T *acc, T *prod,
+// acc: the tile's row sums, stored as A (T, or double for the delta form's FP32 SpMV)
+template <typename T, typename A>
+__device__ void tile_spmv_stream(const CsrT<T> &M, const T *__restrict__ v, const Tile &t, A *acc, T *prod,
                                  int32_t *rp) {
     const int tid = threadIdx.x;
     const int nrows = t.re - t.rb;
@@ -75,8 +76,8 @@ __device__ __forceinline__ T row_dot(const CsrT<T> &M, const T *__restrict__ v, 
     return (a0 + a1) + (a2 + a3);
 }
 
-template <typename T>
-__device__ void tile_spmv(const CsrT<T> &M, const T *__restrict__ v, const Tile &t, T *acc, double *sh) {
+template <typename T, typename A>
+__device__ void tile_spmv(const CsrT<T> &M, const T *__restrict__ v, const Tile &t, A *acc, double *sh) {
     const int tid = threadIdx.x;
     const int nrows = t.re - t.rb;
     const int G = t.g;
@@ -97,7 +98,7 @@ __device__ void tile_spmv(const CsrT<T> &M, const T *__restrict__ v, const Tile 
             __syncthreads();
             if ((tid & 63) == 0) sh[tid >> 6] = (double)sum;
             __syncthreads();
-            if (tid == 0) acc[r] = (T)sh[0] + (T)sh[1] + (T)sh[2] + (T)sh[3];
+            if (tid == 0) acc[r] = (T)sh[0] + (T)sh[1] + (T)sh[2] + (T)sh[3];  // summed in T
         }
     }
 }
@@ -135,6 +136,7 @@ __device__ __forceinline__ void halpern_weights(const InstScal &s, int64_t k, do
 
 template <typename T>
 struct OpNoQPrimal {
+    using V = T;  // raw / projected values
     NoQPrimalArgsT<T> a;
     T sigma, h1, h2;
     __device__ bool bind(int inst) {
@@ -163,6 +165,7 @@ struct OpNoQPrimal {
 
 template <typename T>
 struct OpDual {
+    using V = T;  // raw / projected values
     DualArgsT<T> a;
     T f1, f2, h1, h2;
     __device__ bool bind(int inst) {
@@ -194,6 +197,7 @@ struct OpDual {
 
 template <typename T>
 struct OpQPPrimal {
+    using V = T;  // raw / projected values
     QPPrimalArgsT<T> a;
     T sigma, h1, h2, f1s, f2s;
     __device__ bool bind(int inst) {
@@ -229,6 +233,7 @@ struct OpQPPrimal {
 
 template <typename T>
 struct OpW2 {
+    using V = T;  // raw / projected values
     W2ArgsT<T> a;
     T fs, h1, h2;
     __device__ bool bind(int inst) {
@@ -246,8 +251,10 @@ struct OpW2 {
         const T wi = a.w[i], atyi = a.aty[i];
         const T f = a.diag ? a.fact[i] : fs;
         const T wbn = a.w_bar[i] + f * (rb - atyi);
-        a.w[i] = h1 * a.last_w[i] + h2 * (T(2) * wbn - wi);
-        a.aty[i] = h1 * a.last_aty[i] + h2 * (T(2) * rb - atyi);
+        // null anchors read as zero (the delta form iterates on differences from them)
+        const T lw = a.last_w ? a.last_w[i] : T(0), la = a.last_aty ? a.last_aty[i] : T(0);
+        a.w[i] = h1 * lw + h2 * (T(2) * wbn - wi);
+        a.aty[i] = h1 * la + h2 * (T(2) * rb - atyi);
         if (a.full) {
             a.w_bar[i] = wbn;
             a.dw[i] = wbn - wi;
@@ -260,6 +267,7 @@ struct OpW2 {
 // plain SpMV output (all instances, no scalars)
 template <typename T>
 struct OpStore {
+    using V = T;
     T *out;
     const T *diag, *v2;
     __device__ bool bind(int) { return true; }
@@ -270,6 +278,7 @@ struct OpStore {
 
 template <typename T>
 struct OpTempv {
+    using V = T;  // raw / projected values
     TempvArgsT<T> a;
     T sigma;
     __device__ bool bind(int inst) {
@@ -406,11 +415,14 @@ __global__ __launch_bounds__(kSortBlock) void k_spmv_sorted(CsrT<T> M, const T *
 // SRC: 0 = no SpMV, 1 = CSR SpMV inside the tile, 2 = sum of the XCD-slice partials
 enum : int { SRC_NONE = 0, SRC_CSR = 1, SRC_SLICES = 2 };
 
+// T: SpMV arithmetic; Op::V: raw and projected values (T, or double for the delta form).
 template <typename T, class Op, int SRC, bool CONES>
 __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles, CsrT<T> M, const T *__restrict__ vin,
                                                  const int32_t *__restrict__ cone_idx, HugeInfo huge, Op op_in,
-                                                 T *__restrict__ raw_buf, double *__restrict__ partials) {
-    __shared__ T acc[kTileRows];
+                                                 typename Op::V *__restrict__ raw_buf,
+                                                 double *__restrict__ partials) {
+    using V = typename Op::V;
+    __shared__ V acc[kTileRows];
     __shared__ double sh[4];
     __shared__ T prod[SRC == SRC_CSR ? kStreamNnz : 1];
     __shared__ int32_t rps[SRC == SRC_CSR ? kTileRows + 1 : 1];
@@ -436,7 +448,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
     if (!CONES || t.kind == TILE_LINEAR) {
         if (tid < nrows) {
             const int i = t.rb + tid;
-            const T r = op.raw(i, acc[tid], false);
+            const V r = op.raw(i, acc[tid], false);
             op.finish(i, r, op.lin_proj(i, r));
         }
         return;
@@ -448,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
             const int s = cone_idx[c], e = cone_idx[c + 1];
             double sq = 0.0, head = 0.0;
             for (int i = s + lane; i < e; i += G2) {
-                const T r = op.raw(i, acc[i - t.rb], true);
+                const V r = op.raw(i, acc[i - t.rb], true);
                 acc[i - t.rb] = r;
                 if (i == s) head = (double)r;
                 else sq += (double)r * (double)r;
@@ -457,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
             head = __shfl(head, 0, G2);
             const ConeCase cc = cone_case(head, sqrt(sq));
             for (int i = s + lane; i < e; i += G2) {
-                const T r = acc[i - t.rb];
+                const V r = acc[i - t.rb];
                 op.finish(i, r, cone_apply(cc, i == s, r));
             }
         }
@@ -467,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
     double sq = 0.0;
     if (tid < nrows) {
         const int i = t.rb + tid;
-        const T r = op.raw(i, acc[tid], true);
+        const V r = op.raw(i, acc[tid], true);
         raw_buf[i] = r;
         if (i != huge.start[t.c0]) sq = (double)r * (double)r;
     }
@@ -476,10 +488,11 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
 }
 
 // HUGE segment, phase 2: the cone's |xi| from all partials, then the projection.
-template <typename T, class Op>
+template <class Op>
 __global__ __launch_bounds__(kBlock) void k_huge_finish(const Tile *__restrict__ tiles, HugeInfo huge, Op op_in,
-                                                        const T *__restrict__ raw_buf,
+                                                        const typename Op::V *__restrict__ raw_buf,
                                                         const double *__restrict__ partials) {
+    using V = typename Op::V;
     __shared__ double sh[4];
     const Tile t = tiles[blockIdx.x];
     Op op = op_in;
@@ -494,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_huge_finish(const Tile *__restrict__
     const int nrows = t.re - t.rb;
     if ((int)threadIdx.x < nrows) {
         const int i = t.rb + threadIdx.x;
-        const T r = raw_buf[i];
+        const V r = raw_buf[i];
         op.finish(i, r, cone_apply(cc, i == s, r));
     }
 }
@@ -503,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_huge_finish(const Tile *__restrict__
 // k_spmv_slices and the tile kernel sums the slice partials.
 template <typename T, class Op, int SRC, bool CONES>
 static void run_tiles(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, const T *v, const int32_t *cone_idx,
-                      const Op &op, T *raw_buf, double *partials) {
+                      const Op &op, typename Op::V *raw_buf, double *partials) {
     if (plan.ntiles == 0) return;
     if (SRC == SRC_CSR && M.nslices > 1) {
         if (M.s_ntiles > 0 && M.s_sorted)
@@ -519,7 +532,7 @@ static void run_tiles(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, con
     if (CONES) {
         const int nh = plan.ntiles - plan.first_huge;
         if (nh > 0)
-            hipLaunchKernelGGL((k_huge_finish<T, Op>), dim3(nh), dim3(kBlock), 0, st, plan.tiles + plan.first_huge,
+            hipLaunchKernelGGL((k_huge_finish<Op>), dim3(nh), dim3(kBlock), 0, st, plan.tiles + plan.first_huge,
                                plan.huge, op, raw_buf, partials);
     }
 }
@@ -554,6 +567,92 @@ template <typename T>
 void launch_spmv_tempv(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, const T *wbar,
                        const TempvArgsT<T> &a) {
     run_tiles<T, OpTempv<T>, SRC_CSR, false>(st, plan, M, wbar, nullptr, OpTempv<T>{a}, nullptr, nullptr);
+}
+
+// ------------------------------------------------------------ delta form
+// State = FP64 anchor of the restart cycle + FP32 delta (DPrimalArgs).  The
+// raw values are ra + (FP32 terms) and rya + (FP32 terms), formed and
+// projected in FP64; the reflections and Halpern steps are written as FP32
+// differences from the anchors, in which the anchor terms cancel:
+//   dx'  = h2 (x_hat - last_x),  dy' = h2 (2 (y_bar - last_y) - dy).
+template <bool QP>
+struct OpDPrimal {
+    using V = double;
+    DPrimalArgs a;
+    double sigma, h2, f1s, f2s;
+    __device__ bool bind(int inst) {
+        const InstScal s = a.isc[inst];
+        double d1, d2;
+        halpern_weights(s, a.kstep, d1, d2);
+        sigma = s.sigma;
+        h2 = d2;
+        f2s = 1.0 / (1.0 + s.sigma * s.lambda_Q);
+        f1s = 1.0 - f2s;
+        return inst_runs<float>(s);
+    }
+    __device__ double raw(int i, double acc, bool) const {
+        if constexpr (QP) {
+            const double qd = a.sdiag ? a.dq[i] * (double)a.dw[i] : (double)a.dqw[i];
+            return a.ra[i] + ((double)a.dx[i] + sigma * ((double)a.daty[i] - qd));
+        } else {
+            return a.ra[i] + ((double)a.dx[i] + sigma * acc);
+        }
+    }
+    __device__ double lin_proj(int i, double r) const { return jclamp(r, a.l[i], a.u[i]); }
+    __device__ void finish(int i, double, double p) const {
+        const double xa = a.xa[i], dxi = (double)a.dx[i];
+        const double dxh = 2.0 * (p - xa) - dxi;  // x_hat - last_x, x_hat = 2p - x
+        a.dx[i] = (float)(h2 * dxh);
+        a.dxh[i] = (float)dxh;
+        if constexpr (QP) {
+            const double wa = a.wa[i], dwi = (double)a.dw[i];
+            const double xh = 2.0 * p - (xa + dxi), w = wa + dwi;
+            const double wb = a.vec_fact ? a.fact1[i] * w + a.fact2[i] * xh : f1s * w + f2s * xh;
+            const double dwb = wb - wa;
+            a.dwb[i] = (float)dwb;
+            // tempv = x_hat + sigma dq (w - w_bar), as a difference from last_x
+            if (a.sdiag) a.dtv[i] = (float)(dxh + sigma * (a.dq[i] * (dwi - dwb)));
+        }
+    }
+};
+
+struct OpDDual {
+    using V = double;
+    DDualArgs a;
+    double f1, f2, h2;
+    __device__ bool bind(int inst) {
+        const InstScal s = a.isc[inst];
+        double d1, d2;
+        halpern_weights(s, a.kstep, d1, d2);
+        f1 = s.lambda_A * s.sigma;
+        f2 = 1.0 / f1;
+        h2 = d2;
+        return inst_runs<float>(s);
+    }
+    // rya holds A last_x - rhs (SOC rows) - f1 last_y
+    __device__ double raw(int i, double acc, bool) const { return a.rya[i] + (acc - f1 * (double)a.dy[i]); }
+    __device__ double lin_proj(int i, double r) const { return jclamp(r, a.AL[i], a.AU[i]); }
+    __device__ void finish(int i, double raw, double p) const {
+        const double dyb = f2 * (p - raw) - a.ya[i];  // y_bar - last_y
+        a.dy[i] = (float)(h2 * (2.0 * dyb - (double)a.dy[i]));
+        a.dyb[i] = (float)dyb;
+    }
+};
+
+void launch_dprimal(hipStream_t st, const DevPlan &plan, const CsrT<float> &AT, const int32_t *var_idx,
+                    const DPrimalArgs &a, bool qp, double *raw_buf, double *partials) {
+    if (qp) {
+        CsrT<float> none{};
+        run_tiles<float, OpDPrimal<true>, SRC_NONE, true>(st, plan, none, nullptr, var_idx, OpDPrimal<true>{a},
+                                                         raw_buf, partials);
+    } else {
+        run_tiles<float, OpDPrimal<false>, SRC_CSR, true>(st, plan, AT, a.dy, var_idx, OpDPrimal<false>{a},
+                                                          raw_buf, partials);
+    }
+}
+void launch_ddual(hipStream_t st, const DevPlan &plan, const CsrT<float> &A, const int32_t *con_idx,
+                  const float *v, const DDualArgs &a, double *raw_buf, double *partials) {
+    run_tiles<float, OpDDual, SRC_CSR, true>(st, plan, A, v, con_idx, OpDDual{a}, raw_buf, partials);
 }
 
 // ------------------------------------------------------ gather ceiling probe

@@ -279,6 +279,12 @@ struct hprsocp_handle {
     bool fp32_dirty = true;    // FP64 state changed since the last demotion
     int64_t nnzF = 0;
     IterView<float> v32{};
+    // precision FP32_DELTA (single problem): the FP32 mirrors of x, y, w, aty
+    // hold differences from last_x, last_y, last_w, last_aty; ra and rya hold
+    // the anchors' images (see DPrimalArgs)
+    bool delta = false;
+    bool delta_live = false;  // the deltas hold the current state (the FP64 x, y, w, aty are stale)
+    double *d_ra = nullptr, *d_rya = nullptr, *d_img_n = nullptr, *d_img_m = nullptr;
     // timing (profiled iterations)
     bool profile = false;
     std::vector<hipEvent_t> ev;
@@ -1156,6 +1162,57 @@ void promote_state(hprsocp_handle *h) {
 #undef HPR_PRO_M
 }
 
+// Delta form: the instance runs the FP32 kernels (f32 = 1) or the FP64 ones.
+void delta_prec(hprsocp_handle *h, int f32) {
+    if (h->I(0).fp32 == f32) return;
+    h->I(0).fp32 = f32;
+    upload_isc(h);
+}
+
+// Delta form: FP64 state -> deltas from the anchors, the anchors' images
+// (ra, rya) and the FP32 data of the float kernels.  Runs whenever the FP64
+// state changed (iterations, restarts, sigma, initial points).
+void delta_split(hprsocp_handle *h) {
+    hipStream_t st = h->st;
+    IterView<float> &V = h->v32;
+    const int64_t n = h->n, m = h->m;
+    launch_delta_split(st, n, h->x, h->last_x, V.x);
+    launch_delta_split(st, m, h->y, h->last_y, V.y);
+    const double sigma = h->I(0).sigma;
+    if (h->has_quad) {
+        launch_delta_split(st, n, h->w, h->last_w, V.w);
+        launch_delta_split(st, n, h->aty, h->last_aty, V.aty);
+        if (h->Q_is_diag) launch_demote(st, n, h->fact, V.fact);
+        if (V.Fdiag) launch_demote(st, n, h->Fdiag, V.Fdiag);
+        qmap(h, h->last_w, h->d_img_n);  // Q last_w
+        launch_delta_ra(st, n, h->last_x, h->last_aty, h->d_img_n, nullptr, nullptr, h->c, sigma, h->d_ra);
+    } else {
+        launch_spmv(st, h->plan_AT, h->AT, h->last_y, h->d_img_n);  // A^T last_y
+        launch_delta_ra(st, n, h->last_x, h->d_img_n, nullptr, nullptr, nullptr, h->c, sigma, h->d_ra);
+    }
+    if (m > 0) {
+        launch_spmv(st, h->plan_A, h->A, h->last_x, h->d_img_m);  // A last_x
+        launch_delta_rya(st, m, h->m1, h->d_img_m, h->rhs, h->last_y, h->I(0).lambda_A * sigma, h->d_rya);
+    }
+    h->fp32_dirty = false;
+}
+
+// Delta form: deltas -> FP64 state (x = last_x + dx, ...), before anything
+// reads or writes the FP64 state.
+void delta_join(hprsocp_handle *h) {
+    if (!h->delta || !h->delta_live) return;
+    hipStream_t st = h->st;
+    IterView<float> &V = h->v32;
+    const int64_t n = h->n, m = h->m;
+    launch_delta_join(st, n, h->last_x, V.x, h->x);
+    launch_delta_join(st, m, h->last_y, V.y, h->y);
+    if (h->has_quad) {
+        launch_delta_join(st, n, h->last_w, V.w, h->w);
+        launch_delta_join(st, n, h->last_aty, V.aty, h->aty);
+    }
+    h->delta_live = false;
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -1206,8 +1263,11 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     h->save_path = P.save_filename && P.save_filename[0] ? P.save_filename : kDefaultSaveName;
     P.save_filename = h->save_path.c_str();
     if (P.auto_save && ranges) throw Err{HPRSOCP_ERR_UNSUPPORTED, "auto_save applies to single-problem handles, not batches"};
-    if (P.precision != HPRSOCP_PREC_FP64 && P.precision != HPRSOCP_PREC_FP32_REFINE)
+    if (P.precision != HPRSOCP_PREC_FP64 && P.precision != HPRSOCP_PREC_FP32_REFINE &&
+        P.precision != HPRSOCP_PREC_FP32_DELTA)
         throw Err{HPRSOCP_ERR_INVALID, "unknown precision mode"};
+    if (P.precision == HPRSOCP_PREC_FP32_DELTA && ranges)
+        throw Err{HPRSOCP_ERR_UNSUPPORTED, "precision FP32_DELTA applies to single-problem handles, not batches"};
     if (P.precision == HPRSOCP_PREC_FP32_REFINE && (!(P.refine_tol >= 0.0) || P.fp32_stall_checks <= 0))
         throw Err{HPRSOCP_ERR_INVALID, "FP32 mode needs refine_tol >= 0 and fp32_stall_checks > 0"};
     if (P.sigma != -1.0 && !(P.sigma > 0.0)) {
@@ -1510,6 +1570,14 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     if (P.precision == HPRSOCP_PREC_FP32_REFINE) {
         setup_fp32(h);
         for (int i = 0; i < h->nb; ++i) h->I(i).fp32 = 1;
+    } else if (P.precision == HPRSOCP_PREC_FP32_DELTA) {
+        setup_fp32(h);
+        h->fp32 = false;  // the instance is an FP64 one outside the delta iterations
+        h->delta = true;
+        h->d_ra = h->dalloc<double>(n);
+        h->d_img_n = h->dalloc<double>(n);
+        h->d_rya = h->dalloc<double>(m);
+        h->d_img_m = h->dalloc<double>(m);
     }
     upload_isc(h);
     if (h->Q_is_diag) update_q_factors(h);
@@ -1549,6 +1617,8 @@ int hprsocp_create(const hprsocp_problem *prob, const hprsocp_params *params, hp
 
 int hprsocp_estimate_lambda(hprsocp_handle *h) {
     ABI_TRY(h, {
+        delta_join(h);
+        h->fp32_dirty = true;
         const double t0 = now_s();
         const int nb = h->nb;
         std::vector<double> la(nb, 0.0), lq(nb, 0.0);
@@ -1582,6 +1652,7 @@ void set_initial_impl(hprsocp_handle *h, const double *x0, const double *y0, con
                       const std::vector<int32_t> &fy) {
     hipStream_t st = h->st;
     const int32_t m = h->m, n = h->n, nb = h->nb;
+    delta_join(h);
     auto set_flags = [&](const std::vector<int32_t> &f) {
         memcpy(h->flag_h, f.data(), sizeof(int32_t) * nb);
         HIPCHK(hipMemcpyAsync(h->flag, h->flag_h, sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
@@ -1707,6 +1778,56 @@ static void do_update_t(hprsocp_handle *h, IterView<T> &V, int64_t kstep, int fu
     launch_w2<T>(st, pAT, V.AT, w2);
     ev(5);
 }
+// One partial HPR iteration in the delta form (precision FP32_DELTA): the same
+// launch groups as do_update_t, with FP32 SpMVs on the FP32 deltas and the
+// primal / dual epilogues forming raw values and projections in FP64 from the
+// anchors (DPrimalArgs).  The w / aty update needs no anchor: OpW2 on the
+// deltas with null anchors is the delta update.
+static void do_update_delta_t(hprsocp_handle *h, int64_t kstep) {
+    hipStream_t st = h->st;
+    IterView<float> &V = h->v32;
+    auto ev = [&](int k) {
+        if (h->profile) HIPCHK(hipEventRecord(h->ev[k], st));
+    };
+    const int sdiag = h->q_struct_diag && h->q_kind == HPRSOCP_Q_CSR;
+    DPrimalArgs a{V.x, V.x_hat, V.w_bar, V.tempv, V.y, V.w, V.aty, V.qw, h->d_ra, h->last_x, h->last_w, h->l, h->u,
+                  h->diag_Q, h->fact1, h->fact2, h->isc, kstep, sdiag, h->Q_is_diag};
+    DDualArgs d{V.y, V.y_bar, h->d_rya, h->last_y, h->AL, h->AU, h->isc, kstep};
+    if (!h->has_quad) {
+        ev(0);
+        launch_dprimal(st, h->plan_AT, V.AT, h->var_idx, a, false, h->raw_n, h->part_n);
+        ev(1);
+        if (h->m > 0) launch_ddual(st, h->plan_A, V.A, h->con_idx, V.x_hat, d, h->raw_m, h->part_m);
+        ev(2);
+        return;
+    }
+    ev(0);
+    if (!sdiag) qmap_t<float>(h, V, V.w, V.qw);  // Q dw
+    ev(1);
+    launch_dprimal(st, h->plan_AT, V.AT, h->var_idx, a, true, h->raw_n, h->part_n);
+    ev(2);
+    if (h->m > 0) {
+        if (!sdiag) {  // tempv - last_x = (x_hat - last_x) + sigma (Q dw - Q (w_bar - last_w))
+            if (h->q_kind == HPRSOCP_Q_FACTORED) {
+                launch_spmv<float>(st, h->plan_F, V.F, V.w_bar, V.Ftmp);
+                TempvArgsT<float> t{V.x_hat, V.qw, V.tempv, V.Fdiag, V.w_bar, h->isc};
+                launch_spmv_tempv<float>(st, h->plan_FT, V.FT, V.Ftmp, t);
+            } else {
+                TempvArgsT<float> t{V.x_hat, V.qw, V.tempv, nullptr, nullptr, h->isc};
+                launch_spmv_tempv<float>(st, h->plan_Q, V.Q, V.w_bar, t);
+            }
+        }
+        ev(3);
+        launch_ddual(st, h->plan_A, V.A, h->con_idx, V.tempv, d, h->raw_m, h->part_m);
+    } else {
+        ev(3);
+    }
+    ev(4);
+    W2ArgsT<float> w2{V.y_bar, V.w,    V.w_bar, V.aty,  nullptr, nullptr, nullptr, nullptr,
+                      nullptr, V.fact, h->isc,  kstep,  0,       h->Q_is_diag};
+    launch_w2<float>(st, h->plan_AT, V.AT, w2);
+    ev(5);
+}
 }  // extern "C++"
 
 // Sets every instance's Halpern counter (InstScal::inner0) and restarts the
@@ -1724,6 +1845,22 @@ static void set_inner(hprsocp_handle *h, const std::vector<int64_t> &inner) {
 static void do_update(hprsocp_handle *h, int full, bool sync64 = false) {
     h->to_check = full;
     const int64_t k = h->kstep++;
+    if (h->delta) {  // check iterations in FP64 on the joined state, the others on the deltas
+        if (full) {
+            delta_join(h);
+            delta_prec(h, 0);
+            IterView<double> V = view64(h);
+            do_update_t<double>(h, V, k, 1);
+            h->fp32_dirty = true;
+        } else {
+            if (h->fp32_dirty) delta_split(h);
+            delta_prec(h, 1);
+            do_update_delta_t(h, k);
+            h->delta_live = true;
+            if (sync64) delta_join(h);
+        }
+        return;
+    }
     if (h->n32 > 0) {
         if (h->fp32_dirty) demote_state(h);
         do_update_t<float>(h, h->v32, k, full);
@@ -1768,8 +1905,13 @@ static void run_partial_updates(hprsocp_handle *h, int64_t L) {
         return;
     }
     h->to_check = 0;
-    if (h->n32 > 0 && h->fp32_dirty) demote_state(h);
-    const int fp = (h->n32 > 0 ? 1 : 0) | (h->n64 > 0 ? 2 : 0);
+    if (h->delta) {
+        if (h->fp32_dirty) delta_split(h);
+        delta_prec(h, 1);
+    } else if (h->n32 > 0 && h->fp32_dirty) {
+        demote_state(h);
+    }
+    const int fp = h->delta ? 4 : (h->n32 > 0 ? 1 : 0) | (h->n64 > 0 ? 2 : 0);
     const auto key = std::make_tuple(h->kstep, (int32_t)L, fp);
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
@@ -1781,6 +1923,7 @@ static void run_partial_updates(hprsocp_handle *h, int64_t L) {
         for (int64_t k = 0; k < L; ++k) {
             if (fp & 1) do_update_t<float>(h, h->v32, h->kstep + k, 0);
             if (fp & 2) do_update_t<double>(h, V, h->kstep + k, 0);
+            if (fp & 4) do_update_delta_t(h, h->kstep + k);
         }
         HIPCHK(hipStreamEndCapture(h->st, &g));
         const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -1790,6 +1933,7 @@ static void run_partial_updates(hprsocp_handle *h, int64_t L) {
     }
     HIPCHK(hipGraphLaunch(it->second, h->st));
     h->kstep += L;
+    if (h->delta) h->delta_live = true;
 }
 
 int hprsocp_update(hprsocp_handle *h, int64_t inner, int32_t to_check) {
@@ -1808,6 +1952,7 @@ int hprsocp_update(hprsocp_handle *h, int64_t inner, int32_t to_check) {
 // compute_residuals! (algorithm.jl:113-196) for every instance; r has nb entries.
 static void residuals_impl(hprsocp_handle *h, int64_t iter, hprsocp_residuals *r) {
     hipStream_t st = h->st;
+    delta_join(h);
     const int32_t m = h->m, n = h->n;
     const bool skip_q = !h->has_quad;
     if (!skip_q) qmap(h, h->x_bar, h->qx);
@@ -2537,6 +2682,7 @@ bool should_print(const hprsocp_params &p, int64_t iter, double t_start) {
 void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
     const hprsocp_params &p = h->params;
     const double t_start = now_s();
+    delta_join(h);
     if (!h->lambda_set) hprsocp_estimate_lambda(h);
     const int nb = h->nb;
     std::vector<Restart> R((size_t)nb);
@@ -2738,14 +2884,14 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
             };
             while (plain(nx + L) && L < check_iter) ++L;
             for (int i = 0; i < nb; ++i)
-                if (active[i] && h->I(i).fp32) iter32[i] += L;
+                if (active[i] && (h->I(i).fp32 || h->delta)) iter32[i] += L;
             run_partial_updates(h, L);
             for (int i = 0; i < nb; ++i)
                 if (active[i]) R[i].inner += L - 1;
             iter += L - 1;
         } else {
             for (int i = 0; i < nb; ++i)
-                if (active[i] && h->I(i).fp32) ++iter32[i];
+                if (active[i] && h->I(i).fp32 && !h->delta) ++iter32[i];
             do_update(h, 1);
         }
         const bool need_cur = (nx % check_iter) == 0;
@@ -2770,6 +2916,7 @@ extern "C" {
 
 int hprsocp_mnorm(hprsocp_handle *h, double *out) {
     ABI_TRY(h, {
+        delta_join(h);
         mnorm_enqueue(h, SL_MN0);
         read_slots(h);
         MnormCtx c;
@@ -2879,6 +3026,7 @@ int hprsocp_get_vector(hprsocp_handle *h, int32_t id, double *out) {
         return HPRSOCP_ERR_INVALID;
     }
     ABI_TRY(h, {
+        delta_join(h);
         if (len > 0) HIPCHK(hipMemcpyAsync(out, p, sizeof(double) * len, hipMemcpyDeviceToHost, h->st));
         h->sync();
     });
@@ -2893,6 +3041,7 @@ int hprsocp_set_vector(hprsocp_handle *h, int32_t id, const double *in) {
         return HPRSOCP_ERR_INVALID;
     }
     ABI_TRY(h, {
+        delta_join(h);
         if (len > 0) HIPCHK(hipMemcpyAsync(p, in, sizeof(double) * len, hipMemcpyHostToDevice, h->st));
         h->fp32_dirty = true;
         h->sync();
@@ -2917,6 +3066,7 @@ int hprsocp_get_scalars(hprsocp_handle *h, hprsocp_scalars *o) {
 
 int hprsocp_set_scalars(hprsocp_handle *h, double sigma, double la, double lq) {
     ABI_TRY(h, {
+        delta_join(h);
         for (int i = 0; i < h->nb; ++i) {
             if (sigma >= 0) h->I(i).sigma = sigma;
             if (la >= 0) h->I(i).lambda_A = la;
@@ -2974,6 +3124,14 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
         HIPCHK(hipEventCreate(&e1));
         for (int i = 0; i < h->nb; ++i) h->I(i).active = 1;
         set_inner(h, std::vector<int64_t>(h->nb, 0));
+        // the FP32 copies of a changed FP64 state are made before the timed
+        // region (a solve makes them once per check window)
+        if (h->delta) {
+            if (h->fp32_dirty) delta_split(h);
+            delta_prec(h, 1);
+        } else if (h->n32 > 0 && h->fp32_dirty) {
+            demote_state(h);
+        }
         HIPCHK(hipEventRecord(e0, h->st));
         for (int k = 0; k < iters; ++k) do_update(h, 0);
         HIPCHK(hipEventRecord(e1, h->st));
@@ -3002,6 +3160,7 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
             for (int j = 0; j < 6; ++j) ms_kernel[j] = iters > 0 ? acc[j] / iters : 0.0;
         }
         if (h->n32 > 0) promote_state(h);
+        delta_join(h);
         h->sync();
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

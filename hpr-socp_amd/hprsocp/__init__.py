@@ -81,7 +81,7 @@ class HPRSOCP_parameters:
     verbose: bool = True
     use_gpu: bool = True
     # extensions of this backend
-    precision: str = "fp64"          # "fp64" | "fp32_refine"
+    precision: str = "fp64"          # "fp64" | "fp32_refine" | "fp32_delta" (single problems)
     refine_tol: float = 1e-6         # FP32 mode: KKT at which FP64 iterations take over
     fp32_stall_checks: int = 40      # FP32 mode: stalled checks before FP64 takes over
     use_graph: bool = True
@@ -124,7 +124,8 @@ class HPRSOCP_parameters:
         p.save_filename = fname
         p.verbose = int(bool(self.verbose))
         p.use_gpu = int(bool(self.use_gpu))
-        p.precision = {"fp64": _abi.PREC_FP64, "fp32_refine": _abi.PREC_FP32_REFINE}[self.precision]
+        p.precision = {"fp64": _abi.PREC_FP64, "fp32_refine": _abi.PREC_FP32_REFINE,
+                       "fp32_delta": _abi.PREC_FP32_DELTA}[self.precision]
         p.refine_tol = float(self.refine_tol)
         p.fp32_stall_checks = int(self.fp32_stall_checks)
         p.use_graph = int(bool(self.use_graph))

@@ -20,7 +20,7 @@ STATUS_NAMES = {0: "OPTIMAL", 1: "MAX_ITER", 2: "TIME_LIMIT", 3: "CONTINUE"}
 # hprsocp_q_kind
 Q_CSR, Q_FACTORED = 0, 1
 # hprsocp_precision
-PREC_FP64, PREC_FP32_REFINE = 0, 1
+PREC_FP64, PREC_FP32_REFINE, PREC_FP32_DELTA = 0, 1, 2
 BC_NORMS = {"l2": 0, "rms": 1, "linf": 2, "l_inf": 2, "inf": 2}
 SOC_STRATEGIES = {
     "max": 1, "rms": 2, "geom": 3, "adaptive": 4, "hybrid": 5, "ruiz_geom": 6,

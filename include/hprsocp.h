@@ -75,13 +75,20 @@ typedef enum {
 /* Arithmetic of the iteration. */
 typedef enum {
     HPRSOCP_PREC_FP64 = 0,         /* plain FP64 method (reference numerics)          */
-    HPRSOCP_PREC_FP32_REFINE = 1   /* FP32 bulk: SpMV, projections and Halpern
+    HPRSOCP_PREC_FP32_REFINE = 1,  /* FP32 bulk: SpMV, projections and Halpern
                                       averaging in FP32 (cone norms t^2 - |xi|^2 in
                                       FP64); every check window ends in FP64:
                                       residuals, gap, M-norm, restart and sigma run
                                       on the FP64 copy of the state; FP64 iterations
                                       take over once the KKT residual reaches
                                       refine_tol or stalls in FP32                    */
+    HPRSOCP_PREC_FP32_DELTA = 2    /* FP32 deltas from FP64 anchors: the state is the
+                                      FP64 Halpern anchor of the restart cycle plus an
+                                      FP32 difference; SpMVs run in FP32 on the
+                                      differences, raw values and projections in
+                                      FP64, and every restart moves the anchor (an
+                                      FP64 refinement).  Check iterations run in
+                                      FP64.  Single-problem handles only              */
 } hprsocp_precision;
 
 /* ------------------------------------------------------------ problem */
