@@ -7,13 +7,15 @@ variables, m_A = 5e5 linear rows with log-normal row lengths (mean 20) plus
 1e5 x 1e6 at 5 nnz per column (factored operator form); seeded synthetic data.
 A "step" is one HPR iteration (main update, partial outputs) over the resident
 instance; `value` = FP64 HPR iterations/s.  Also measured: FP32+refine
-iterations/s, time to KKT 1e-6 of full solves in both precisions (time-capped),
+iterations/s of the two FP32 forms (fp32_refine: FP32 state, FP64 checks, FP64
+takeover; fp32_delta: FP32 differences from the FP64 restart anchors), time to
+KKT 1e-6 of full solves in all three precisions (time-capped),
 the roofline of the dominant launch group (HIP events on the solver's stream)
 and the CPU oracle on a bounded sample.
 
 Batch (--config c3, and part of the default run unless --no-batch): 512
 portfolio SOCPs, n log-uniform in [200, 20000], solved together by the batched
-backend (FP64 and FP32+refine) against sequential single-problem FP64 solves
+backend (all three precisions) against sequential single-problem FP64 solves
 (timed on an evenly spaced sample and scaled to the batch).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c0|c3] [--scale s]
@@ -31,6 +33,14 @@ PKG = os.path.join(ROOT, "hpr-socp_amd")
 sys.path.insert(0, PKG)
 
 METRIC = "time-to-1e-6 KKT & HPR iters/s (FP64 vs FP32+refine); batched solves/s"
+SINGLE_PRECS = ["fp64", "fp32_refine", "fp32_delta"]
+BATCH_PRECS = ["fp64", "fp32_refine", "fp32_delta"]
+
+
+def precisions(args, every):
+    if args.precision == "all":
+        return every
+    return ["fp64", "fp32_refine"] if args.precision == "both" else [args.precision]
 PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per launch of the dominant launch group, from two rocprofv3 --pmc
@@ -206,7 +216,7 @@ def bench_single(args):
     model, wl = make_workload(args.config, args.scale)
     m, n = model.A.shape
     log(f"workload {wl} generated in {time.time() - t0:.1f}s")
-    precs = ["fp64", "fp32_refine"] if args.precision == "both" else [args.precision]
+    precs = precisions(args, SINGLE_PRECS)
     res = {}
     for prec in precs:
         p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=args.single_stoptol, time_limit=args.solve_time,
@@ -240,7 +250,7 @@ def bench_single(args):
 
     base = res[precs[0]]
     ms = base["ms_per_iter"]
-    vbytes = 4 if precs[0] == "fp32_refine" else 8
+    vbytes = 8 if precs[0] == "fp64" else 4
     bytes_by, order = algorithmic_bytes(model, vbytes)
     groups = [(name, base["kern"][i]) for i, name in enumerate(order)]
     dom, dom_ms = max(groups, key=lambda t: t[1])
@@ -264,11 +274,17 @@ def bench_single(args):
     for prec, rec in res.items():
         forms[prec] = {"iters_per_s": round(1000.0 / rec["ms_per_iter"], 2),
                        "ms_per_iter": round(rec["ms_per_iter"], 5), "solve_to_1e-6": rec["solve"]}
-    if "fp64" in res and "fp32_refine" in res:
-        s64, s32 = res["fp64"]["solve"], res["fp32_refine"]["solve"]
-        forms["fp32_vs_fp64_iter_rate"] = round(res["fp64"]["ms_per_iter"] / res["fp32_refine"]["ms_per_iter"], 3)
-        if s64 and s32 and s64["time_to_1e-6_s"] and s32["time_to_1e-6_s"]:
-            forms["fp32_vs_fp64_time_to_1e-6"] = round(s64["time_to_1e-6_s"] / s32["time_to_1e-6_s"], 3)
+    if "fp64" in res:
+        vs = {}
+        for prec in ("fp32_refine", "fp32_delta"):
+            if prec not in res:
+                continue
+            s64, s32 = res["fp64"]["solve"], res[prec]["solve"]
+            vs[prec] = {"iter_rate": round(res["fp64"]["ms_per_iter"] / res[prec]["ms_per_iter"], 3)}
+            if s64 and s32 and s64["time_to_1e-6_s"] and s32["time_to_1e-6_s"]:
+                vs[prec]["time_to_1e-6"] = round(s64["time_to_1e-6_s"] / s32["time_to_1e-6_s"], 3)
+        if vs:
+            forms["vs_fp64"] = vs
 
     return {
         "metric": METRIC,
@@ -281,7 +297,7 @@ def bench_single(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if precs[0] == "fp32_refine" else "f64",
+        "dtype": "f64" if precs[0] == "fp64" else "f32",
         "data": "synthetic (seeded feasibility construction)",
         "device": device_info(),
         "config": {"workload": wl, "config": args.config, "scale": args.scale, "m": m, "n": n,
@@ -333,7 +349,8 @@ def bench_batch(args, name="c3"):
     tol = args.stoptol
     out = {"workload": wl, "count": N}
     objs = None
-    for prec in (["fp64", "fp32_refine"] if args.precision == "both" else [args.precision]):
+    bprecs = precisions(args, BATCH_PRECS)
+    for prec in bprecs:
         p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=tol, time_limit=args.solve_time, precision=prec,
                                        use_graph=not args.no_graph)
         t0 = time.time()
@@ -397,7 +414,7 @@ def bench_batch(args, name="c3"):
         out["cpu_oracle"] = {"sample": int(Kc), "sample_wall_s": round(t_cpu, 3),
                              "solves_per_s": round(N / (t_cpu / frac_c), 4)}
         log(f"[cpu oracle] {out['cpu_oracle']}")
-    best = max(out[k]["solves_per_s"] for k in ("fp64", "fp32_refine") if k in out)
+    best = max(out[k]["solves_per_s"] for k in BATCH_PRECS if k in out)
     out["batch_vs_sequential"] = round(best / out["sequential_fp64"]["solves_per_s"], 2)
     return out
 
@@ -415,7 +432,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched C3 measurement")
     ap.add_argument("--no-graph", action="store_true", help="launch every update eagerly (no HIP graphs)")
     ap.add_argument("--no-gather", action="store_true", help="skip the SpMV gather-ceiling measurement")
-    ap.add_argument("--precision", default="both", choices=["fp64", "fp32_refine", "both"])
+    ap.add_argument("--precision", default="all", choices=["fp64", "fp32_refine", "fp32_delta", "both", "all"],
+                    help="all: fp64, fp32_refine and fp32_delta; both: fp64 and fp32_refine")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--c4-count", type=int, default=64)
@@ -432,7 +450,7 @@ def main():
     if args.config in ("c3", "c4"):
         b = bench_batch(args, args.config)
         first = b.get("fp64") or b.get("fp32_refine")
-        best = max(b[k]["solves_per_s"] for k in ("fp64", "fp32_refine") if k in b)
+        best = max(b[k]["solves_per_s"] for k in BATCH_PRECS if k in b)
         out = {"metric": METRIC, "value": best, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
                "warmup": args.warmup, "ms_per_step": first["ms_per_iter"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)",

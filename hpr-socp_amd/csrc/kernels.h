@@ -55,15 +55,17 @@ void launch_demote(hipStream_t st, int64_t n, const double *src, float *dst);
 void launch_promote(hipStream_t st, int64_t n, const float *src, double *dst);
 // dst = (double)src on the chunks of the active instances whose updates run in FP32
 void launch_promote_prec(hipStream_t st, const SegPlan &sp, const float *src, double *dst, const InstScal *isc);
-// delta form (one instance): d = (float)(v - anchor); v = anchor + (double)d
-void launch_delta_split(hipStream_t st, int64_t n, const double *v, const double *anchor, float *d);
-void launch_delta_join(hipStream_t st, int64_t n, const double *anchor, const float *d, double *v);
-// ra = xa + sigma ((img - q) - c), q = qwa, or dq .* wa when qwa is null (both
-// null: q = 0); rya = A xa - rhs (rows >= m1) - f1 ya
-void launch_delta_ra(hipStream_t st, int64_t n, const double *xa, const double *img, const double *qwa,
-                     const double *dq, const double *wa, const double *c, double sigma, double *ra);
-void launch_delta_rya(hipStream_t st, int64_t m, int64_t m1, const double *axa, const double *rhs, const double *ya,
-                      double f1, double *rya);
+// delta form, on the chunks of the active instances: d = (float)(v - anchor);
+// v = anchor + (double)d; ra = xa + sigma ((img - qwa) - c) (qwa may be null);
+// rya = A xa - rhs (rows >= m1, the SOC rows) - lambda_A sigma ya
+void launch_delta_split(hipStream_t st, const SegPlan &sp, const double *v, const double *anchor, float *d,
+                        const InstScal *isc);
+void launch_delta_join(hipStream_t st, const SegPlan &sp, const double *anchor, const float *d, double *v,
+                       const InstScal *isc);
+void launch_delta_ra(hipStream_t st, const SegPlan &sp, const double *xa, const double *img, const double *qwa,
+                     const double *c, const InstScal *isc, double *ra);
+void launch_delta_rya(hipStream_t st, const SegPlan &sp, int64_t m1, const double *axa, const double *rhs,
+                      const double *ya, const InstScal *isc, double *rya);
 
 // ---- residual kernels (compute_Rd!/compute_Rp!, kernels.jl:118-292) ---------
 struct ResidNArgs {

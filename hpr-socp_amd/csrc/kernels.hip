@@ -121,38 +121,6 @@ void launch_demote(hipStream_t st, int64_t n, const double *src, float *dst) {
 void launch_promote(hipStream_t st, int64_t n, const float *src, double *dst) {
     if (n > 0) hipLaunchKernelGGL(k_promote, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, src, dst);
 }
-__global__ void k_delta_split(int64_t n, const double *v, const double *a, float *d) {
-    GRID_STRIDE(i, n) d[i] = (float)(v[i] - a[i]);
-}
-__global__ void k_delta_join(int64_t n, const double *a, const float *d, double *v) {
-    GRID_STRIDE(i, n) v[i] = a[i] + (double)d[i];
-}
-__global__ void k_delta_ra(int64_t n, const double *xa, const double *img, const double *qwa, const double *dq,
-                           const double *wa, const double *c, double sigma, double *ra) {
-    GRID_STRIDE(i, n) {
-        const double q = qwa ? qwa[i] : (dq ? dq[i] * wa[i] : 0.0);
-        ra[i] = xa[i] + sigma * ((-q + img[i]) - c[i]);
-    }
-}
-__global__ void k_delta_rya(int64_t m, int64_t m1, const double *axa, const double *rhs, const double *ya, double f1,
-                            double *rya) {
-    GRID_STRIDE(i, m) rya[i] = (i >= m1 ? axa[i] - rhs[i] : axa[i]) - f1 * ya[i];
-}
-void launch_delta_split(hipStream_t st, int64_t n, const double *v, const double *anchor, float *d) {
-    if (n > 0) hipLaunchKernelGGL(k_delta_split, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, v, anchor, d);
-}
-void launch_delta_join(hipStream_t st, int64_t n, const double *anchor, const float *d, double *v) {
-    if (n > 0) hipLaunchKernelGGL(k_delta_join, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, anchor, d, v);
-}
-void launch_delta_ra(hipStream_t st, int64_t n, const double *xa, const double *img, const double *qwa,
-                     const double *dq, const double *wa, const double *c, double sigma, double *ra) {
-    if (n > 0)
-        hipLaunchKernelGGL(k_delta_ra, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, xa, img, qwa, dq, wa, c, sigma, ra);
-}
-void launch_delta_rya(hipStream_t st, int64_t m, int64_t m1, const double *axa, const double *rhs, const double *ya,
-                      double f1, double *rya) {
-    if (m > 0) hipLaunchKernelGGL(k_delta_rya, dim3(ew_blocks(m)), dim3(kBlock), 0, st, m, m1, axa, rhs, ya, f1, rya);
-}
 void launch_axpby(hipStream_t st, int64_t n, double a, const double *x, double b, const double *y, double *out) {
     if (n > 0) hipLaunchKernelGGL(k_axpby, dim3(ew_blocks(n)), dim3(kBlock), 0, st, n, a, x, b, y, out);
 }
@@ -735,6 +703,50 @@ __global__ void k_promote_prec(SegPlan sp, const float *src, double *dst, const 
 }
 void launch_promote_prec(hipStream_t st, const SegPlan &sp, const float *src, double *dst, const InstScal *isc) {
     if (sp.nchunks > 0) hipLaunchKernelGGL(k_promote_prec, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, src, dst, isc);
+}
+
+__global__ void k_delta_split(SegPlan sp, const double *v, const double *a, float *d, const InstScal *isc) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    if (!isc[ch.inst].active) return;
+    CHUNK_LOOP(i, ch) d[i] = (float)(v[i] - a[i]);
+}
+void launch_delta_split(hipStream_t st, const SegPlan &sp, const double *v, const double *anchor, float *d,
+                        const InstScal *isc) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_delta_split, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, v, anchor, d, isc);
+}
+__global__ void k_delta_join(SegPlan sp, const double *a, const float *d, double *v, const InstScal *isc) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    if (!isc[ch.inst].active) return;
+    CHUNK_LOOP(i, ch) v[i] = a[i] + (double)d[i];
+}
+void launch_delta_join(hipStream_t st, const SegPlan &sp, const double *anchor, const float *d, double *v,
+                       const InstScal *isc) {
+    if (sp.nchunks > 0) hipLaunchKernelGGL(k_delta_join, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, anchor, d, v, isc);
+}
+__global__ void k_delta_ra(SegPlan sp, const double *xa, const double *img, const double *qwa, const double *c,
+                           const InstScal *isc, double *ra) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const InstScal s = isc[ch.inst];
+    if (!s.active) return;
+    CHUNK_LOOP(i, ch) ra[i] = xa[i] + s.sigma * ((-(qwa ? qwa[i] : 0.0) + img[i]) - c[i]);
+}
+void launch_delta_ra(hipStream_t st, const SegPlan &sp, const double *xa, const double *img, const double *qwa,
+                     const double *c, const InstScal *isc, double *ra) {
+    if (sp.nchunks > 0)
+        hipLaunchKernelGGL(k_delta_ra, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, xa, img, qwa, c, isc, ra);
+}
+__global__ void k_delta_rya(SegPlan sp, int64_t m1, const double *axa, const double *rhs, const double *ya,
+                            const InstScal *isc, double *rya) {
+    const SegChunk ch = sp.chunks[blockIdx.x];
+    const InstScal s = isc[ch.inst];
+    if (!s.active) return;
+    const double f1 = s.lambda_A * s.sigma;
+    CHUNK_LOOP(i, ch) rya[i] = (i >= m1 ? axa[i] - rhs[i] : axa[i]) - f1 * ya[i];
+}
+void launch_delta_rya(hipStream_t st, const SegPlan &sp, int64_t m1, const double *axa, const double *rhs,
+                      const double *ya, const InstScal *isc, double *rya) {
+    if (sp.nchunks > 0)
+        hipLaunchKernelGGL(k_delta_rya, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, m1, axa, rhs, ya, isc, rya);
 }
 
 __global__ void k_copy_flagged(SegPlan sp, const double *src, double *dst, const int32_t *flag) {

@@ -1162,11 +1162,15 @@ void promote_state(hprsocp_handle *h) {
 #undef HPR_PRO_M
 }
 
-// Delta form: the instance runs the FP32 kernels (f32 = 1) or the FP64 ones.
+// Delta form: the instances run the FP32 kernels (f32 = 1) or the FP64 ones.
 void delta_prec(hprsocp_handle *h, int f32) {
-    if (h->I(0).fp32 == f32) return;
-    h->I(0).fp32 = f32;
-    upload_isc(h);
+    bool change = false;
+    for (int i = 0; i < h->nb; ++i)
+        if (h->I(i).fp32 != f32) {
+            h->I(i).fp32 = f32;
+            change = true;
+        }
+    if (change) upload_isc(h);
 }
 
 // Delta form: FP64 state -> deltas from the anchors, the anchors' images
@@ -1176,23 +1180,23 @@ void delta_split(hprsocp_handle *h) {
     hipStream_t st = h->st;
     IterView<float> &V = h->v32;
     const int64_t n = h->n, m = h->m;
-    launch_delta_split(st, n, h->x, h->last_x, V.x);
-    launch_delta_split(st, m, h->y, h->last_y, V.y);
-    const double sigma = h->I(0).sigma;
+    const SegPlan &sn = h->seg_n, &sm = h->seg_m;
+    launch_delta_split(st, sn, h->x, h->last_x, V.x, h->isc);
+    launch_delta_split(st, sm, h->y, h->last_y, V.y, h->isc);
     if (h->has_quad) {
-        launch_delta_split(st, n, h->w, h->last_w, V.w);
-        launch_delta_split(st, n, h->aty, h->last_aty, V.aty);
+        launch_delta_split(st, sn, h->w, h->last_w, V.w, h->isc);
+        launch_delta_split(st, sn, h->aty, h->last_aty, V.aty, h->isc);
         if (h->Q_is_diag) launch_demote(st, n, h->fact, V.fact);
         if (V.Fdiag) launch_demote(st, n, h->Fdiag, V.Fdiag);
         qmap(h, h->last_w, h->d_img_n);  // Q last_w
-        launch_delta_ra(st, n, h->last_x, h->last_aty, h->d_img_n, nullptr, nullptr, h->c, sigma, h->d_ra);
+        launch_delta_ra(st, sn, h->last_x, h->last_aty, h->d_img_n, h->c, h->isc, h->d_ra);
     } else {
         launch_spmv(st, h->plan_AT, h->AT, h->last_y, h->d_img_n);  // A^T last_y
-        launch_delta_ra(st, n, h->last_x, h->d_img_n, nullptr, nullptr, nullptr, h->c, sigma, h->d_ra);
+        launch_delta_ra(st, sn, h->last_x, h->d_img_n, nullptr, h->c, h->isc, h->d_ra);
     }
     if (m > 0) {
         launch_spmv(st, h->plan_A, h->A, h->last_x, h->d_img_m);  // A last_x
-        launch_delta_rya(st, m, h->m1, h->d_img_m, h->rhs, h->last_y, h->I(0).lambda_A * sigma, h->d_rya);
+        launch_delta_rya(st, sm, h->m1, h->d_img_m, h->rhs, h->last_y, h->isc, h->d_rya);
     }
     h->fp32_dirty = false;
 }
@@ -1203,12 +1207,12 @@ void delta_join(hprsocp_handle *h) {
     if (!h->delta || !h->delta_live) return;
     hipStream_t st = h->st;
     IterView<float> &V = h->v32;
-    const int64_t n = h->n, m = h->m;
-    launch_delta_join(st, n, h->last_x, V.x, h->x);
-    launch_delta_join(st, m, h->last_y, V.y, h->y);
+    const SegPlan &sn = h->seg_n, &sm = h->seg_m;
+    launch_delta_join(st, sn, h->last_x, V.x, h->x, h->isc);
+    launch_delta_join(st, sm, h->last_y, V.y, h->y, h->isc);
     if (h->has_quad) {
-        launch_delta_join(st, n, h->last_w, V.w, h->w);
-        launch_delta_join(st, n, h->last_aty, V.aty, h->aty);
+        launch_delta_join(st, sn, h->last_w, V.w, h->w, h->isc);
+        launch_delta_join(st, sn, h->last_aty, V.aty, h->aty, h->isc);
     }
     h->delta_live = false;
 }
@@ -1266,8 +1270,6 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     if (P.precision != HPRSOCP_PREC_FP64 && P.precision != HPRSOCP_PREC_FP32_REFINE &&
         P.precision != HPRSOCP_PREC_FP32_DELTA)
         throw Err{HPRSOCP_ERR_INVALID, "unknown precision mode"};
-    if (P.precision == HPRSOCP_PREC_FP32_DELTA && ranges)
-        throw Err{HPRSOCP_ERR_UNSUPPORTED, "precision FP32_DELTA applies to single-problem handles, not batches"};
     if (P.precision == HPRSOCP_PREC_FP32_REFINE && (!(P.refine_tol >= 0.0) || P.fp32_stall_checks <= 0))
         throw Err{HPRSOCP_ERR_INVALID, "FP32 mode needs refine_tol >= 0 and fp32_stall_checks > 0"};
     if (P.sigma != -1.0 && !(P.sigma > 0.0)) {
@@ -1790,21 +1792,25 @@ static void do_update_delta_t(hprsocp_handle *h, int64_t kstep) {
         if (h->profile) HIPCHK(hipEventRecord(h->ev[k], st));
     };
     const int sdiag = h->q_struct_diag && h->q_kind == HPRSOCP_Q_CSR;
+    // the running instances' tiles (batches after retirements), as do_update_t
+    const DevPlan &pA = h->use_act ? h->act_A : h->plan_A;
+    const DevPlan &pAT = h->use_act ? h->act_AT : h->plan_AT;
+    const DevPlan &pQ = h->use_act ? h->act_Q : h->plan_Q;
     DPrimalArgs a{V.x, V.x_hat, V.w_bar, V.tempv, V.y, V.w, V.aty, V.qw, h->d_ra, h->last_x, h->last_w, h->l, h->u,
                   h->diag_Q, h->fact1, h->fact2, h->isc, kstep, sdiag, h->Q_is_diag};
     DDualArgs d{V.y, V.y_bar, h->d_rya, h->last_y, h->AL, h->AU, h->isc, kstep};
     if (!h->has_quad) {
         ev(0);
-        launch_dprimal(st, h->plan_AT, V.AT, h->var_idx, a, false, h->raw_n, h->part_n);
+        launch_dprimal(st, pAT, V.AT, h->var_idx, a, false, h->raw_n, h->part_n);
         ev(1);
-        if (h->m > 0) launch_ddual(st, h->plan_A, V.A, h->con_idx, V.x_hat, d, h->raw_m, h->part_m);
+        if (h->m > 0) launch_ddual(st, pA, V.A, h->con_idx, V.x_hat, d, h->raw_m, h->part_m);
         ev(2);
         return;
     }
     ev(0);
     if (!sdiag) qmap_t<float>(h, V, V.w, V.qw);  // Q dw
     ev(1);
-    launch_dprimal(st, h->plan_AT, V.AT, h->var_idx, a, true, h->raw_n, h->part_n);
+    launch_dprimal(st, pAT, V.AT, h->var_idx, a, true, h->raw_n, h->part_n);
     ev(2);
     if (h->m > 0) {
         if (!sdiag) {  // tempv - last_x = (x_hat - last_x) + sigma (Q dw - Q (w_bar - last_w))
@@ -1814,18 +1820,18 @@ static void do_update_delta_t(hprsocp_handle *h, int64_t kstep) {
                 launch_spmv_tempv<float>(st, h->plan_FT, V.FT, V.Ftmp, t);
             } else {
                 TempvArgsT<float> t{V.x_hat, V.qw, V.tempv, nullptr, nullptr, h->isc};
-                launch_spmv_tempv<float>(st, h->plan_Q, V.Q, V.w_bar, t);
+                launch_spmv_tempv<float>(st, pQ, V.Q, V.w_bar, t);
             }
         }
         ev(3);
-        launch_ddual(st, h->plan_A, V.A, h->con_idx, V.tempv, d, h->raw_m, h->part_m);
+        launch_ddual(st, pA, V.A, h->con_idx, V.tempv, d, h->raw_m, h->part_m);
     } else {
         ev(3);
     }
     ev(4);
     W2ArgsT<float> w2{V.y_bar, V.w,    V.w_bar, V.aty,  nullptr, nullptr, nullptr, nullptr,
                       nullptr, V.fact, h->isc,  kstep,  0,       h->Q_is_diag};
-    launch_w2<float>(st, h->plan_AT, V.AT, w2);
+    launch_w2<float>(st, pAT, V.AT, w2);
     ev(5);
 }
 }  // extern "C++"
@@ -1877,8 +1883,8 @@ static void do_update(hprsocp_handle *h, int full, bool sync64 = false) {
 static void compact_active(hprsocp_handle *h) {
     const int nb = h->nb;
     std::vector<char> k64(nb), k32(nb);
-    for (int i = 0; i < nb; ++i) {
-        k64[i] = h->I(i).active && !h->I(i).fp32;
+    for (int i = 0; i < nb; ++i) {  // the delta form runs every instance in both kernel sets
+        k64[i] = h->I(i).active && (!h->I(i).fp32 || h->delta);
         k32[i] = h->I(i).active && h->I(i).fp32;
     }
     compact_plan(h, h->hp_A, h->act_A, k64);

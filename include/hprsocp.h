@@ -88,7 +88,7 @@ typedef enum {
                                       differences, raw values and projections in
                                       FP64, and every restart moves the anchor (an
                                       FP64 refinement).  Check iterations run in
-                                      FP64.  Single-problem handles only              */
+                                      FP64                                            */
 } hprsocp_precision;
 
 /* ------------------------------------------------------------ problem */

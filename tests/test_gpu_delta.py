@@ -159,8 +159,3 @@ def test_delta_graph_matches_eager(gpu_lib):
     assert out[0][0] == out[1][0] == 1000
     np.testing.assert_array_equal(out[0][1], out[1][1])
     np.testing.assert_array_equal(out[0][2], out[1][2])
-
-
-def test_delta_rejected_for_batches(gpu_lib):
-    with pytest.raises(hprsocp.HPRSOCPError, match="FP32_DELTA"):
-        hprsocp.BatchSolver([model("c0"), model("c0")], params(**DELTA))
