@@ -2753,7 +2753,7 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
             else if (iter == p.max_iter) status[i] = HPRSOCP_STATUS_MAX_ITER;
             else if (tnow > p.time_limit) status[i] = HPRSOCP_STATUS_TIME_LIMIT;
         }
-        if (h->n32 > 0 && iter > 0 && check) {
+        if (h->n32 > 0 && !h->delta && iter > 0 && check) {
             // the FP64 state holds this check's iterate (the last update was a full one).
             // Stalled: no 1% improvement of the best KKT over fp32_stall_checks checks
             // that also saw two restarts (a long restart cycle alone is not a stall).
@@ -3165,7 +3165,7 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
             h->profile = false;
             for (int j = 0; j < 6; ++j) ms_kernel[j] = iters > 0 ? acc[j] / iters : 0.0;
         }
-        if (h->n32 > 0) promote_state(h);
+        if (h->n32 > 0 && !h->delta) promote_state(h);  // (the delta form's mirrors hold differences)
         delta_join(h);
         h->sync();
         (void)hipEventDestroy(e0);

@@ -164,9 +164,9 @@ typedef struct {
     int32_t precision;         /* hprsocp_precision, default FP64             */
     int32_t use_graph;         /* capture the non-check iterations of a
                                   check window in one hipGraph (default 1)    */
-    double refine_tol;         /* FP32 mode: KKT below which the iteration
+    double refine_tol;         /* FP32_REFINE: KKT below which the iteration
                                   continues in FP64 (default 1e-6)            */
-    int32_t fp32_stall_checks; /* FP32 mode: checks without a 1% improvement
+    int32_t fp32_stall_checks; /* FP32_REFINE: checks without a 1% improvement
                                   of the best KKT (and with two restarts or
                                   more) before FP64 takes over (default 40)   */
     int32_t reserved[7];
@@ -184,7 +184,7 @@ typedef struct {
     double sigma, lambda_max_A, lambda_max_Q;
     int32_t status;            /* hprsocp_status                              */
     int32_t restarts;
-    int64_t iter_fp32;         /* iterations run in FP32 (precision FP32_REFINE) */
+    int64_t iter_fp32;         /* iterations run on the FP32 kernels (FP32 forms) */
 } hprsocp_result;
 
 /* Residual block, HPRSOCP_residuals (reference src/structs.jl:281-294). */

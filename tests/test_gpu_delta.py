@@ -159,3 +159,23 @@ def test_delta_graph_matches_eager(gpu_lib):
     assert out[0][0] == out[1][0] == 1000
     np.testing.assert_array_equal(out[0][1], out[1][1])
     np.testing.assert_array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("name", ["qp_factored", "c0"])
+def test_delta_timing_leaves_the_state(name, oracle_lib, gpu_lib):
+    """hprsocp_time_iterations runs partial steps and then joins the deltas
+    into the FP64 state: the anchors are untouched and the iterate is the
+    oracle's after the same steps (the FP32 form's mirror promotion must not
+    run on the delta form's mirrors, which hold differences)."""
+    g, o, kind = start_pair(name, oracle_lib)
+    with g, o:
+        anchors = {ANCHOR[v]: np.asarray(o.get(ANCHOR[v])).copy() for v in STATE[kind]}
+        g.time_iterations(5)
+        for k in range(5):
+            o.update(k, False)
+        D = distance(o, kind)
+        for a, val in anchors.items():
+            np.testing.assert_array_equal(g.get(a), val)
+        for v in STATE[kind]:
+            b = o.get(v)
+            assert abs_err(g.get(v), b) <= 1e-4 * D + 1e-11 * (1 + inf(b)), v
