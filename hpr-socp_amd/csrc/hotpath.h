@@ -13,15 +13,22 @@ constexpr int kTileRows = 256;    // max rows of one tile (one row per thread in
 constexpr int kStreamNnz = 2048;  // LDS product buffer of the CSR-stream SpMV paths
 constexpr int kSliceRows = 1024;  // rows of one slice tile
 constexpr int kSliceTileNnz = 60000;  // cap of one slice tile (per-slice offsets fit uint16)
-// Column-sorted slice tiles: a chunk of <= kSortChunk consecutive (row-major)
-// entries of one slice tile is stored sorted by column, each entry packed as
-// (column - slice start) << kSortPosBits | (row-major position in the chunk),
-// so that the lanes of one gather instruction hit few distinct cache lines.
-constexpr int kSortChunk = 8192;   // LDS products of one chunk (power of two)
-constexpr int kSortPosBits = 13;   // log2(kSortChunk)
-constexpr int kSortColBits = 32 - kSortPosBits;  // slice width limit 2^19
+// Column-sorted slice tiles: a chunk of <= SortSpec<T>::chunk consecutive
+// (row-major) entries of one slice tile is stored sorted by column, each entry
+// packed as (column - slice start) << pos_bits | (row-major position in the
+// chunk), so that the lanes of one gather instruction hit few distinct cache
+// lines.  A chunk's products fill 64 KB of LDS in either precision: FP32
+// chunks hold twice the entries, so their columns lie twice as dense.
+template <typename T>
+struct SortSpec {
+    static constexpr int chunk = 65536 / (int)sizeof(T);       // entries of one chunk (power of two)
+    static constexpr int pos_bits = sizeof(T) == 8 ? 13 : 14;  // log2(chunk)
+    static constexpr int col_bits = 32 - pos_bits;             // slice width limit 2^col_bits
+    static constexpr int rows = chunk / 2;                     // rows of one sorted slice tile
+};
 constexpr int kSortBlock = 512;    // threads of the sorted-slice kernel
-constexpr int kSortRows = 4096;    // rows of one sorted slice tile
+constexpr int kSortChunk = SortSpec<double>::chunk;       // the FP64 layout (gather probe)
+constexpr int kSortPosBits = SortSpec<double>::pos_bits;
 
 enum TileKind : int32_t { TILE_LINEAR = 0, TILE_CONES = 1, TILE_HUGE = 2 };
 
@@ -82,8 +89,8 @@ struct CsrT {
     // stored per slice as uint16 relative to the slice tile's base:
     //   entries of (slice p, row r) start at s_base[p*(s_ntiles+1) + t] + s_off[p*nrows + r]
     // with t the slice tile holding r; the tile's end is s_base[... + t + 1].
-    // With s_sorted = 1 the entries of each chunk of kSortChunk are in column
-    // order and s_col holds packed words (see kSortChunk); s_ncols gives the
+    // With s_sorted = 1 the entries of each chunk (SortSpec<T>) are in column
+    // order and s_col holds packed words (see SortSpec); s_ncols gives the
     // slice starts (ncols * p / nslices).
     int32_t nslices = 1;
     int32_t s_sorted = 0;

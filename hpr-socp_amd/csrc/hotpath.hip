@@ -360,7 +360,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_slices(CsrT<T> M, const T *__re
 // owns rows t, t + kSortBlock, ...
 template <typename T>
 __global__ __launch_bounds__(kSortBlock) void k_spmv_sorted(CsrT<T> M, const T *__restrict__ x) {
-    __shared__ T prod[kSortChunk];
+    using Spec = SortSpec<T>;
+    __shared__ T prod[Spec::chunk];
     const int P = M.nslices;
     const int p = blockIdx.x % P;
     const int tix = blockIdx.x / P;
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(kSortBlock) void k_spmv_sorted(CsrT<T> M, const T *
     const int32_t tnnz = base[1] - k0;
     const uint16_t *off = M.s_off + (int64_t)p * M.nrows + S.rb;
     const T *__restrict__ xs = x + (int64_t)M.s_ncols * p / P;
-    constexpr int RPT = kSortRows / kSortBlock;
+    constexpr int RPT = Spec::rows / kSortBlock;
     T acc[RPT];
     int32_t ra[RPT], re[RPT];
 #pragma unroll
@@ -384,14 +385,14 @@ __global__ __launch_bounds__(kSortBlock) void k_spmv_sorted(CsrT<T> M, const T *
     }
     const T *__restrict__ val = M.s_val + k0;
     const uint32_t *__restrict__ pk = (const uint32_t *)M.s_col + k0;
-    constexpr uint32_t kPosMask = (1u << kSortPosBits) - 1;
-    for (int c0 = 0; c0 < tnnz; c0 += kSortChunk) {
-        const int c1 = min(c0 + kSortChunk, tnnz);
+    constexpr uint32_t kPosMask = (1u << Spec::pos_bits) - 1;
+    for (int c0 = 0; c0 < tnnz; c0 += Spec::chunk) {
+        const int c1 = min(c0 + Spec::chunk, tnnz);
 #pragma unroll 4
         for (int k = c0 + tid; k < c1; k += kSortBlock) {
             const uint32_t w = __builtin_nontemporal_load(pk + k);
             const T v = __builtin_nontemporal_load(val + k);
-            prod[w & kPosMask] = v * xs[w >> kSortPosBits];
+            prod[w & kPosMask] = v * xs[w >> Spec::pos_bits];
         }
         __syncthreads();
 #pragma unroll

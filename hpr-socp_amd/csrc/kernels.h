@@ -175,6 +175,7 @@ void launch_scale_csr_t(hipStream_t st, int32_t nrows, const int32_t *rowptr, co
 void launch_sym_q(hipStream_t st, int64_t nnz, const double *src, const int32_t *tpos, double *dst);
 // dst[k] = src[perm[k]] (fills the slice-major copy of a matrix' values)
 void launch_gather_perm(hipStream_t st, int64_t nnz, const double *src, const int32_t *perm, double *dst);
+void launch_gather_perm(hipStream_t st, int64_t nnz, const float *src, const int32_t *perm, float *dst);
 constexpr int kMaxGolden = 64;
 // out = (v .* d) ./ s(inst)   (warm-start scaling, algorithm.jl:2628, 2646)
 void launch_scale_vd(hipStream_t st, const SegPlan &sp, const double *v, const double *d, InstVal s, double *out);

@@ -758,11 +758,17 @@ void launch_copy_flagged(hipStream_t st, const SegPlan &sp, const double *src, d
     if (sp.nchunks > 0) hipLaunchKernelGGL(k_copy_flagged, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, src, dst, flag);
 }
 
-__global__ void k_gather_perm(int64_t nnz, const double *src, const int32_t *perm, double *dst) {
+template <typename T>
+__global__ void k_gather_perm(int64_t nnz, const T *src, const int32_t *perm, T *dst) {
     GRID_STRIDE(k, nnz) dst[k] = src[perm[k]];
 }
 void launch_gather_perm(hipStream_t st, int64_t nnz, const double *src, const int32_t *perm, double *dst) {
-    if (nnz > 0) hipLaunchKernelGGL(k_gather_perm, dim3(ew_blocks(nnz)), dim3(kBlock), 0, st, nnz, src, perm, dst);
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_gather_perm<double>, dim3(ew_blocks(nnz)), dim3(kBlock), 0, st, nnz, src, perm, dst);
+}
+void launch_gather_perm(hipStream_t st, int64_t nnz, const float *src, const int32_t *perm, float *dst) {
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_gather_perm<float>, dim3(ew_blocks(nnz)), dim3(kBlock), 0, st, nnz, src, perm, dst);
 }
 
 // symmetrize Q values in place through the transpose-partner map: v = (v + v[tpos]) / 2

@@ -729,7 +729,9 @@ std::vector<double> power_iteration(hprsocp_handle *h, bool onA) {
 // k_spmv_sorted: tiles of <= kSortRows rows whose entries in each slice fill
 // about one chunk of kSortChunk, each chunk stored in column order as packed
 // (column offset, row-major position) words.
-bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double *dval, int32_t P, bool sorted) {
+template <typename T>
+bool build_slices(hprsocp_handle *h, CsrT<T> &M, const HostCsr &hm, const T *dval, int32_t P, bool sorted) {
+    using Spec = SortSpec<T>;
     if (P <= 1 || hm.ci.empty()) return false;
     const int32_t R = hm.nrows;
     const int64_t nnz = (int64_t)hm.ci.size();
@@ -739,7 +741,7 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
     for (int p = 0; p <= P; ++p) bound[p] = (int64_t)hm.ncols * p / P;
     if (sorted)
         for (int p = 0; p < P; ++p)
-            if (bound[p + 1] - bound[p] > (1ll << kSortColBits)) return false;  // column offsets must fit
+            if (bound[p + 1] - bound[p] > (1ll << Spec::col_bits)) return false;  // column offsets must fit
     auto slice_of = [&](int32_t c) {
         int p = (int)((int64_t)c * P / std::max<int64_t>(hm.ncols, 1));
         while (p + 1 < P && c >= bound[p + 1]) ++p;
@@ -765,16 +767,16 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
             r = e;
         }
     } else {
-        // <= kSortRows rows, <= kSortChunk entries in every slice (a single
+        // <= Spec::rows rows, <= Spec::chunk entries in every slice (a single
         // longer row forms a tile of its own; the kernel then runs several chunks)
         std::vector<int64_t> tc((size_t)P);
         int32_t r = 0;
         while (r < R) {
             for (int p = 0; p < P; ++p) tc[p] = cnt[(size_t)p * R + r];
             int32_t e = r + 1;
-            while (e < R && e - r < kSortRows) {
+            while (e < R && e - r < Spec::rows) {
                 bool fits = true;
-                for (int p = 0; p < P && fits; ++p) fits = tc[p] + cnt[(size_t)p * R + e] <= kSortChunk;
+                for (int p = 0; p < P && fits; ++p) fits = tc[p] + cnt[(size_t)p * R + e] <= Spec::chunk;
                 if (!fits) break;
                 for (int p = 0; p < P; ++p) tc[p] += cnt[(size_t)p * R + e];
                 ++e;
@@ -823,17 +825,17 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
                 const int p = (int)(it / NT);
                 const int32_t t = (int32_t)(it % NT);
                 const int64_t b0 = base[(size_t)p * (NT + 1) + t], b1 = base[(size_t)p * (NT + 1) + t + 1];
-                for (int64_t c0 = b0; c0 < b1; c0 += kSortChunk) {
-                    const int64_t c1 = std::min<int64_t>(c0 + kSortChunk, b1);
+                for (int64_t c0 = b0; c0 < b1; c0 += Spec::chunk) {
+                    const int64_t c1 = std::min<int64_t>(c0 + Spec::chunk, b1);
                     const int32_t len = (int32_t)(c1 - c0);
                     key.resize(len);
                     pv.assign(perm.begin() + c0, perm.begin() + c1);
                     for (int32_t j = 0; j < len; ++j)
-                        key[j] = ((uint32_t)(scol[c0 + j] - bound[p]) << kSortPosBits) | (uint32_t)j;
+                        key[j] = ((uint32_t)(scol[c0 + j] - bound[p]) << Spec::pos_bits) | (uint32_t)j;
                     std::sort(key.begin(), key.end());
                     for (int32_t j = 0; j < len; ++j) {
                         scol[c0 + j] = (int32_t)key[j];
-                        perm[c0 + j] = pv[key[j] & ((1u << kSortPosBits) - 1)];
+                        perm[c0 + j] = pv[key[j] & ((1u << Spec::pos_bits) - 1)];
                     }
                 }
             }
@@ -850,19 +852,20 @@ bool build_slices(hprsocp_handle *h, CsrView &M, const HostCsr &hm, const double
     M.s_off = h->upload<uint16_t>(off.data(), (int64_t)off.size());
     M.s_col = h->upload<int32_t>(scol.data(), nnz);
     int32_t *dperm = h->upload<int32_t>(perm.data(), nnz);
-    double *sval = h->dalloc<double>(nnz);
+    T *sval = h->dalloc<T>(nnz);
     launch_gather_perm(h->st, nnz, dval, dperm, sval);
     h->dfree(dperm);
     M.s_val = sval;
     M.s_tiles = h->upload<SliceTile>(tiles.data(), NT);
     M.s_ntiles = NT;
-    M.s_part = h->dalloc<double>((int64_t)P * R);
+    M.s_part = h->dalloc<T>((int64_t)P * R);
     h->sync();
     return true;
 }
 
 // Releases the device arrays of a sliced form built by build_slices.
-void free_slices(hprsocp_handle *h, const CsrView &S) {
+template <typename T>
+void free_slices(hprsocp_handle *h, const CsrT<T> &S) {
     h->dfree(S.s_base);
     h->dfree(S.s_off);
     h->dfree(S.s_col);
@@ -1086,7 +1089,14 @@ IterView<double> view64(hprsocp_handle *h) {
 
 // FP32 copy of a (scaled) operator: values only; index arrays and the slice
 // layout are shared with the FP64 operator.
-CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz) {
+// FP32 copy of an operator: the FP64 index arrays and slice layout with FP32
+// values.  A column-sorted sliced operator also gets a layout of its own with
+// FP32 chunks (SortSpec<float>: twice the entries per chunk, so the gathers of
+// one instruction fall on fewer cache lines); the copy keeps whichever of the
+// two runs one timed SpMV faster (always the own layout when the slicing is
+// forced through HPRSOCP_XCD_SLICES).  Both sum every row in row-major order,
+// so the results are the same.
+CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const HostCsr *hm, const DevPlan *plan) {
     CsrT<float> f;
     f.rowptr = M.rowptr;
     f.col = M.col;
@@ -1095,33 +1105,75 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz) {
     float *v = h->dalloc<float>(nnz);
     launch_demote(h->st, nnz, M.val, v);
     f.val = v;
+    if (M.nslices <= 1) return f;
+    CsrT<float> g = f;
     f.nslices = M.nslices;
     f.s_sorted = M.s_sorted;
     f.s_ncols = M.s_ncols;
-    if (M.nslices > 1) {
-        f.s_base = M.s_base;
-        f.s_off = M.s_off;
-        f.s_col = M.s_col;
-        f.s_tiles = M.s_tiles;
-        f.s_ntiles = M.s_ntiles;
-        float *sv = h->dalloc<float>(nnz);
-        launch_demote(h->st, nnz, M.s_val, sv);
-        f.s_val = sv;
-        f.s_part = h->dalloc<float>((int64_t)M.nslices * M.nrows);
+    f.s_base = M.s_base;
+    f.s_off = M.s_off;
+    f.s_col = M.s_col;
+    f.s_tiles = M.s_tiles;
+    f.s_ntiles = M.s_ntiles;
+    float *sv = h->dalloc<float>(nnz);
+    launch_demote(h->st, nnz, M.s_val, sv);
+    f.s_val = sv;
+    f.s_part = h->dalloc<float>((int64_t)M.nslices * M.nrows);
+    if (!M.s_sorted || !hm || !plan || !build_slices<float>(h, g, *hm, f.val, M.nslices, true)) return f;
+    bool own = true;
+    if (!getenv("HPRSOCP_XCD_SLICES")) {
+        float *xin = h->dalloc<float>(hm->ncols), *yout = h->dalloc<float>(M.nrows);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        auto time_one = [&](const CsrT<float> &V) {
+            std::vector<float> t;
+            for (int rep = 0; rep < 6; ++rep) {
+                HIPCHK(hipEventRecord(e0, h->st));
+                launch_spmv<float>(h->st, *plan, V, xin, yout);
+                HIPCHK(hipEventRecord(e1, h->st));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (rep) t.push_back(ms);
+            }
+            std::sort(t.begin(), t.end());
+            return t[t.size() / 2];
+        };
+        const float tf = time_one(f), tg = time_one(g);
+        if (h->params.verbose)
+            printf("FP32 column-sorted layout: %.4f ms with %d-entry chunks, %.4f ms with the FP64 layout\n", tg,
+                   SortSpec<float>::chunk, tf);
+        own = tg < tf;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        h->dfree(xin);
+        h->dfree(yout);
     }
-    return f;
+    if (!own) {
+        free_slices(h, g);
+        return f;
+    }
+    h->dfree(f.s_val);
+    h->dfree(f.s_part);
+    return g;
 }
 
 // Allocates the FP32 mirrors; called once the FP64 operators are final.
-void setup_fp32(hprsocp_handle *h) {
+// Host structure of the operators (for the FP32 sorted layouts).
+struct HostOps {
+    const HostCsr *A, *AT, *Q, *F, *FT;
+};
+
+void setup_fp32(hprsocp_handle *h, const HostOps &ho) {
     IterView<float> &v = h->v32;
     const int64_t n = h->n, m = h->m;
-    v.A = csr_to_f32(h, h->A, h->nnzA);
-    v.AT = csr_to_f32(h, h->AT, h->nnzA);
-    if (h->nnzQ > 0) v.Q = csr_to_f32(h, h->Q, h->nnzQ);
+    v.A = csr_to_f32(h, h->A, h->nnzA, ho.A, &h->plan_A);
+    v.AT = csr_to_f32(h, h->AT, h->nnzA, ho.AT, &h->plan_AT);
+    if (h->nnzQ > 0) v.Q = csr_to_f32(h, h->Q, h->nnzQ, ho.Q, &h->plan_Q);
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
-        v.F = csr_to_f32(h, h->F, h->nnzF);
-        v.FT = csr_to_f32(h, h->FT, h->nnzF);
+        v.F = csr_to_f32(h, h->F, h->nnzF, ho.F, &h->plan_F);
+        v.FT = csr_to_f32(h, h->FT, h->nnzF, ho.FT, &h->plan_FT);
         v.Ftmp = h->dalloc<float>(h->Frows);
         if (h->Fdiag) v.Fdiag = h->dalloc<float>(n);
     }
@@ -1569,11 +1621,12 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
             h->I(i).sigma = P.sigma;
         }
     }
+    const HostOps ho{&A, &AT, &Qh, &F, &FT};
     if (P.precision == HPRSOCP_PREC_FP32_REFINE) {
-        setup_fp32(h);
+        setup_fp32(h, ho);
         for (int i = 0; i < h->nb; ++i) h->I(i).fp32 = 1;
     } else if (P.precision == HPRSOCP_PREC_FP32_DELTA) {
-        setup_fp32(h);
+        setup_fp32(h, ho);
         h->fp32 = false;  // the instance is an FP64 one outside the delta iterations
         h->delta = true;
         h->d_ra = h->dalloc<double>(n);

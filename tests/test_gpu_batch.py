@@ -94,13 +94,17 @@ def test_batch_matches_single_solves(kind, oracle_lib, gpu_lib):
         assert 0.8 * rs.iter <= r.iter <= 1.25 * rs.iter + 150
 
 
-def test_batch_warm_start(gpu_lib):
+@pytest.mark.parametrize("precision", ["fp64", "fp32_delta"])
+def test_batch_warm_start(precision, gpu_lib):
+    """Half the instances warm-started (the C4 setting).  With fp32_delta the
+    step schedule of an instance depends on the batch's restarts, so a cold
+    instance follows its cold-batch trajectory only to rounding."""
     models = noq_models()[:4]
-    cold = batch_solve(models, stoptol=1e-6)
+    cold = batch_solve(models, stoptol=1e-6, precision=precision)
     rng = np.random.default_rng(5)
     xs = [r.x * (1 + 0.01 * rng.standard_normal(r.x.shape)) if i % 2 == 0 else None for i, r in enumerate(cold)]
     ys = [r.y * (1 + 0.01 * rng.standard_normal(r.y.shape)) if i % 2 == 0 else None for i, r in enumerate(cold)]
-    with hprsocp.BatchSolver(models, params(stoptol=1e-6)) as b:
+    with hprsocp.BatchSolver(models, params(stoptol=1e-6, precision=precision)) as b:
         b.estimate_lambda()
         b.set_initial(xs, ys)
         warm = b.solve()
@@ -109,7 +113,10 @@ def test_batch_warm_start(gpu_lib):
         assert w.status == "OPTIMAL"
         assert abs(w.primal_obj - c.primal_obj) <= 1e-5 * (1 + abs(c.primal_obj))
         if i % 2 == 1:  # cold instances of the warm batch follow the cold trajectory
-            assert w.iter == c.iter
+            if precision == "fp64":
+                assert w.iter == c.iter
+            else:
+                assert abs(w.iter - c.iter) <= 300
 
 
 def test_batch_fp32(gpu_lib):

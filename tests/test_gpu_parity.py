@@ -330,11 +330,11 @@ def _slice_model(name):
     return _extra[name]
 
 
-def _sliced_run(name, P, srt, monkeypatch, precision="fp64", solve=True):
+def _sliced_run(name, P, srt, monkeypatch, precision="fp64", solve=True, max_iter=3000):
     monkeypatch.setenv("HPRSOCP_XCD_SLICES", str(P))
     monkeypatch.setenv("HPRSOCP_XCD_SORTED", str(int(srt)))
     m = _slice_model(name)
-    with hprsocp.Solver(m, params(stoptol=1e-6, max_iter=3000, precision=precision)) as g:
+    with hprsocp.Solver(m, params(stoptol=1e-6, max_iter=max_iter, precision=precision)) as g:
         rng = np.random.default_rng(2)
         v, w = rng.standard_normal(m.A.shape[1]), rng.standard_normal(m.A.shape[0])
         ax, atw = g.spmv(v), g.spmv(w, trans=True)
@@ -348,12 +348,13 @@ def _sliced_run(name, P, srt, monkeypatch, precision="fp64", solve=True):
     return ax, atw, r
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine", "fp32_delta"])
 @pytest.mark.parametrize("name", ["c1_small", "c2_tiny", "qp_factored"])
 def test_sorted_slices_bit_identical(name, precision, monkeypatch, gpu_lib):
     """The column-sorted layout only reorders the gathers: every row still sums
     its products in row-major order, so SpMV results and (up to 3000 iterations
-    of) whole solves equal the row-major sliced form bit for bit."""
+    of) whole solves equal the row-major sliced form bit for bit.  The FP32
+    forms run the FP32 sorted layout (chunks of twice the entries)."""
     print(f"[case] sorted vs row-major slices {name} {precision}", flush=True)
     a = _sliced_run(name, 8, False, monkeypatch, precision)
     b = _sliced_run(name, 8, True, monkeypatch, precision)
@@ -373,3 +374,15 @@ def test_sorted_slices_multi_chunk_rows(monkeypatch, gpu_lib):
     b = _sliced_run("dense_rows", 3, True, monkeypatch, solve=False)
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("precision", ["fp32_refine", "fp32_delta"])
+def test_sorted_slices_multi_chunk_rows_fp32(precision, monkeypatch, gpu_lib):
+    """The same rows in the FP32 sorted layout (18 K entries per slice, more than
+    one FP32 chunk): 200 iterations equal the row-major FP32 form bit for bit."""
+    print(f"[case] FP32 sorted slices, rows longer than a chunk ({precision})", flush=True)
+    a = _sliced_run("dense_rows", 3, False, monkeypatch, precision, max_iter=200)
+    b = _sliced_run("dense_rows", 3, True, monkeypatch, precision, max_iter=200)
+    assert a[2].iter == b[2].iter
+    np.testing.assert_array_equal(a[2].x, b[2].x)
+    np.testing.assert_array_equal(a[2].y, b[2].y)
