@@ -89,11 +89,13 @@ struct CsrT {
     // stored per slice as uint16 relative to the slice tile's base:
     //   entries of (slice p, row r) start at s_base[p*(s_ntiles+1) + t] + s_off[p*nrows + r]
     // with t the slice tile holding r; the tile's end is s_base[... + t + 1].
-    // With s_sorted = 1 the entries of each chunk (SortSpec<T>) are in column
-    // order and s_col holds packed words (see SortSpec); s_ncols gives the
-    // slice starts (ncols * p / nslices).
+    // With s_sorted = 1 the entries of each chunk of 2^s_pos_bits entries are in
+    // column order and s_col holds packed words (see SortSpec; an FP32 copy may
+    // keep the FP64 layout's smaller chunks); s_ncols gives the slice starts
+    // (ncols * p / nslices).
     int32_t nslices = 1;
     int32_t s_sorted = 0;
+    int32_t s_pos_bits = 0;
     int32_t s_ncols = 0;
     const int32_t *s_base = nullptr;
     const uint16_t *s_off = nullptr;

@@ -385,14 +385,16 @@ __global__ __launch_bounds__(kSortBlock) void k_spmv_sorted(CsrT<T> M, const T *
     }
     const T *__restrict__ val = M.s_val + k0;
     const uint32_t *__restrict__ pk = (const uint32_t *)M.s_col + k0;
-    constexpr uint32_t kPosMask = (1u << Spec::pos_bits) - 1;
-    for (int c0 = 0; c0 < tnnz; c0 += Spec::chunk) {
-        const int c1 = min(c0 + Spec::chunk, tnnz);
+    // chunks of the layout (at most Spec::chunk entries, Spec::rows rows)
+    const int pbits = M.s_pos_bits, chunk = 1 << pbits;
+    const uint32_t pmask = (uint32_t)chunk - 1u;
+    for (int c0 = 0; c0 < tnnz; c0 += chunk) {
+        const int c1 = min(c0 + chunk, tnnz);
 #pragma unroll 4
         for (int k = c0 + tid; k < c1; k += kSortBlock) {
             const uint32_t w = __builtin_nontemporal_load(pk + k);
             const T v = __builtin_nontemporal_load(val + k);
-            prod[w & kPosMask] = v * xs[w >> Spec::pos_bits];
+            prod[w & pmask] = v * xs[w >> pbits];
         }
         __syncthreads();
 #pragma unroll

@@ -847,6 +847,7 @@ bool build_slices(hprsocp_handle *h, CsrT<T> &M, const HostCsr &hm, const T *dva
     }
     M.nslices = P;
     M.s_sorted = sorted ? 1 : 0;
+    M.s_pos_bits = sorted ? Spec::pos_bits : 0;
     M.s_ncols = hm.ncols;
     M.s_base = h->upload<int32_t>(base.data(), (int64_t)base.size());
     M.s_off = h->upload<uint16_t>(off.data(), (int64_t)off.size());
@@ -1093,9 +1094,14 @@ IterView<double> view64(hprsocp_handle *h) {
 // values.  A column-sorted sliced operator also gets a layout of its own with
 // FP32 chunks (SortSpec<float>: twice the entries per chunk, so the gathers of
 // one instruction fall on fewer cache lines); the copy keeps whichever of the
-// two runs one timed SpMV faster (always the own layout when the slicing is
-// forced through HPRSOCP_XCD_SLICES).  Both sum every row in row-major order,
-// so the results are the same.
+// two runs one timed SpMV faster.  HPRSOCP_FP32_OWN_LAYOUT=<0|1> forces the
+// choice (tests); with the slicing forced through HPRSOCP_XCD_SLICES the own
+// layout is kept.  Both sum every row in row-major order, so the results are
+// the same.  The float kernel reads either layout: it takes the chunk size
+// from s_pos_bits, and the FP64 layout's chunks and tiles fit its LDS buffer
+// and row registers.
+static_assert(SortSpec<double>::chunk <= SortSpec<float>::chunk && SortSpec<double>::rows <= SortSpec<float>::rows,
+              "the FP32 sorted-slice kernel must hold the FP64 layout's chunks");
 CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const HostCsr *hm, const DevPlan *plan) {
     CsrT<float> f;
     f.rowptr = M.rowptr;
@@ -1109,6 +1115,7 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
     CsrT<float> g = f;
     f.nslices = M.nslices;
     f.s_sorted = M.s_sorted;
+    f.s_pos_bits = M.s_pos_bits;
     f.s_ncols = M.s_ncols;
     f.s_base = M.s_base;
     f.s_off = M.s_off;
@@ -1120,8 +1127,9 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
     f.s_val = sv;
     f.s_part = h->dalloc<float>((int64_t)M.nslices * M.nrows);
     if (!M.s_sorted || !hm || !plan || !build_slices<float>(h, g, *hm, f.val, M.nslices, true)) return f;
-    bool own = true;
-    if (!getenv("HPRSOCP_XCD_SLICES")) {
+    const char *force = getenv("HPRSOCP_FP32_OWN_LAYOUT");
+    bool own = !force || atoi(force) != 0;
+    if (!force && !getenv("HPRSOCP_XCD_SLICES")) {
         float *xin = h->dalloc<float>(hm->ncols), *yout = h->dalloc<float>(M.nrows);
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));

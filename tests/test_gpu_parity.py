@@ -360,7 +360,27 @@ def test_sorted_slices_bit_identical(name, precision, monkeypatch, gpu_lib):
     b = _sliced_run(name, 8, True, monkeypatch, precision)
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
-    print(f"[result] {name} {precision}: iter {a[2].iter} / {b[2].iter} status {a[2].status}", flush=True)
+    print(f"[result] {name} {precision}: iter {a[2].iter} / {b[2].iter} status {a[2].status} "
+          f"kkt {a[2].residuals:.2e}", flush=True)
+    assert np.isfinite(a[2].residuals) and np.all(np.isfinite(a[2].x)) and np.all(np.isfinite(a[2].y))
+    assert a[2].iter == b[2].iter and a[2].primal_obj == b[2].primal_obj
+    np.testing.assert_array_equal(a[2].x, b[2].x)
+    np.testing.assert_array_equal(a[2].y, b[2].y)
+
+
+@pytest.mark.parametrize("precision", ["fp32_refine", "fp32_delta"])
+@pytest.mark.parametrize("name", ["c1_small", "qp_factored"])
+def test_fp32_sorted_layouts_agree(name, precision, monkeypatch, gpu_lib):
+    """An FP32 operator runs either its own sorted layout (16 K-entry chunks) or
+    the FP64 layout (8 K) under the same float kernel, which takes the chunk
+    size from the layout: both give the same iterates, bit for bit, and finite."""
+    monkeypatch.setenv("HPRSOCP_FP32_OWN_LAYOUT", "0")
+    a = _sliced_run(name, 8, True, monkeypatch, precision)
+    monkeypatch.setenv("HPRSOCP_FP32_OWN_LAYOUT", "1")
+    b = _sliced_run(name, 8, True, monkeypatch, precision)
+    print(f"[result] {name} {precision}: iter {a[2].iter} / {b[2].iter} kkt {a[2].residuals:.2e} / "
+          f"{b[2].residuals:.2e}", flush=True)
+    assert np.isfinite(a[2].residuals) and np.all(np.isfinite(a[2].x)) and np.all(np.isfinite(a[2].y))
     assert a[2].iter == b[2].iter and a[2].primal_obj == b[2].primal_obj
     np.testing.assert_array_equal(a[2].x, b[2].x)
     np.testing.assert_array_equal(a[2].y, b[2].y)
@@ -383,6 +403,7 @@ def test_sorted_slices_multi_chunk_rows_fp32(precision, monkeypatch, gpu_lib):
     print(f"[case] FP32 sorted slices, rows longer than a chunk ({precision})", flush=True)
     a = _sliced_run("dense_rows", 3, False, monkeypatch, precision, max_iter=200)
     b = _sliced_run("dense_rows", 3, True, monkeypatch, precision, max_iter=200)
+    assert np.all(np.isfinite(a[2].x)) and np.all(np.isfinite(a[2].y))
     assert a[2].iter == b[2].iter
     np.testing.assert_array_equal(a[2].x, b[2].x)
     np.testing.assert_array_equal(a[2].y, b[2].y)
