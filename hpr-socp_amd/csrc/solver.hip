@@ -1091,10 +1091,10 @@ IterView<double> view64(hprsocp_handle *h) {
 // FP32 copy of a (scaled) operator: values only; index arrays and the slice
 // layout are shared with the FP64 operator.
 // FP32 copy of an operator: the FP64 index arrays and slice layout with FP32
-// values.  A column-sorted sliced operator also gets a layout of its own with
-// FP32 chunks (SortSpec<float>: twice the entries per chunk, so the gathers of
-// one instruction fall on fewer cache lines); the copy keeps whichever of the
-// two runs one timed SpMV faster.  HPRSOCP_FP32_OWN_LAYOUT=<0|1> forces the
+// values.  A sliced operator also gets a column-sorted layout of its own with
+// the same slices and FP32 chunks (SortSpec<float>: twice the entries per
+// chunk, so the gathers of one instruction fall on fewer cache lines); the
+// copy keeps whichever of the two runs one timed SpMV faster.  HPRSOCP_FP32_OWN_LAYOUT=<0|1> forces the
 // choice (tests); with the slicing forced through HPRSOCP_XCD_SLICES the own
 // layout is kept.  Both sum every row in row-major order, so the results are
 // the same.  The float kernel reads either layout: it takes the chunk size
@@ -1126,7 +1126,7 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
     launch_demote(h->st, nnz, M.s_val, sv);
     f.s_val = sv;
     f.s_part = h->dalloc<float>((int64_t)M.nslices * M.nrows);
-    if (!M.s_sorted || !hm || !plan || !build_slices<float>(h, g, *hm, f.val, M.nslices, true)) return f;
+    if (!hm || !plan || !build_slices<float>(h, g, *hm, f.val, M.nslices, true)) return f;
     const char *force = getenv("HPRSOCP_FP32_OWN_LAYOUT");
     bool own = !force || atoi(force) != 0;
     if (!force && !getenv("HPRSOCP_XCD_SLICES")) {

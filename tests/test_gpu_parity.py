@@ -354,9 +354,12 @@ def test_sorted_slices_bit_identical(name, precision, monkeypatch, gpu_lib):
     """The column-sorted layout only reorders the gathers: every row still sums
     its products in row-major order, so SpMV results and (up to 3000 iterations
     of) whole solves equal the row-major sliced form bit for bit.  The FP32
-    forms run the FP32 sorted layout (chunks of twice the entries)."""
+    forms compare the FP32 row-major copy with the FP32 sorted layout (chunks of
+    twice the entries)."""
     print(f"[case] sorted vs row-major slices {name} {precision}", flush=True)
+    monkeypatch.setenv("HPRSOCP_FP32_OWN_LAYOUT", "0")
     a = _sliced_run(name, 8, False, monkeypatch, precision)
+    monkeypatch.setenv("HPRSOCP_FP32_OWN_LAYOUT", "1")
     b = _sliced_run(name, 8, True, monkeypatch, precision)
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
@@ -401,7 +404,9 @@ def test_sorted_slices_multi_chunk_rows_fp32(precision, monkeypatch, gpu_lib):
     """The same rows in the FP32 sorted layout (18 K entries per slice, more than
     one FP32 chunk): 200 iterations equal the row-major FP32 form bit for bit."""
     print(f"[case] FP32 sorted slices, rows longer than a chunk ({precision})", flush=True)
+    monkeypatch.setenv("HPRSOCP_FP32_OWN_LAYOUT", "0")
     a = _sliced_run("dense_rows", 3, False, monkeypatch, precision, max_iter=200)
+    monkeypatch.setenv("HPRSOCP_FP32_OWN_LAYOUT", "1")
     b = _sliced_run("dense_rows", 3, True, monkeypatch, precision, max_iter=200)
     assert np.all(np.isfinite(a[2].x)) and np.all(np.isfinite(a[2].y))
     assert a[2].iter == b[2].iter
