@@ -244,7 +244,10 @@ int hprsocp_set_initial(hprsocp_handle *h, const double *x0, const double *y0);
 /* ------------------------------------------------------------ hot path */
 /* One HPR step, main_update_gpu! (algorithm.jl:2284-2309) with
  * Halpern_fact1 = 1/(inner+2); to_check selects full (1) or partial (0)
- * outputs.  No host synchronisation. */
+ * outputs.  The per-instance scalars are published synchronously, the step
+ * is queued on the handle's stream.  With an FP32 precision the FP64 state
+ * holds the step's result afterwards (FP32_DELTA: partial steps run on the
+ * FP32 differences, full steps in FP64). */
 int hprsocp_update(hprsocp_handle *h, int64_t inner, int32_t to_check);
 /* compute_residuals! (algorithm.jl:113-196) at iteration `iter`. */
 int hprsocp_compute_residuals(hprsocp_handle *h, int64_t iter, hprsocp_residuals *out);
