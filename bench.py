@@ -226,10 +226,12 @@ def bench_single(args):
         log(f"[{prec}] setup (transfer + scaling + plans) {time.time() - t0:.2f}s")
         sc = g.estimate_lambda()
         log(f"[{prec}] lambda_A={sc['lambda_max_A']:.6e} lambda_Q={sc['lambda_max_Q']:.6e} sigma={sc['sigma']:.4e}")
-        if args.probe_copy_mb > 0 and prec == precs[0]:
-            pms = g.probe_copy(args.probe_copy_mb << 20)
-            log(f"copy probe: {args.probe_copy_mb} MiB in {pms:.4f} ms "
-                f"({2 * (args.probe_copy_mb << 20) / (pms * 1e-3) / 1e9:.0f} GB/s read+write)")
+        if prec == precs[0]:
+            # a plain device copy: the streaming bandwidth the kernels can reach on this box
+            cmb = args.probe_copy_mb or 512
+            pms = g.probe_copy(cmb << 20)
+            copy_gbs = 2 * (cmb << 20) / (pms * 1e-3) / 1e9
+            log(f"copy probe: {cmb} MiB in {pms:.4f} ms ({copy_gbs:.0f} GB/s read+write)")
         g.time_iterations(args.warmup)
         ms, _ = g.time_iterations(args.steps)
         _, kern = g.time_iterations(max(args.steps // 4, 10), per_kernel=True)
@@ -253,6 +255,10 @@ def bench_single(args):
     vbytes = 8 if precs[0] == "fp64" else 4
     bytes_by, order = algorithmic_bytes(model, vbytes)
     groups = [(name, base["kern"][i]) for i, name in enumerate(order)]
+    group_roof = {name: {"ms": round(t, 5), "algorithmic_MB": round(bytes_by[name] / 1e6, 2),
+                         "GBps": round(bytes_by[name] / (t * 1e-3) / 1e9, 1) if t > 0 else None,
+                         "frac": round(bytes_by[name] / (t * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) if t > 0 else None}
+                  for name, t in groups}
     dom, dom_ms = max(groups, key=lambda t: t[1])
     dom_bytes = bytes_by[dom]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
@@ -307,7 +313,8 @@ def bench_single(args):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": int(dom_bytes), "ms": round(dom_ms, 5),
-                     "gather": base["gather"]},
+                     "gather": base["gather"], "groups": group_roof,
+                     "copy_GBps": round(copy_gbs, 1)},
         "kernel_ms": {name: round(t, 5) for name, t in groups},
         "forms": forms,
         "cpu_baseline": {"value": cpu_rate and round(cpu_rate, 4), "unit": "HPR iters/s", "cores": 1,
