@@ -74,7 +74,8 @@ def test_autosave_reference_checks(tmp_path, gpu_lib):
     assert "parameters/initial_x" not in d
 
 
-@pytest.mark.parametrize("name,precision", [("qp_csr", "fp64"), ("c0", "fp64"), ("qp_csr", "fp32_refine")])
+@pytest.mark.parametrize("name,precision", [("qp_csr", "fp64"), ("c0", "fp64"), ("qp_csr", "fp32_refine"),
+                                            ("qp_csr", "fp32_delta"), ("c0", "fp32_delta")])
 def test_autosave_last_checkpoint_is_the_result(name, precision, tmp_path, gpu_lib):
     m = model(name)
     path = tmp_path / "state.npz"

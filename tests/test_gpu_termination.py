@@ -22,10 +22,11 @@ def solve(m, **kw):
         return g.solve()
 
 
-def test_time_limit(gpu_lib):
+@pytest.mark.parametrize("precision", ["fp64", "fp32_delta"])
+def test_time_limit(precision, gpu_lib):
     m = model("c1_tiny")
-    print("[case] time limit 0.01 s on c1_tiny", flush=True)
-    r = solve(m, stoptol=1e-14, time_limit=0.01, max_iter=10**7)
+    print(f"[case] time limit 0.01 s on c1_tiny ({precision})", flush=True)
+    r = solve(m, stoptol=1e-14, time_limit=0.01, max_iter=10**7, precision=precision)
     print(f"[result] status {r.status} after {r.iter} iterations, {r.time:.3f} s", flush=True)
     assert r.status == "TIME_LIMIT"
     assert r.iter > 0 and np.isfinite(r.residuals)
