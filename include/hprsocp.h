@@ -301,10 +301,11 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
  * of known traffic.  (No counterpart in the reference; measurement hook.) */
 int hprsocp_probe_copy(hprsocp_handle *h, int64_t bytes, double *ms);
 
-/* Median HIP-event time (ms) of `reps` launches of the SpMV A x (trans = 0)
- * or A^T y (trans = 1) exactly as the iteration runs it (sliced or plain form
- * chosen by the autotuner), on device-resident operands.  Single-problem
- * handles.  (No counterpart in the reference; measurement hook.) */
+/* Median HIP-event time (ms) of `reps` launches of the FP64 SpMV A x
+ * (trans = 0) or A^T y (trans = 1) exactly as the FP64 iteration runs it
+ * (sliced or plain form chosen by the autotuner), on device-resident
+ * operands.  Single-problem handles.  (No counterpart in the reference;
+ * measurement hook.) */
 int hprsocp_time_spmv(hprsocp_handle *h, int32_t trans, int32_t reps, double *ms);
 
 /* Gather ceiling of the sliced SpMV: times a synthetic kernel that streams
