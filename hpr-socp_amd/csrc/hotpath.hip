@@ -41,27 +41,60 @@ __device__ void tile_spmv_stream(const CsrT<T> &M, const T *__restrict__ v, cons
     }
 }
 
+// 16-byte vectors of T with only T's alignment (rows start anywhere; global
+// loads on CDNA need element alignment only).
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4), aligned(4)));
+    static constexpr int W = 4;
+};
+template <>
+struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2), aligned(8)));
+    static constexpr int W = 2;
+};
+
 // Lane-group SpMV: g lanes per row (1..64), or the whole workgroup per row
 // (256).  Each lane keeps four independent partial sums so that four gathers
 // are in flight per lane (long rows are otherwise bound by the latency of one
 // dependent load per step); the partials are combined in a fixed order.  Rows
-// with consecutive columns (M.c0) read the input without column indices, in
-// the same order, so the sums are the same.
+// with consecutive columns (M.c0) read the matrix and the input without
+// column indices, in 16-byte vectors: lane l takes vectors l, l + G, ... of
+// the row, two in flight, and the row's last n mod W entries go to the first
+// lanes; the order is fixed for a given g.
 template <typename T>
 __device__ __forceinline__ T row_dot(const CsrT<T> &M, const T *__restrict__ v, int row, int s, int e, int lane,
                                      int G) {
     T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
     int k = s + lane;
     if (M.c0) {
-        const int off = M.c0[row] - s;  // column of entry k is off + k
-        for (; k + 3 * G < e; k += 4 * G) {
-            const T w0 = M.val[k], w1 = M.val[k + G], w2 = M.val[k + 2 * G], w3 = M.val[k + 3 * G];
-            a0 += w0 * v[off + k];
-            a1 += w1 * v[off + k + G];
-            a2 += w2 * v[off + k + 2 * G];
-            a3 += w3 * v[off + k + 3 * G];
+        using VT = typename Vec16<T>::type;
+        constexpr int W = Vec16<T>::W;
+        const VT *__restrict__ pw = reinterpret_cast<const VT *>(M.val + s);
+        const VT *__restrict__ px = reinterpret_cast<const VT *>(v + M.c0[row]);
+        const int nv = (e - s) / W;
+        int q = lane;
+        for (; q + G < nv; q += 2 * G) {
+            const VT w0 = pw[q], x0 = px[q], w1 = pw[q + G], x1 = px[q + G];
+#pragma unroll
+            for (int c = 0; c < W; c += 2) {
+                a0 += w0[c] * x0[c];
+                a1 += w0[c + 1] * x0[c + 1];
+                a2 += w1[c] * x1[c];
+                a3 += w1[c + 1] * x1[c + 1];
+            }
         }
-        for (; k < e; k += G) a0 += M.val[k] * v[off + k];
+        if (q < nv) {
+            const VT w0 = pw[q], x0 = px[q];
+#pragma unroll
+            for (int c = 0; c < W; c += 2) {
+                a0 += w0[c] * x0[c];
+                a1 += w0[c + 1] * x0[c + 1];
+            }
+        }
+        for (int t = s + nv * W + lane; t < e; t += G) a0 += M.val[t] * v[M.c0[row] + (t - s)];  // < W left
         return (a0 + a1) + (a2 + a3);
     }
     for (; k + 3 * G < e; k += 4 * G) {
