@@ -1167,12 +1167,12 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
     return g;
 }
 
-// Allocates the FP32 mirrors; called once the FP64 operators are final.
 // Host structure of the operators (for the FP32 sorted layouts).
 struct HostOps {
     const HostCsr *A, *AT, *Q, *F, *FT;
 };
 
+// Allocates the FP32 mirrors; called once the FP64 operators are final.
 void setup_fp32(hprsocp_handle *h, const HostOps &ho) {
     IterView<float> &v = h->v32;
     const int64_t n = h->n, m = h->m;
