@@ -62,7 +62,7 @@ struct Vec16<double> {
 // dependent load per step); the partials are combined in a fixed order.  Rows
 // with consecutive columns (M.c0) read the matrix and the input without
 // column indices, in 16-byte vectors: lane l takes vectors l, l + G, ... of
-// the row, two in flight, and the row's last n mod W entries go to the first
+// the row, four in flight, and the row's last n mod W entries go to the first
 // lanes; the order is fixed for a given g.
 template <typename T>
 __device__ __forceinline__ T row_dot(const CsrT<T> &M, const T *__restrict__ v, int row, int s, int e, int lane,
@@ -76,17 +76,22 @@ __device__ __forceinline__ T row_dot(const CsrT<T> &M, const T *__restrict__ v, 
         const VT *__restrict__ px = reinterpret_cast<const VT *>(v + M.c0[row]);
         const int nv = (e - s) / W;
         int q = lane;
-        for (; q + G < nv; q += 2 * G) {
+        for (; q + 3 * G < nv; q += 4 * G) {  // four vector pairs in flight
             const VT w0 = pw[q], x0 = px[q], w1 = pw[q + G], x1 = px[q + G];
+            const VT w2 = pw[q + 2 * G], x2 = px[q + 2 * G], w3 = pw[q + 3 * G], x3 = px[q + 3 * G];
 #pragma unroll
             for (int c = 0; c < W; c += 2) {
                 a0 += w0[c] * x0[c];
                 a1 += w0[c + 1] * x0[c + 1];
                 a2 += w1[c] * x1[c];
                 a3 += w1[c + 1] * x1[c + 1];
+                a0 += w2[c] * x2[c];
+                a1 += w2[c + 1] * x2[c + 1];
+                a2 += w3[c] * x3[c];
+                a3 += w3[c + 1] * x3[c + 1];
             }
         }
-        if (q < nv) {
+        for (; q < nv; q += G) {
             const VT w0 = pw[q], x0 = px[q];
 #pragma unroll
             for (int c = 0; c < W; c += 2) {
