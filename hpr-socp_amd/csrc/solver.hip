@@ -283,7 +283,8 @@ struct hprsocp_handle {
     // hold differences from last_x, last_y, last_w, last_aty; ra and rya hold
     // the anchors' images (see DPrimalArgs)
     bool delta = false;
-    bool delta_live = false;  // the deltas hold the current state (the FP64 x, y, w, aty are stale)
+    bool delta_live = false;    // the deltas hold the current state (the FP64 x, y, w, aty are stale)
+    bool images_valid = false;  // ra, rya match the anchors, sigma and lambda_A
     double *d_ra = nullptr, *d_rya = nullptr, *d_img_n = nullptr, *d_img_m = nullptr;
     // timing (profiled iterations)
     bool profile = false;
@@ -1193,6 +1194,7 @@ void setup_fp32(hprsocp_handle *h, const HostOps &ho) {
 #undef HPR_ALLOC_M
     h->fp32 = true;
     h->fp32_dirty = true;
+    h->images_valid = false;
 }
 
 // FP64 state and data -> FP32 mirrors (round to nearest).
@@ -1248,6 +1250,10 @@ void delta_split(hprsocp_handle *h) {
         launch_delta_split(st, sn, h->aty, h->last_aty, V.aty, h->isc);
         if (h->Q_is_diag) launch_demote(st, n, h->fact, V.fact);
         if (V.Fdiag) launch_demote(st, n, h->Fdiag, V.Fdiag);
+    }
+    h->fp32_dirty = false;
+    if (h->images_valid) return;  // no restart, sigma or data change since they were formed
+    if (h->has_quad) {
         qmap(h, h->last_w, h->d_img_n);  // Q last_w
         launch_delta_ra(st, sn, h->last_x, h->last_aty, h->d_img_n, h->c, h->isc, h->d_ra);
     } else {
@@ -1258,7 +1264,7 @@ void delta_split(hprsocp_handle *h) {
         launch_spmv(st, h->plan_A, h->A, h->last_x, h->d_img_m);  // A last_x
         launch_delta_rya(st, sm, h->m1, h->d_img_m, h->rhs, h->last_y, h->isc, h->d_rya);
     }
-    h->fp32_dirty = false;
+    h->images_valid = true;
 }
 
 // Delta form: deltas -> FP64 state (x = last_x + dx, ...), before anything
@@ -1682,6 +1688,7 @@ int hprsocp_estimate_lambda(hprsocp_handle *h) {
     ABI_TRY(h, {
         delta_join(h);
         h->fp32_dirty = true;
+        h->images_valid = false;
         const double t0 = now_s();
         const int nb = h->nb;
         std::vector<double> la(nb, 0.0), lq(nb, 0.0);
@@ -1759,6 +1766,7 @@ void set_initial_impl(hprsocp_handle *h, const double *x0, const double *y0, con
         }
     }
     h->fp32_dirty = true;
+    h->images_valid = false;
     h->sync();
 }
 }  // namespace
@@ -2781,6 +2789,7 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
     for (int i = 0; i < nb; ++i) h->I(i).fp32 = h->fp32 ? 1 : 0;
     upload_isc(h);
     h->fp32_dirty = true;
+    h->images_valid = false;
     std::vector<double> best32(nb, INFINITY);
     std::vector<int> stall32(nb, 0);
     std::vector<int64_t> restarts32(nb, 0);
@@ -2848,9 +2857,14 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
         for (int i = 0; i < nb; ++i)
             if (active[i]) check_restart(R[i], iter, check_iter, h->I(i).sigma, rs[i].kkt);
             else R[i].restart_flag = 0;
+        std::vector<double> sig0(nb);
+        for (int i = 0; i < nb; ++i) sig0[i] = h->I(i).sigma;
         update_sigma(h, R, rs, active);
         const bool restarted = do_restart(h, R, active);
         if (restarted) h->fp32_dirty = true;
+        bool sig_changed = false;
+        for (int i = 0; i < nb; ++i) sig_changed = sig_changed || h->I(i).sigma != sig0[i];
+        if (restarted || sig_changed) h->images_valid = false;  // delta form: new anchors or sigma
         if (p.verbose && single && (print_yes || status[0] != HPRSOCP_STATUS_CONTINUE))
             printf("%5lld    %3.2e    %3.2e    %+7.6e    %+7.6e    %3.2e    %3.2e    %6.2f\n", (long long)iter,
                    rs[0].err_Rp, rs[0].err_Rd, rs[0].primal_obj, rs[0].dual_obj, rs[0].rel_gap, h->I(0).sigma,
@@ -3111,6 +3125,7 @@ int hprsocp_set_vector(hprsocp_handle *h, int32_t id, const double *in) {
         delta_join(h);
         if (len > 0) HIPCHK(hipMemcpyAsync(p, in, sizeof(double) * len, hipMemcpyHostToDevice, h->st));
         h->fp32_dirty = true;
+        h->images_valid = false;
         h->sync();
     });
     return 0;
@@ -3143,6 +3158,7 @@ int hprsocp_set_scalars(hprsocp_handle *h, double sigma, double la, double lq) {
         h->lambda_set = true;
         if (h->Q_is_diag) update_q_factors(h);
         h->fp32_dirty = true;
+        h->images_valid = false;
         h->sync();
     });
     return 0;
