@@ -2030,13 +2030,17 @@ static void residuals_impl(hprsocp_handle *h, int64_t iter, hprsocp_residuals *r
     delta_join(h);
     const int32_t m = h->m, n = h->n;
     const bool skip_q = !h->has_quad;
+    // batches after retirements: the SpMVs over the running instances' tiles (the
+    // compacted FP64 lists hold every running instance unless FP32-state
+    // instances are split off into their own lists)
+    const bool act = h->use_act && (!h->fp32 || h->delta);
     if (!skip_q) qmap(h, h->x_bar, h->qx);
-    if (m > 0) launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->atdy);
+    if (m > 0) launch_spmv(st, act ? h->act_AT : h->plan_AT, h->AT, h->y_bar, h->atdy);
     else launch_fill(st, n, 0.0, h->atdy);
     ResidNArgs an{h->c, h->x_bar, h->z_bar, h->col_norm, skip_q ? nullptr : h->qx, h->atdy, h->rd, h->isc};
     launch_resid_n(st, h->seg_n, an, region(h, 0), slot(h, SL_RN), SL_COUNT);
     if (m > 0) {
-        launch_spmv(st, h->plan_A, h->A, h->x_bar, h->ax);
+        launch_spmv(st, act ? h->act_A : h->plan_A, h->A, h->x_bar, h->ax);
         launch_ax_to_org(st, h->seg_m, h->ax, h->row_norm, h->isc);
         launch_cone_rp_stats(st, h->ncon, h->con_idx, h->cinst_con, h->ax, h->rhs, h->row_norm, h->isc, h->cone_t,
                              h->cone_ns);
