@@ -39,7 +39,7 @@ def oracle_lib():
 
 @pytest.fixture(scope="session")
 def gpu_lib():
-    """The HIP backend; builds it in-tree when absent (hipcc is on the GPU box)."""
+    """The HIP backend; builds it in-tree when absent (hipcc ships with ROCm)."""
     path = os.path.join(PKG, "hprsocp", "libhprsocp.so")
     if not os.path.exists(path):
         subprocess.run(["make", "-C", os.path.join(PKG, "csrc"), "-j8"], check=True,

@@ -1,10 +1,11 @@
-"""The Julia binding written out in INTEGRATION.md passes hprsocp_problem,
-hprsocp_params and hprsocp_result to ccall as isbits structs that repeat the
-C layout field by field (Julia lays out isbits structs with C alignment
-rules).  No Julia toolchain exists here, so the shim itself cannot run; this
-test parses its struct definitions and checks them against the C layout: the
-same fields in the same order at the same offsets, and the same size, as the
-ctypes mirror (whose sizes test_abi checks against the compiled library)."""
+"""The Julia binding written out in the repository's integration notes (module
+HPRSOCPHip) passes hprsocp_problem, hprsocp_params and hprsocp_result to ccall
+as isbits structs that repeat the C layout field by field (Julia lays out
+isbits structs with C alignment rules).  No Julia toolchain exists here, so the
+shim itself cannot run; this test parses its struct definitions and checks
+them against the C layout: the same fields in the same order at the same
+offsets, and the same size, as the ctypes mirror (whose sizes test_abi checks
+against the compiled library)."""
 import ctypes as C
 import os
 import re
@@ -69,7 +70,7 @@ def test_julia_structs_match_the_c_layout(jname, cls):
 
 
 def test_parser_sees_a_changed_field(tmp_path, monkeypatch):
-    """The check fails when the doc's struct drifts: a copy of the doc without
+    """The check fails when the notes' struct drifts: a copy without
     CParams.save_filename no longer matches the C layout."""
     text = open(DOC).read()
     assert "auto_save::Cint; save_filename::Cstring;" in text
