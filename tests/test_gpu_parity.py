@@ -236,10 +236,14 @@ def test_gpu_kat_residuals(gpu_lib):
 @pytest.mark.parametrize("name", ["solve_soc_variables_only", "solve_pure_soc_constraints",
                                   "solve_mixed_linear_soc", "clarabel_var_cone",
                                   "clarabel_con_cone", "jump_soc", "trust_region", "model_cbf"])
-def test_gpu_kat_solves(name, gpu_lib):
+@pytest.mark.parametrize("precision", ["fp64", "fp32_delta"])
+def test_gpu_kat_solves(name, precision, gpu_lib):
+    """The reference's small end-to-end solves (and the closed forms they pin),
+    in FP64 and in the FP32 delta form, which must meet the same checks."""
     s = K[name]
     tol = s.get("stoptol", 1e-8)
-    r = hprsocp.optimize(model_from(s), params(stoptol=tol, max_iter=s.get("max_iter", 100000)))
+    r = hprsocp.optimize(model_from(s), params(stoptol=tol, max_iter=s.get("max_iter", 100000),
+                                               precision=precision))
     assert r.status == "OPTIMAL"
     if "expect_x" in s:
         np.testing.assert_allclose(r.x, s["expect_x"], atol=s.get("atol", 1e-5))
