@@ -200,3 +200,26 @@ def test_portfolio_contiguous_rows(oracle_lib, gpu_lib):
         assert r.status == rs.status == ro.status == "OPTIMAL"
         assert r.iter == rs.iter and r.primal_obj == rs.primal_obj
         assert abs(r.primal_obj - ro.primal_obj) <= 1e-6 * (1 + abs(ro.primal_obj))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine", "fp32_delta"])
+def test_batch_general_q(precision, oracle_lib, gpu_lib):
+    """Batches of QPs with a general (non-diagonal) CSR Q, whose Q applies run as
+    tile SpMVs over the block-diagonal Q: every instance is optimal, its
+    objective matches the oracle's single solve, and retirements (uneven sizes)
+    leave the others running."""
+    models = [synthetic.make_small_qp(seed=60 + k, n1=30 + 20 * k, var_sizes=(3, 4 + k), m1=8 + 4 * k,
+                                      con_sizes=(3, 5), density=0.25, q="csr") for k in range(4)]
+    print(f"[case] general-Q batch ({precision})", flush=True)
+    rb = batch_solve(models, stoptol=1e-8, precision=precision)
+    for i, (m, r) in enumerate(zip(models, rb)):
+        with hprsocp.Solver(m, params(stoptol=1e-9, max_iter=200000), lib=oracle_lib, prefix="oracle_") as o:
+            o.estimate_lambda()
+            ro = o.solve()
+        rel = abs(r.primal_obj - ro.primal_obj) / (1 + abs(ro.primal_obj))
+        cert = certificate(m, r.x, r.y, r.z)
+        print(f"[result] general-Q[{i}] {precision}: iter {r.iter} obj rel err {rel:.2e} "
+              f"cert Rp {cert['err_Rp']:.1e} Rd {cert['err_Rd']:.1e}", flush=True)
+        assert r.status == "OPTIMAL" and ro.status == "OPTIMAL"
+        assert rel <= 1e-7
+        assert cert["err_Rp"] < 1e-7 and cert["err_Rd"] < 1e-7
