@@ -476,7 +476,29 @@ def main():
         except Exception as e:  # the single-problem line stands on its own
             log(f"batch measurement failed: {e!r}")
             out["batch_c3"] = {"error": repr(e)}
+    out["summary"] = headline_summary(out)
     print(json.dumps(out), flush=True)
+
+
+def headline_summary(out):
+    """The headline comparisons in one place: FP64 vs the best FP32 form in time
+    to the single-problem tolerance, and the best batched form vs sequential."""
+    s = {"fp64_iters_per_s": out.get("value")}
+    forms = out.get("forms", {})
+    t64 = ((forms.get("fp64") or {}).get("solve_to_1e-6") or {}).get("time_to_1e-6_s")
+    best = None
+    for prec in ("fp32_refine", "fp32_delta"):
+        t = ((forms.get(prec) or {}).get("solve_to_1e-6") or {}).get("time_to_1e-6_s")
+        if t and (best is None or t < best[1]):
+            best = (prec, t)
+    if t64 and best:
+        s.update({"fp64_time_to_1e-6_s": t64, "best_fp32_form": best[0], "best_fp32_time_to_1e-6_s": best[1],
+                  "fp32_vs_fp64_time_to_1e-6": round(t64 / best[1], 3)})
+    b = out.get("batch_c3") or {}
+    if "batch_vs_sequential" in b:
+        s["c3_best_batch_solves_per_s"] = max(b[k]["solves_per_s"] for k in BATCH_PRECS if k in b)
+        s["c3_batch_vs_sequential"] = b["batch_vs_sequential"]
+    return s
 
 
 if __name__ == "__main__":
