@@ -434,7 +434,8 @@ def main():
     ap.add_argument("--config", default="c1")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--solve-time", type=float, default=90.0, help="time cap of each solve")
-    ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--cpu-iters", type=int, default=100,
+                    help="C1 iterations of the 1-thread CPU oracle sample (about 12 s; also run on all cores)")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched C3 measurement")
     ap.add_argument("--no-graph", action="store_true", help="launch every update eagerly (no HIP graphs)")
