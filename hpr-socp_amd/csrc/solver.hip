@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <map>
 #include <thread>
@@ -87,11 +88,62 @@ void start_vector(int64_t len, double *out) {
     }
 }
 
+// std::allocator that leaves trivially constructible elements uninitialized on
+// resize, so large host arrays are first touched (and paged in) by the threads
+// that fill them rather than by one zeroing pass.
+template <class T>
+struct RawAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = RawAlloc<U>;
+    };
+    RawAlloc() = default;
+    template <class U>
+    RawAlloc(const RawAlloc<U> &) {}
+    template <class U>
+    void construct(U *q) noexcept {
+        ::new ((void *)q) U;
+    }
+    template <class U, class... Args>
+    void construct(U *q, Args &&...a) {
+        ::new ((void *)q) U(std::forward<Args>(a)...);
+    }
+};
+template <class T>
+using RawVec = std::vector<T, RawAlloc<T>>;
+
 struct HostCsr {
     int32_t nrows = 0, ncols = 0;
-    std::vector<int32_t> rp, ci;
-    std::vector<double> v;
+    std::vector<int32_t> rp;
+    RawVec<int32_t> ci;
+    RawVec<double> v;
 };
+
+// fn(i) for every i in [0, count), spread over up to 16 host threads
+template <class F>
+void host_parallel(int64_t count, F fn) {
+    const int nth = (int)std::max<int64_t>(
+        1, std::min<int64_t>(std::min<int64_t>(count, 16), (int64_t)std::thread::hardware_concurrency()));
+    std::atomic<int64_t> next{0};
+    auto work = [&] {
+        for (int64_t i; (i = next.fetch_add(1)) < count;) fn(i);
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nth; ++w) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+}
+
+// dst = src[0, n), copied in 1 M-element pieces on several threads
+template <class T>
+void par_assign(RawVec<T> &dst, const T *src, int64_t n) {
+    dst.resize((size_t)n);
+    const int64_t piece = 1 << 20;
+    host_parallel((n + piece - 1) / piece, [&](int64_t b) {
+        const int64_t lo = b * piece, hi = std::min(n, lo + piece);
+        memcpy(dst.data() + lo, src + lo, (size_t)(hi - lo) * sizeof(T));
+    });
+}
 
 // First column of every row of `a` on the device when each row's columns are
 // consecutive (CsrT::c0), null otherwise.
@@ -102,6 +154,10 @@ int32_t *contig_c0(hprsocp_handle *h, const HostCsr &a);
 // entries and scans the input rows in order, so every output row comes out
 // sorted by column, exactly as in the sequential fill.
 HostCsr transpose(const HostCsr &a) {
+    // Row ranges of about equal nnz go to threads; each counts its columns, the
+    // counts turn into per-(thread, column) start positions, and each thread
+    // scatters its rows in order, so every column lists its rows ascending.
+    // Threads are limited so the count table stays within nnz entries.
     HostCsr t;
     t.nrows = a.ncols;
     t.ncols = a.nrows;
@@ -109,38 +165,56 @@ HostCsr transpose(const HostCsr &a) {
     const int64_t nnz = (int64_t)a.ci.size();
     t.ci.resize((size_t)nnz);
     t.v.resize((size_t)nnz);
-    for (int32_t c : a.ci) t.rp[c + 1]++;
-    for (int32_t j = 0; j < a.ncols; ++j) t.rp[j + 1] += t.rp[j];
-    const int nth =
-        nnz < (1 << 16) ? 1 : (int)std::max<unsigned>(1, std::min<unsigned>(16, std::thread::hardware_concurrency()));
-    std::vector<int32_t> cut((size_t)nth + 1, a.ncols);
+    const int64_t nc = a.ncols;
+    const int nth = nnz < (1 << 16) ? 1
+                                    : (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(16, nnz / std::max<int64_t>(nc, 1)),
+                                                                                  std::thread::hardware_concurrency()));
+    std::vector<int32_t> cut((size_t)nth + 1, a.nrows);
     cut[0] = 0;
-    for (int w = 1; w < nth; ++w)  // first column whose entries start at or after w/nth of nnz
-        cut[w] = (int32_t)(std::lower_bound(t.rp.begin(), t.rp.end() - 1, (int32_t)(nnz * w / nth)) - t.rp.begin());
-    auto fill = [&](int w) {
-        const int32_t lo = cut[w], hi = std::max(cut[w], cut[w + 1]);
-        if (lo >= hi) return;
-        std::vector<int32_t> next(t.rp.begin() + lo, t.rp.begin() + hi);
-        for (int32_t i = 0; i < a.nrows; ++i)
+    for (int w = 1; w < nth; ++w)  // first row whose entries start at or after w/nth of nnz
+        cut[w] = (int32_t)(std::lower_bound(a.rp.begin(), a.rp.end() - 1, (int32_t)(nnz * w / nth)) - a.rp.begin());
+    RawVec<int32_t> cnt((size_t)nth * nc);
+    host_parallel(nth, [&](int64_t w) {
+        int32_t *c = cnt.data() + w * nc;
+        std::fill(c, c + nc, 0);
+        const int32_t r1 = std::max(cut[w], cut[w + 1]);
+        for (int32_t k = a.rp[cut[w]]; k < a.rp[r1]; ++k) c[a.ci[k]]++;
+    });
+    const int64_t piece = 1 << 16, pieces = (nc + piece - 1) / piece;
+    host_parallel(pieces, [&](int64_t b) {  // column totals; counts -> offsets within the column
+        for (int64_t j = b * piece; j < std::min(nc, (b + 1) * piece); ++j) {
+            int32_t run = 0;
+            for (int w = 0; w < nth; ++w) {
+                const int32_t k = cnt[(size_t)w * nc + j];
+                cnt[(size_t)w * nc + j] = run;
+                run += k;
+            }
+            t.rp[j + 1] = run;
+        }
+    });
+    for (int64_t j = 0; j < nc; ++j) t.rp[j + 1] += t.rp[j];
+    host_parallel(pieces, [&](int64_t b) {
+        for (int64_t j = b * piece; j < std::min(nc, (b + 1) * piece); ++j)
+            for (int w = 0; w < nth; ++w) cnt[(size_t)w * nc + j] += t.rp[j];
+    });
+    host_parallel(nth, [&](int64_t w) {
+        int32_t *next = cnt.data() + w * nc;
+        const int32_t r1 = std::max(cut[w], cut[w + 1]);
+        for (int32_t i = cut[w]; i < r1; ++i)
             for (int32_t k = a.rp[i]; k < a.rp[i + 1]; ++k) {
-                const int32_t c = a.ci[k];
-                if (c < lo || c >= hi) continue;
-                const int32_t d = next[c - lo]++;
+                const int32_t d = next[a.ci[k]]++;
                 t.ci[d] = i;
                 t.v[d] = a.v[k];
             }
-    };
-    std::vector<std::thread> pool;
-    for (int w = 1; w < nth; ++w) pool.emplace_back(fill, w);
-    fill(0);
-    for (auto &th : pool) th.join();
+    });
     return t;
 }
 
 // rows sorted by column, duplicates summed
 void canonicalize(HostCsr &a) {
-    std::vector<int32_t> rp(1, 0), ci;
-    std::vector<double> v;
+    std::vector<int32_t> rp(1, 0);
+    RawVec<int32_t> ci;
+    RawVec<double> v;
     std::vector<std::pair<int32_t, double>> row;
     for (int32_t i = 0; i < a.nrows; ++i) {
         row.clear();
@@ -1426,8 +1500,8 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     A.ncols = n;
     A.rp.assign(p->A_rowptr ? p->A_rowptr : nullptr, p->A_rowptr ? p->A_rowptr + m + 1 : nullptr);
     if (A.rp.empty()) A.rp.assign((size_t)m + 1, 0);
-    A.ci.assign(p->A_colidx, p->A_colidx + p->A_nnz);
-    A.v.assign(p->A_val, p->A_val + p->A_nnz);
+    par_assign(A.ci, p->A_colidx, p->A_nnz);
+    par_assign(A.v, p->A_val, p->A_nnz);
     HostCsr AT = transpose(A);
     h->nnzA = p->A_nnz;
     h->q_kind = p->Q_kind;
@@ -3268,16 +3342,27 @@ namespace {
 // linear rows, then all SOC rows; all boxed columns, then all SOC columns).
 struct Combined {
     hprsocp_problem p{};
-    std::vector<int32_t> rp, ci, qrp, qci, con_idx, var_idx;
-    std::vector<double> v, qv, c, AL, AU, l, u, rhs;
+    std::vector<int32_t> rp, qrp, con_idx, var_idx;
+    RawVec<int32_t> ci, qci;
+    RawVec<double> v, qv;
+    std::vector<double> c, AL, AU, l, u, rhs;
     std::vector<InstRange> rng;
     std::vector<double> objc;
 };
 
 void concat_problems(const hprsocp_problem *const *probs, int32_t count, Combined &C) {
-    std::string e;
     int64_t M1 = 0, MC = 0, N1 = 0, NC = 0, KC = 0, KV = 0;
     const bool has_q = probs[0] && probs[0]->Q_kind == HPRSOCP_Q_CSR && probs[0]->Q_nnz > 0;
+    // the CSR scans first, in parallel; the checks below report them in order
+    std::vector<int> rcA(count, 0), rcQ(count, 0);
+    std::vector<std::string> errA(count), errQ(count);
+    host_parallel(count, [&](int64_t i) {
+        const hprsocp_problem *q = probs[i];
+        if (!q || q->m < 0 || q->n < 0) return;
+        rcA[i] = validate_csr(errA[i], "A", q->m, q->n, q->A_nnz, q->A_rowptr, q->A_colidx, q->A_val);
+        if (q->Q_kind == HPRSOCP_Q_CSR && q->Q_nnz > 0)
+            rcQ[i] = validate_csr(errQ[i], "Q", q->n, q->n, q->Q_nnz, q->Q_rowptr, q->Q_colidx, q->Q_val);
+    });
     for (int32_t i = 0; i < count; ++i) {
         const hprsocp_problem *q = probs[i];
         char b[200];
@@ -3285,14 +3370,11 @@ void concat_problems(const hprsocp_problem *const *probs, int32_t count, Combine
         snprintf(b, sizeof b, "batch instance %d: ", i);
         const std::string pre(b);
         if (q->m < 0 || q->n < 0) throw Err{HPRSOCP_ERR_INVALID, pre + "negative dimensions"};
-        if (int rc = validate_csr(e, "A", q->m, q->n, q->A_nnz, q->A_rowptr, q->A_colidx, q->A_val))
-            throw Err{rc, pre + e};
+        if (rcA[i]) throw Err{rcA[i], pre + errA[i]};
         if (q->Q_kind != HPRSOCP_Q_CSR) throw Err{HPRSOCP_ERR_UNSUPPORTED, pre + "batches take CSR Q only"};
         if ((q->Q_nnz > 0) != has_q)
             throw Err{HPRSOCP_ERR_INVALID, pre + "all instances of a batch must have a quadratic term, or none"};
-        if (q->Q_nnz > 0)
-            if (int rc = validate_csr(e, "Q", q->n, q->n, q->Q_nnz, q->Q_rowptr, q->Q_colidx, q->Q_val))
-                throw Err{rc, pre + e};
+        if (q->Q_nnz > 0 && rcQ[i]) throw Err{rcQ[i], pre + errQ[i]};
         if (q->number_linear_con < 0 || q->number_linear_con > q->m || q->number_lu_x < 0 || q->number_lu_x > q->n)
             throw Err{HPRSOCP_ERR_INVALID, pre + "number_linear_con / number_lu_x out of range"};
         if (!q->c || !q->l || !q->u || (q->m > 0 && (!q->AL || !q->AU)))
@@ -3339,37 +3421,53 @@ void concat_problems(const hprsocp_problem *const *probs, int32_t count, Combine
         const int32_t n1 = probs[i]->number_lu_x;
         return c < n1 ? C.rng[i].box0 + c : C.rng[i].var0 + (c - n1);
     };
-    // A rows in global order
-    C.rp.assign(1, 0);
-    for (int pass = 0; pass < 2; ++pass)
-        for (int32_t i = 0; i < count; ++i) {
-            const hprsocp_problem *q = probs[i];
-            const int32_t r0 = pass ? q->number_linear_con : 0, r1 = pass ? q->m : q->number_linear_con;
-            for (int32_t r = r0; r < r1; ++r) {
-                if (q->A_rowptr)
-                    for (int32_t k = q->A_rowptr[r]; k < q->A_rowptr[r + 1]; ++k) {
-                        C.ci.push_back(gcol(i, q->A_colidx[k]));
-                        C.v.push_back(q->A_val[k]);
-                    }
-                C.rp.push_back((int32_t)C.ci.size());
-            }
+    // Rows of instance i's pass 0 (linear rows of A / boxed columns of Q) and
+    // pass 1 (SOC rows / SOC columns) land at precomputed offsets, so the
+    // 2 x count segments fill in parallel.
+    auto pack = [&](bool q_mat, int64_t nrows_total, std::vector<int32_t> &rp, RawVec<int32_t> &ci,
+                    RawVec<double> &v) {
+        const int64_t S = 2 * (int64_t)count;
+        std::vector<int64_t> row_off(S + 1, 0), nnz_off(S + 1, 0);
+        auto seg = [&](int64_t s, int32_t &r0, int32_t &r1, const int32_t *&crp, const int32_t *&cci,
+                       const double *&cv) {
+            const hprsocp_problem *q = probs[s % count];
+            const bool pass = s >= count;
+            const int32_t split = q_mat ? q->number_lu_x : q->number_linear_con, end = q_mat ? q->n : q->m;
+            r0 = pass ? split : 0;
+            r1 = pass ? end : split;
+            crp = q_mat ? q->Q_rowptr : q->A_rowptr;
+            cci = q_mat ? q->Q_colidx : q->A_colidx;
+            cv = q_mat ? q->Q_val : q->A_val;
+        };
+        for (int64_t s = 0; s < S; ++s) {
+            int32_t r0, r1;
+            const int32_t *crp, *cci;
+            const double *cv;
+            seg(s, r0, r1, crp, cci, cv);
+            row_off[s + 1] = row_off[s] + (r1 - r0);
+            nnz_off[s + 1] = nnz_off[s] + (crp ? crp[r1] - crp[r0] : 0);
         }
-    // Q rows (= columns) in global column order
-    if (has_q) {
-        C.qrp.assign(1, 0);
-        for (int pass = 0; pass < 2; ++pass)
-            for (int32_t i = 0; i < count; ++i) {
-                const hprsocp_problem *q = probs[i];
-                const int32_t r0 = pass ? q->number_lu_x : 0, r1 = pass ? q->n : q->number_lu_x;
-                for (int32_t r = r0; r < r1; ++r) {
-                    for (int32_t k = q->Q_rowptr[r]; k < q->Q_rowptr[r + 1]; ++k) {
-                        C.qci.push_back(gcol(i, q->Q_colidx[k]));
-                        C.qv.push_back(q->Q_val[k]);
-                    }
-                    C.qrp.push_back((int32_t)C.qci.size());
+        if (nnz_off[S] > INT32_MAX) throw Err{HPRSOCP_ERR_INVALID, "batch too large for 32-bit indices"};
+        rp.assign((size_t)nrows_total + 1, 0);
+        ci.resize((size_t)nnz_off[S]);
+        v.resize((size_t)nnz_off[S]);
+        host_parallel(S, [&](int64_t s) {
+            int32_t r0, r1;
+            const int32_t *crp, *cci;
+            const double *cv;
+            seg(s, r0, r1, crp, cci, cv);
+            const int32_t i = (int32_t)(s % count);
+            const int64_t base = crp ? crp[r0] : 0, o = nnz_off[s] - base;
+            for (int32_t r = r0; r < r1; ++r) rp[row_off[s] + (r - r0) + 1] = (int32_t)(o + (crp ? crp[r + 1] : 0));
+            if (crp)
+                for (int64_t k = crp[r0]; k < crp[r1]; ++k) {
+                    ci[o + k] = gcol(i, cci[k]);
+                    v[o + k] = cv[k];
                 }
-            }
-    }
+        });
+    };
+    pack(false, M, C.rp, C.ci, C.v);
+    if (has_q) pack(true, N, C.qrp, C.qci, C.qv);
     C.c.resize(N);
     C.l.resize(N);
     C.u.resize(N);
