@@ -365,9 +365,12 @@ struct hprsocp_handle {
     std::vector<hipEvent_t> ev;
     // captured runs of partial updates: (first kstep, length, fp32) -> executable graph
     std::map<std::tuple<int64_t, int32_t, int>, hipGraphExec_t> graphs;
+    // batches: windows requested since the last compaction, not yet captured
+    std::map<std::tuple<int64_t, int32_t, int>, int> graph_seen;
     void clear_graphs() {
         for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
         graphs.clear();
+        graph_seen.clear();
     }
 
     InstScal &I(int i) { return isc_h[i]; }
@@ -1312,6 +1315,12 @@ void delta_prec(hprsocp_handle *h, int f32) {
 // Delta form: FP64 state -> deltas from the anchors, the anchors' images
 // (ra, rya) and the FP32 data of the float kernels.  Runs whenever the FP64
 // state changed (iterations, restarts, sigma, initial points).
+// Batches after retirements: the compacted tile lists of A, A^T (and Q) cover the
+// running instances.  The FP64 lists hold every running instance unless
+// FP32-state instances are split off into their own lists; the products they
+// skip belong to retired instances, whose outputs nothing reads.
+static bool act64(const hprsocp_handle *h) { return h->use_act && (!h->fp32 || h->delta); }
+
 void delta_split(hprsocp_handle *h) {
     hipStream_t st = h->st;
     IterView<float> &V = h->v32;
@@ -1331,11 +1340,11 @@ void delta_split(hprsocp_handle *h) {
         qmap(h, h->last_w, h->d_img_n);  // Q last_w
         launch_delta_ra(st, sn, h->last_x, h->last_aty, h->d_img_n, h->c, h->isc, h->d_ra);
     } else {
-        launch_spmv(st, h->plan_AT, h->AT, h->last_y, h->d_img_n);  // A^T last_y
+        launch_spmv(st, act64(h) ? h->act_AT : h->plan_AT, h->AT, h->last_y, h->d_img_n);  // A^T last_y
         launch_delta_ra(st, sn, h->last_x, h->d_img_n, nullptr, h->c, h->isc, h->d_ra);
     }
     if (m > 0) {
-        launch_spmv(st, h->plan_A, h->A, h->last_x, h->d_img_m);  // A last_x
+        launch_spmv(st, act64(h) ? h->act_A : h->plan_A, h->A, h->last_x, h->d_img_m);  // A last_x
         launch_delta_rya(st, sm, h->m1, h->d_img_m, h->rhs, h->last_y, h->isc, h->d_rya);
     }
     h->images_valid = true;
@@ -2047,9 +2056,17 @@ static void compact_active(hprsocp_handle *h) {
 // kernels read sigma, the Halpern counter base and the retirement mask from
 // device memory, so one capture serves every check window that starts at the
 // same step.  Graphs are dropped when the launch shapes change (tile
-// compaction) and captured again on demand.
+// compaction) and captured again on demand.  In a batch, instances retire at
+// many different checks and every retirement compacts the tile lists, so a
+// window is captured only when it is requested a second time since the last
+// compaction; the first time it runs eagerly (the same kernels, so the same
+// results), which keeps capture costs out of the stretches between retirements.
 static void run_partial_updates(hprsocp_handle *h, int64_t L) {
-    if (!h->params.use_graph || L < 2) {
+    const int fp = h->delta ? 4 : (h->n32 > 0 ? 1 : 0) | (h->n64 > 0 ? 2 : 0);
+    const auto key = std::make_tuple(h->kstep, (int32_t)L, fp);
+    bool eager = !h->params.use_graph || L < 2;
+    if (!eager && h->nb > 1 && !h->graphs.count(key)) eager = h->graph_seen[key]++ == 0;
+    if (eager) {
         for (int64_t k = 0; k < L; ++k) do_update(h, 0);
         return;
     }
@@ -2060,8 +2077,6 @@ static void run_partial_updates(hprsocp_handle *h, int64_t L) {
     } else if (h->n32 > 0 && h->fp32_dirty) {
         demote_state(h);
     }
-    const int fp = h->delta ? 4 : (h->n32 > 0 ? 1 : 0) | (h->n64 > 0 ? 2 : 0);
-    const auto key = std::make_tuple(h->kstep, (int32_t)L, fp);
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
         if (h->graphs.size() >= 64) h->clear_graphs();
@@ -2107,7 +2122,7 @@ static void residuals_impl(hprsocp_handle *h, int64_t iter, hprsocp_residuals *r
     // batches after retirements: the SpMVs over the running instances' tiles (the
     // compacted FP64 lists hold every running instance unless FP32-state
     // instances are split off into their own lists)
-    const bool act = h->use_act && (!h->fp32 || h->delta);
+    const bool act = act64(h);
     if (!skip_q) qmap(h, h->x_bar, h->qx);
     if (m > 0) launch_spmv(st, act ? h->act_AT : h->plan_AT, h->AT, h->y_bar, h->atdy);
     else launch_fill(st, n, 0.0, h->atdy);
@@ -2302,7 +2317,7 @@ void mnorm_enqueue(hprsocp_handle *h, int base) {
     hipStream_t st = h->st;
     if (q) qmap(h, h->dw, h->dqw);
     if (m > 0) {
-        launch_spmv(st, h->plan_AT, h->AT, h->dy, h->atdy);
+        launch_spmv(st, act64(h) ? h->act_AT : h->plan_AT, h->AT, h->dy, h->atdy);
         if (q) qmap(h, h->atdy, h->qatdy);
     }
     std::vector<RedTerm> tn;
@@ -2486,7 +2501,7 @@ void update_sigma(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<
             launch_axpby(st, n, 1.0, h->aty_bar, -1.0, h->last_aty, h->atdy);
             qmap(h, h->atdy, h->qatdy);
         } else {
-            launch_spmv(st, h->plan_AT, h->AT, h->dy, h->atdy);
+            launch_spmv(st, act64(h) ? h->act_AT : h->plan_AT, h->AT, h->dy, h->atdy);
         }
     }
     // n-side dots: b=|dx|^2, <dQw,ATdy>, |dQw|^2, <dw,dQw>, <ATdy,QATdy>
@@ -2638,7 +2653,7 @@ bool do_restart(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<ch
         launch_copy_flagged(st, sm, h->y_bar, h->y, h->flag);
         launch_copy_flagged(st, sm, h->y_bar, h->last_y, h->flag);
         if (h->has_quad) {
-            launch_spmv(st, h->plan_AT, h->AT, h->y_bar, h->tmp_n2);
+            launch_spmv(st, act64(h) ? h->act_AT : h->plan_AT, h->AT, h->y_bar, h->tmp_n2);
             launch_copy_flagged(st, sn, h->tmp_n2, h->aty_bar, h->flag);
             launch_copy_flagged(st, sn, h->tmp_n2, h->last_aty, h->flag);
             launch_copy_flagged(st, sn, h->tmp_n2, h->aty, h->flag);
