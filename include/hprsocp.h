@@ -163,7 +163,9 @@ typedef struct {
     /* extensions */
     int32_t precision;         /* hprsocp_precision, default FP64             */
     int32_t use_graph;         /* capture the non-check iterations of a
-                                  check window in one hipGraph (default 1)    */
+                                  check window in one hipGraph (default 1);
+                                  batches capture a window on its second
+                                  request after a retirement, eager before */
     double refine_tol;         /* FP32_REFINE: KKT below which the iteration
                                   continues in FP64 (default 1e-6)            */
     int32_t fp32_stall_checks; /* FP32_REFINE: checks without a 1% improvement
