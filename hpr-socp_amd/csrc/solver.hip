@@ -278,6 +278,9 @@ struct InstHost {
     int32_t Q_is_diag = 0;
 };
 
+struct SolveLoop;
+void free_loop(SolveLoop *s);
+
 struct hprsocp_handle {
     std::string err;
     hprsocp_params params{};
@@ -360,6 +363,8 @@ struct hprsocp_handle {
     bool delta_live = false;    // the deltas hold the current state (the FP64 x, y, w, aty are stale)
     bool images_valid = false;  // ra, rya match the anchors, sigma and lambda_A
     double *d_ra = nullptr, *d_rya = nullptr, *d_img_n = nullptr, *d_img_m = nullptr;
+    // the solve loop in progress (hprsocp_iterate)
+    SolveLoop *loop = nullptr;
     // timing (profiled iterations)
     bool profile = false;
     std::vector<hipEvent_t> ev;
@@ -406,6 +411,7 @@ struct hprsocp_handle {
     void sync() { HIPCHK(hipStreamSynchronize(st)); }
     ~hprsocp_handle() {
         if (st) (void)hipStreamSynchronize(st);
+        free_loop(loop);
         clear_graphs();
         for (void *p : allocs) (void)hipFree(p);
         if (hscal) (void)hipHostFree(hscal);
@@ -2366,23 +2372,24 @@ double mnorm_finish(hprsocp_handle *h, int i, const double *d, const MnormCtx &c
 }
 
 // ------------------------------------------------------------------ restart
-struct Restart {  // HPRSOCP_restart, structs.jl:600-623
-    int restart_flag = 0;
-    bool first_restart = true;
-    double last_gap = INFINITY, current_gap = INFINITY, save_gap = INFINITY;
-    int64_t inner = 0, times = 0, sufficient = 0, necessary = 0, longr = 0;
-    double weighted_norm = INFINITY, best_gap = INFINITY, best_kkt = INFINITY, best_sigma = 1.0;
-    int64_t best_iter = 0;
-    bool corr_active = false;
-    int corr_hold = 0, corr_dir = 0;
-};
+// HPRSOCP_restart (structs.jl:600-623) is the public hprsocp_restart_info.
+using Restart = hprsocp_restart_info;
+
+// initialize_restart, algorithm.jl:1886-1908
+void init_restart(Restart &r) {
+    memset(&r, 0, sizeof(r));
+    r.first_restart = 1;
+    r.last_gap = r.current_gap = r.save_gap = INFINITY;
+    r.weighted_norm = r.best_gap = r.best_kkt = INFINITY;
+    r.best_sigma = 1.0;
+}
 
 // check_restart, algorithm.jl:1016-1099
 void check_restart(Restart &r, int64_t iter, int64_t check_iter, double sigma, double kkt) {
     r.restart_flag = 0;
     if (r.first_restart) {
         if (iter == check_iter) {
-            r.first_restart = false;
+            r.first_restart = 0;
             r.restart_flag = 1;
             r.weighted_norm = r.current_gap;
             r.best_gap = r.current_gap;
@@ -2405,7 +2412,7 @@ void check_restart(Restart &r, int64_t iter, int64_t check_iter, double sigma, d
     if (((double)r.inner >= jmax(500.0, fact * (double)iter) ||
          (r.inner >= 10000 && r.current_gap <= 0.75 * r.last_gap)) &&
         r.current_gap > 0.9 * r.save_gap) {
-        r.longr++;
+        r.long_restarts++;
         r.restart_flag = 3;
     }
     // should_refresh_best_sigma, algorithm.jl:682-693
@@ -2591,37 +2598,37 @@ void update_sigma(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<
         const bool keep = !both && ((es <= 0.8 * THR) || (ir >= 0.1 * IMB));
         const int cdir = ed >= ep ? 1 : -1;
         if (both) {
-            r.corr_active = false;
-            r.corr_hold = -1;
-            r.corr_dir = 0;
-        } else if (trig || (r.corr_active && keep)) {
-            r.corr_active = true;
-            r.corr_hold = 0;
-            r.corr_dir = cdir;
+            r.sigma_correction_active = 0;
+            r.sigma_correction_hold = -1;
+            r.sigma_correction_dir = 0;
+        } else if (trig || (r.sigma_correction_active && keep)) {
+            r.sigma_correction_active = 1;
+            r.sigma_correction_hold = 0;
+            r.sigma_correction_dir = cdir;
         } else {
-            r.corr_active = false;
-            r.corr_hold = -1;
-            r.corr_dir = 0;
+            r.sigma_correction_active = 0;
+            r.sigma_correction_hold = -1;
+            r.sigma_correction_dir = 0;
         }
         double beta = 0.0;
         if (!both) {
             if (one) beta = 0.5;
             if (extreme) beta = 1.0;
-            else if (r.corr_active) beta = 0.5;
+            else if (r.sigma_correction_active) beta = 0.5;
         }
         double kappa = 1.0;
         if (beta > 0.0) {
             double rk = ed / ep;
-            if (r.corr_active) {
-                if (r.corr_dir > 0) rk = jmax(rk, FLOOR);
-                else if (r.corr_dir < 0) rk = jmin(rk, 1.0 / FLOOR);
+            if (r.sigma_correction_active) {
+                if (r.sigma_correction_dir > 0) rk = jmax(rk, FLOOR);
+                else if (r.sigma_correction_dir < 0) rk = jmin(rk, 1.0 / FLOOR);
             }
             kappa = exp(beta * jclamp(log(rk), -LOG100, LOG100));
         }
         double sn = sbase * kappa;
-        if (r.corr_active) {
-            if (r.corr_dir > 0) sn = jmax(sn, sigma_old * 1.5);
-            else if (r.corr_dir < 0) sn = jmin(sn, sigma_old / 1.5);
+        if (r.sigma_correction_active) {
+            if (r.sigma_correction_dir > 0) sn = jmax(sn, sigma_old * 1.5);
+            else if (r.sigma_correction_dir < 0) sn = jmin(sn, sigma_old / 1.5);
         }
         sn = jclamp(sn, SMIN, SMAX);
         h->I(i).sigma = sn;
@@ -2847,15 +2854,42 @@ bool should_print(const hprsocp_params &p, int64_t iter, double t_start) {
 // checks, restarts and sigma updates are per instance; an instance that stops
 // is retired (its tiles skip all work) and its result recorded.  res has nb
 // entries.
-void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
+}  // namespace
+
+// State of the solve loop (algorithm.jl:2806-2904) between calls, so the
+// loop can run in windows (hprsocp_iterate) as well as in one piece.
+struct SolveLoop {
+    bool live = false;  // a loop is in progress
+    double t_start = 0;
+    int64_t iter = 0;   // next loop body
+    int nactive = 0;
+    std::vector<Restart> R;
+    std::vector<hprsocp_residuals> rs;
+    std::vector<int64_t> it4, it6, inner, restarts32, iter32;
+    std::vector<double> t4, t6, best32;
+    std::vector<char> first4, first6, active, pend0, pend1;
+    std::vector<int> stall32;
+    // deferred M-norms: slot 0 -> last_gap candidate, slot 1 -> current_gap
+    MnormCtx mn[2];
+    std::vector<hprsocp_result> out;  // results of the retired instances
+};
+
+namespace {
+
+// Everything solve() does between the setup and the first loop body.
+void begin_loop(hprsocp_handle *h) {
     const hprsocp_params &p = h->params;
-    const double t_start = now_s();
+    if (!h->loop) h->loop = new SolveLoop();
+    SolveLoop &S = *h->loop;
+    S = SolveLoop{};
+    S.t_start = now_s();
     delta_join(h);
     if (!h->lambda_set) hprsocp_estimate_lambda(h);
     const int nb = h->nb;
-    std::vector<Restart> R((size_t)nb);
+    S.R.resize(nb);
     for (int i = 0; i < nb; ++i) {
-        R[i].best_sigma = h->I(i).sigma;
+        init_restart(S.R[i]);
+        S.R[i].best_sigma = h->I(i).sigma;
         h->I(i).active = 1;
     }
     h->saved = hprsocp_handle::Saved{};
@@ -2867,15 +2901,19 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
         if (single) printf(" iter     errRp     errRd         p_obj           d_obj          gap        sigma       time\n");
         else printf("batch of %d instances\n", nb);
     }
-    std::vector<hprsocp_residuals> rs((size_t)nb);
-    std::vector<int64_t> it4(nb, 0), it6(nb, 0), inner(nb, 0);
-    std::vector<double> t4(nb, 0.0), t6(nb, 0.0);
-    std::vector<char> first4(nb, 1), first6(nb, 1), active(nb, 1);
-    int nactive = nb;
-    const int64_t check_iter = p.check_iter;
-    // deferred M-norms: slot 0 -> last_gap candidate, slot 1 -> current_gap
-    MnormCtx mn[2];
-    std::vector<char> pend0(nb, 0), pend1(nb, 0);
+    S.rs.assign(nb, hprsocp_residuals{});
+    S.it4.assign(nb, 0);
+    S.it6.assign(nb, 0);
+    S.inner.assign(nb, 0);
+    S.t4.assign(nb, 0.0);
+    S.t6.assign(nb, 0.0);
+    S.first4.assign(nb, 1);
+    S.first6.assign(nb, 1);
+    S.active.assign(nb, 1);
+    S.nactive = nb;
+    S.pend0.assign(nb, 0);
+    S.pend1.assign(nb, 0);
+    S.out.assign(nb, hprsocp_result{});
     h->to_check = 1;
     // FP32 bulk: windows of FP32 iterations, FP64 checks; an instance continues
     // in FP64 once it reaches refine_tol or stops improving
@@ -2883,11 +2921,43 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
     upload_isc(h);
     h->fp32_dirty = true;
     h->images_valid = false;
-    std::vector<double> best32(nb, INFINITY);
-    std::vector<int> stall32(nb, 0);
-    std::vector<int64_t> restarts32(nb, 0);
-    std::vector<int64_t> iter32(nb, 0);
-    for (int64_t iter = 0; iter <= p.max_iter; ++iter) {
+    S.best32.assign(nb, INFINITY);
+    S.stall32.assign(nb, 0);
+    S.restarts32.assign(nb, 0);
+    S.iter32.assign(nb, 0);
+    S.live = true;
+}
+
+// Runs loop bodies S.iter .. stop-1 (or to termination).  Returns true when
+// every instance has terminated; S.out then holds the results.
+bool run_loop(hprsocp_handle *h, int64_t stop) {
+    const hprsocp_params &p = h->params;
+    SolveLoop &S = *h->loop;
+    const double t_start = S.t_start;
+    const int nb = h->nb;
+    const bool single = nb == 1;
+    const int64_t check_iter = p.check_iter;
+    auto &R = S.R;
+    auto &rs = S.rs;
+    auto &it4 = S.it4;
+    auto &it6 = S.it6;
+    auto &inner = S.inner;
+    auto &t4 = S.t4;
+    auto &t6 = S.t6;
+    auto &first4 = S.first4;
+    auto &first6 = S.first6;
+    auto &active = S.active;
+    int &nactive = S.nactive;
+    MnormCtx *mn = S.mn;
+    auto &pend0 = S.pend0;
+    auto &pend1 = S.pend1;
+    auto &best32 = S.best32;
+    auto &stall32 = S.stall32;
+    auto &restarts32 = S.restarts32;
+    auto &iter32 = S.iter32;
+    hprsocp_result *res = S.out.data();
+    for (; S.iter <= p.max_iter && S.iter < stop; ++S.iter) {
+        int64_t &iter = S.iter;
         const bool print_yes = should_print(p, iter, t_start);
         const bool check = iter % check_iter == 0;
         if (check || print_yes) {
@@ -3028,7 +3098,8 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
                 h->I(i).fp32 = h->fp32 ? 1 : 0;
             }
             upload_isc(h);
-            return;
+            S.live = false;
+            return true;
         }
         if (p.verbose && !single && check && iter > 0 && (iter / check_iter) % 20 == 0) {
             double worst = 0;
@@ -3056,7 +3127,7 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
                 if (t >= p.max_iter || t % check_iter == 0) return false;
                 return p.print_frequency == -1 ? !rem_zero(t, print_step(t)) : (t % p.print_frequency != 0);
             };
-            while (plain(nx + L) && L < check_iter) ++L;
+            while (plain(nx + L) && L < check_iter && iter + L < stop) ++L;
             for (int i = 0; i < nb; ++i)
                 if (active[i] && (h->I(i).fp32 || h->delta)) iter32[i] += L;
             run_partial_updates(h, L);
@@ -3082,9 +3153,48 @@ void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
         for (int i = 0; i < nb; ++i)
             if (active[i]) R[i].inner += 1;
     }
+    return false;
+}
+
+// Result of a loop still in progress (single problem): the last check's
+// residuals and the current counters, status CONTINUE.
+void progress_result(hprsocp_handle *h, hprsocp_result *o) {
+    const SolveLoop &S = *h->loop;
+    const hprsocp_residuals &r = S.rs[0];
+    memset(o, 0, sizeof(*o));
+    o->iter = S.iter;
+    o->time = now_s() - S.t_start;
+    o->power_time = h->power_time;
+    o->setup_time = h->setup_time;
+    o->residuals = r.kkt;
+    o->primal_obj = r.primal_obj;
+    o->dual_obj = r.dual_obj;
+    o->gap = r.rel_gap;
+    o->err_Rp = r.err_Rp;
+    o->err_Rp_linear = r.err_Rp_linear;
+    o->err_Rp_soc = r.err_Rp_soc;
+    o->err_Rd = r.err_Rd;
+    o->status = HPRSOCP_STATUS_CONTINUE;
+    o->iter_4 = S.it4[0];
+    o->iter_6 = S.it6[0];
+    o->time_4 = S.t4[0];
+    o->time_6 = S.t6[0];
+    o->sigma = h->I(0).sigma;
+    o->lambda_max_A = h->I(0).lambda_A;
+    o->lambda_max_Q = h->I(0).lambda_Q;
+    o->restarts = (int32_t)S.R[0].times;
+    o->iter_fp32 = S.iter32[0];
+}
+
+void solve_impl(hprsocp_handle *h, hprsocp_result *res) {
+    begin_loop(h);
+    run_loop(h, INT64_MAX);
+    std::copy(h->loop->out.begin(), h->loop->out.end(), res);
 }
 
 }  // namespace
+
+void free_loop(SolveLoop *s) { delete s; }
 
 extern "C" {
 
@@ -3102,6 +3212,62 @@ int hprsocp_mnorm(hprsocp_handle *h, double *out) {
 
 int hprsocp_solve(hprsocp_handle *h, hprsocp_result *res) {
     ABI_TRY(h, { solve_impl(h, res); });
+    return 0;
+}
+
+int hprsocp_iterate(hprsocp_handle *h, int64_t k, int32_t flags, hprsocp_result *res) {
+    ABI_TRY(h, {
+        if (h->nb != 1) throw Err{HPRSOCP_ERR_UNSUPPORTED, "hprsocp_iterate: single-problem handles only"};
+        if (k < 0) throw Err{HPRSOCP_ERR_INVALID, "hprsocp_iterate: k must be >= 0"};
+        if (!res) throw Err{HPRSOCP_ERR_INVALID, "hprsocp_iterate: res is NULL"};
+        if ((flags & HPRSOCP_ITERATE_RESET) || !h->loop || !h->loop->live) begin_loop(h);
+        SolveLoop &S = *h->loop;
+        const int64_t stop = k > INT64_MAX - S.iter ? INT64_MAX : S.iter + k;
+        if (run_loop(h, stop)) *res = S.out[0];
+        else progress_result(h, res);
+    });
+    return 0;
+}
+
+void hprsocp_init_restart(hprsocp_restart_info *r) {
+    if (r) init_restart(*r);
+}
+
+void hprsocp_check_restart(hprsocp_restart_info *r, int64_t iter, int64_t check_iter, double sigma, double kkt) {
+    if (r && check_iter > 0) check_restart(*r, iter, check_iter, sigma, kkt);
+}
+
+int hprsocp_update_sigma(hprsocp_handle *h, hprsocp_restart_info *r, const hprsocp_residuals *res) {
+    ABI_TRY(h, {
+        if (h->nb != 1) throw Err{HPRSOCP_ERR_UNSUPPORTED, "hprsocp_update_sigma: single-problem handles only"};
+        if (!r || !res) throw Err{HPRSOCP_ERR_INVALID, "hprsocp_update_sigma: NULL argument"};
+        if (r->restart_flag >= 1) {
+            delta_join(h);
+            std::vector<Restart> R{*r};
+            std::vector<hprsocp_residuals> RS{*res};
+            const double s0 = h->I(0).sigma;
+            update_sigma(h, R, RS, std::vector<char>{1});
+            if (h->I(0).sigma != s0) h->images_valid = false;
+            *r = R[0];
+        }
+    });
+    return 0;
+}
+
+int hprsocp_restart(hprsocp_handle *h, hprsocp_restart_info *r) {
+    ABI_TRY(h, {
+        if (h->nb != 1) throw Err{HPRSOCP_ERR_UNSUPPORTED, "hprsocp_restart: single-problem handles only"};
+        if (!r) throw Err{HPRSOCP_ERR_INVALID, "hprsocp_restart: NULL argument"};
+        if (r->restart_flag > 0) {
+            delta_join(h);
+            std::vector<Restart> R{*r};
+            if (do_restart(h, R, std::vector<char>{1})) {
+                h->fp32_dirty = true;
+                h->images_valid = false;
+            }
+            *r = R[0];
+        }
+    });
     return 0;
 }
 

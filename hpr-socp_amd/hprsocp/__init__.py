@@ -394,13 +394,39 @@ class Solver:
         self._check(self._fn("spmv")(self.h, int(trans), _abi.ptr_f64(a), _abi.ptr_f64(out)))
         return out
 
-    def solve(self) -> HPRSOCP_results:
-        r = _abi.Result()
-        self._check(self._fn("solve")(self.h, C.byref(r)))
+    def _results(self, r) -> HPRSOCP_results:
         x, y, z, w = np.zeros(self.n), np.zeros(self.m), np.zeros(self.n), np.zeros(self.n)
         self._check(self._fn("get_solution")(self.h, _abi.ptr_f64(x), _abi.ptr_f64(y),
                                               _abi.ptr_f64(z), _abi.ptr_f64(w)))
         return HPRSOCP_results.from_c(r, x, y, z, w)
+
+    def solve(self) -> HPRSOCP_results:
+        r = _abi.Result()
+        self._check(self._fn("solve")(self.h, C.byref(r)))
+        return self._results(r)
+
+    def iterate(self, k: int, reset: bool = False) -> HPRSOCP_results:
+        """Runs the next ``k`` bodies of the solve loop (hprsocp_iterate);
+        ``status`` is "CONTINUE" until the loop terminates."""
+        r = _abi.Result()
+        self._check(self._fn("iterate")(self.h, int(k), _abi.ITERATE_RESET if reset else 0, C.byref(r)))
+        return self._results(r)
+
+    # restart / sigma seams (check_restart, update_sigma!, do_restart!)
+    def init_restart(self) -> _abi.RestartInfo:
+        r = _abi.RestartInfo()
+        self._fn("init_restart")(C.byref(r))
+        return r
+
+    def check_restart(self, r: _abi.RestartInfo, it: int, check_iter: int, sigma: float, kkt: float):
+        self._fn("check_restart")(C.byref(r), int(it), int(check_iter), float(sigma), float(kkt))
+
+    def update_sigma(self, r: _abi.RestartInfo, res: dict):
+        rr = _abi.Residuals(**{k: float(res.get(k, 0.0)) for k, _ in _abi.Residuals._fields_})
+        self._check(self._fn("update_sigma")(self.h, C.byref(r), C.byref(rr)))
+
+    def restart(self, r: _abi.RestartInfo):
+        self._check(self._fn("restart")(self.h, C.byref(r)))
 
     def probe_copy(self, nbytes: int) -> float:
         """Plain device copy of `nbytes` (counter calibration); returns ms."""

@@ -112,6 +112,24 @@ class Scalars(C.Structure):
     ]
 
 
+class RestartInfo(C.Structure):
+    """hprsocp_restart_info: field for field the reference's HPRSOCP_restart."""
+    _fields_ = [
+        ("restart_flag", C.c_int64), ("first_restart", C.c_uint8),
+        ("last_gap", C.c_double), ("current_gap", C.c_double), ("save_gap", C.c_double),
+        ("inner", C.c_int64), ("step", C.c_int64), ("sufficient", C.c_int64),
+        ("necessary", C.c_int64), ("long_restarts", C.c_int64), ("ratio", C.c_int64),
+        ("times", C.c_int64),
+        ("weighted_norm", C.c_double), ("best_gap", C.c_double), ("best_kkt", C.c_double),
+        ("best_sigma", C.c_double), ("best_iter", C.c_int64),
+        ("sigma_correction_active", C.c_uint8),
+        ("sigma_correction_hold", C.c_int64), ("sigma_correction_dir", C.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # name -> (restype, argtypes); "H" = opaque handle pointer
 _SIGS = {
     "default_params": (None, [C.POINTER(Params)]),
@@ -134,7 +152,17 @@ _SIGS = {
     "get_scaled_A": (C.c_int, [C.c_void_p, _f64p]),
     "get_dims": (C.c_int, [C.c_void_p, _i32p, _i32p, C.POINTER(C.c_int64)]),
     "spmv": (C.c_int, [C.c_void_p, C.c_int32, _f64p, _f64p]),
+    "init_restart": (None, [C.POINTER(RestartInfo)]),
+    "check_restart": (None, [C.POINTER(RestartInfo), C.c_int64, C.c_int64, C.c_double, C.c_double]),
+    "update_sigma": (C.c_int, [C.c_void_p, C.POINTER(RestartInfo), C.POINTER(Residuals)]),
+    "restart": (C.c_int, [C.c_void_p, C.POINTER(RestartInfo)]),
 }
+
+# Entry points of the HIP backend only (the oracle runs its loop in one piece).
+_LOOP_SIGS = {
+    "iterate": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(Result)]),
+}
+ITERATE_RESET = 1
 
 # Entry points every library exporting this ABI must provide.
 ABI_FUNCTIONS = sorted(_SIGS)
@@ -164,7 +192,7 @@ _HOOK_SIGS = {
 
 
 def bind_hooks(lib: C.CDLL, prefix: str = "hprsocp_"):
-    for name, (res, args) in _HOOK_SIGS.items():
+    for name, (res, args) in list(_HOOK_SIGS.items()) + list(_LOOP_SIGS.items()):
         fn = getattr(lib, prefix + name)
         fn.restype = res
         fn.argtypes = args

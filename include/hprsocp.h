@@ -212,6 +212,23 @@ typedef enum {
     HPRSOCP_V_COUNT
 } hprsocp_vector_id;
 
+/* Restart bookkeeping, HPRSOCP_restart (reference src/structs.jl:600-623).
+ * Field order and types follow the Julia mutable struct (Int = int64_t,
+ * Bool = uint8_t), so a Julia host passes its own restart_info object by
+ * reference.  Owned by the caller of the fine-grained seams below. */
+typedef struct {
+    int64_t restart_flag;      /* set by check_restart: 0 none, 1 sufficient or
+                                  first restart, 2 necessary, 3 long          */
+    uint8_t first_restart;     /* true until the check at iter == check_iter  */
+    double last_gap, current_gap, save_gap;
+    int64_t inner;             /* iterations since the last restart           */
+    int64_t step, sufficient, necessary, long_restarts, ratio, times;
+    double weighted_norm, best_gap, best_kkt, best_sigma;
+    int64_t best_iter;
+    uint8_t sigma_correction_active;
+    int64_t sigma_correction_hold, sigma_correction_dir;
+} hprsocp_restart_info;
+
 /* Scalar block of a handle. */
 typedef struct {
     double sigma, lambda_max_A, lambda_max_Q;
@@ -258,6 +275,40 @@ int hprsocp_mnorm(hprsocp_handle *h, double *out);
 /* The whole solve loop (algorithm.jl:2765-2905) from the current state;
  * requires hprsocp_estimate_lambda first. */
 int hprsocp_solve(hprsocp_handle *h, hprsocp_result *res);
+
+/* Flags of hprsocp_iterate. */
+#define HPRSOCP_ITERATE_RESET 1   /* start a new solve loop from the current state */
+
+/* A window of the solve loop (algorithm.jl:2806-2904): runs the loop bodies of
+ * iterations iter .. iter+k-1 of the solve that the handle holds, with the
+ * residual checks, restart tests, sigma updates and restarts of the reference
+ * loop, and returns after them (or at termination).  The runs of plain partial
+ * updates inside the window replay captured HIP graphs.  HPRSOCP_ITERATE_RESET
+ * (or a first call, or a call after termination) starts a new loop from the
+ * current state, exactly as hprsocp_solve does; hprsocp_solve is
+ * hprsocp_iterate(h, max_iter + 1, HPRSOCP_ITERATE_RESET, res).  *res receives
+ * the loop's result so far; res->status is HPRSOCP_STATUS_CONTINUE until the
+ * loop ends.  Calls in windows of any length give the same iterates, bit for
+ * bit, as one hprsocp_solve. */
+int hprsocp_iterate(hprsocp_handle *h, int64_t k, int32_t flags, hprsocp_result *res);
+
+/* initialize_restart (algorithm.jl:1886-1908). Host only. */
+void hprsocp_init_restart(hprsocp_restart_info *r);
+/* check_restart (algorithm.jl:1016-1099): restart tests at iteration `iter`
+ * from the gaps in *r and the current KKT. Host only. */
+void hprsocp_check_restart(hprsocp_restart_info *r, int64_t iter, int64_t check_iter, double sigma,
+                           double kkt);
+/* update_sigma! (algorithm.jl:729-990): when r->restart_flag >= 1, the model
+ * proposal from the device dots of (x_bar - last_x, y_bar - last_y,
+ * w_bar - last_w) and their A^T / Q images, blended with r->best_sigma and
+ * corrected by the residual imbalance of *res; updates the handle's sigma (and
+ * the diagonal-Q factors), r->best_iter and the sigma-correction state. */
+int hprsocp_update_sigma(hprsocp_handle *h, hprsocp_restart_info *r, const hprsocp_residuals *res);
+/* do_restart! (algorithm.jl:1130-1166): when r->restart_flag > 0, moves the
+ * iterates and the Halpern anchors to the averaged point (x, last_x <- x_bar;
+ * w, last_w <- w_bar; y, last_y <- y_bar; ATy, last_ATy, ATy_bar <- A^T y_bar
+ * for QP) and updates the gaps and counters of *r. */
+int hprsocp_restart(hprsocp_handle *h, hprsocp_restart_info *r);
 /* Unscaled solution, collect_results! (algorithm.jl:1222-1256).
  * Any pointer may be NULL.  Sizes n, m, n, n. */
 int hprsocp_get_solution(hprsocp_handle *h, double *x, double *y, double *z, double *w);

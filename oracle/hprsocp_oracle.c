@@ -66,6 +66,7 @@ int64_t oracle_abi_sizeof(int32_t which) {
     case 2: return (int64_t)sizeof(hprsocp_result);
     case 3: return (int64_t)sizeof(hprsocp_residuals);
     case 4: return (int64_t)sizeof(hprsocp_scalars);
+    case 5: return (int64_t)sizeof(hprsocp_restart_info);
     default: return -1;
     }
 }
@@ -1262,14 +1263,8 @@ int oracle_mnorm(oracle_ws *w, double *out) {
 }
 
 /* ------------------------------------------------------ restart / sigma */
-typedef struct { /* HPRSOCP_restart, structs.jl:600-623 */
-    int restart_flag, first_restart;
-    double last_gap, current_gap, save_gap;
-    int64_t inner, times, sufficient, necessary, longr;
-    double weighted_norm, best_gap, best_kkt, best_sigma;
-    int64_t best_iter;
-    int corr_active, corr_hold, corr_dir;
-} restart_t;
+/* HPRSOCP_restart (structs.jl:600-623) is the public hprsocp_restart_info. */
+typedef hprsocp_restart_info restart_t;
 
 /* initialize_restart, algorithm.jl:1886-1908. */
 static void init_restart(restart_t *r) {
@@ -1316,7 +1311,7 @@ static void check_restart(restart_t *r, int64_t iter, int64_t check_iter, double
     if (((double)r->inner >= jmax(500.0, fact * (double)iter) ||
          (r->inner >= 10000 && r->current_gap <= 0.75 * r->last_gap)) &&
         r->current_gap > 0.9 * r->save_gap) {
-        r->longr++;
+        r->long_restarts++;
         r->restart_flag = 3;
     }
     if (refresh_best(r->current_gap, r->best_gap, kkt, r->best_kkt, r->best_iter)) {
@@ -1452,37 +1447,37 @@ static void update_sigma(oracle_ws *w, restart_t *r, const hprsocp_residuals *re
     int keep = !both && ((es <= 0.8 * THR) || (ir >= 0.1 * IMB));
     int cdir = ed >= ep ? 1 : -1;
     if (both) {
-        r->corr_active = 0;
-        r->corr_hold = -1;
-        r->corr_dir = 0;
-    } else if (trig || (r->corr_active && keep)) {
-        r->corr_active = 1;
-        r->corr_hold = 0;
-        r->corr_dir = cdir;
+        r->sigma_correction_active = 0;
+        r->sigma_correction_hold = -1;
+        r->sigma_correction_dir = 0;
+    } else if (trig || (r->sigma_correction_active && keep)) {
+        r->sigma_correction_active = 1;
+        r->sigma_correction_hold = 0;
+        r->sigma_correction_dir = cdir;
     } else {
-        r->corr_active = 0;
-        r->corr_hold = -1;
-        r->corr_dir = 0;
+        r->sigma_correction_active = 0;
+        r->sigma_correction_hold = -1;
+        r->sigma_correction_dir = 0;
     }
     double beta = 0.0;
     if (!both) {
         if (one) beta = 0.5;
         if (extreme) beta = 1.0;
-        else if (r->corr_active) beta = 0.5;
+        else if (r->sigma_correction_active) beta = 0.5;
     }
     double kappa = 1.0;
     if (beta > 0.0) {
         double rk = ed / ep;
-        if (r->corr_active) {
-            if (r->corr_dir > 0) rk = jmax(rk, FLOOR);
-            else if (r->corr_dir < 0) rk = jmin(rk, 1.0 / FLOOR);
+        if (r->sigma_correction_active) {
+            if (r->sigma_correction_dir > 0) rk = jmax(rk, FLOOR);
+            else if (r->sigma_correction_dir < 0) rk = jmin(rk, 1.0 / FLOOR);
         }
         kappa = exp(beta * jclamp(log(rk), -LOG100, LOG100));
     }
     double sn = sbase * kappa;
-    if (r->corr_active) {
-        if (r->corr_dir > 0) sn = jmax(sn, sigma_old * 1.5);
-        else if (r->corr_dir < 0) sn = jmin(sn, sigma_old / 1.5);
+    if (r->sigma_correction_active) {
+        if (r->sigma_correction_dir > 0) sn = jmax(sn, sigma_old * 1.5);
+        else if (r->sigma_correction_dir < 0) sn = jmin(sn, sigma_old / 1.5);
     }
     sn = jclamp(sn, SMIN, SMAX);
     w->sigma = sn;
@@ -1513,6 +1508,24 @@ static void do_restart(oracle_ws *w, restart_t *r) {
     r->save_gap = INFINITY;
     r->times += 1;
     r->inner = 0;
+}
+
+/* Seams of the solve loop for known-answer tests, same meaning as the
+ * hprsocp_ entry points of the same names. */
+void oracle_init_restart(hprsocp_restart_info *r) { init_restart(r); }
+
+void oracle_check_restart(hprsocp_restart_info *r, int64_t iter, int64_t check_iter, double sigma, double kkt) {
+    if (check_iter > 0) check_restart(r, iter, check_iter, sigma, kkt);
+}
+
+int oracle_update_sigma(oracle_ws *w, hprsocp_restart_info *r, const hprsocp_residuals *res) {
+    update_sigma(w, r, res);
+    return 0;
+}
+
+int oracle_restart(oracle_ws *w, hprsocp_restart_info *r) {
+    do_restart(w, r);
+    return 0;
 }
 
 /* print_step, algorithm.jl:1910-1912. */

@@ -39,6 +39,14 @@ int oracle_get_scaled_A(oracle_ws *w, double *val);
 int oracle_get_dims(oracle_ws *w, int32_t *m, int32_t *n, int64_t *nnz);
 int oracle_spmv(oracle_ws *w, int32_t trans, const double *in, double *out);
 
+/* Seams of the solve loop (initialize_restart, check_restart, update_sigma!,
+ * do_restart!; algorithm.jl:1886-1908, 1016-1099, 729-990, 1130-1166). */
+void oracle_init_restart(hprsocp_restart_info *r);
+void oracle_check_restart(hprsocp_restart_info *r, int64_t iter, int64_t check_iter, double sigma,
+                          double kkt);
+int oracle_update_sigma(oracle_ws *w, hprsocp_restart_info *r, const hprsocp_residuals *res);
+int oracle_restart(oracle_ws *w, hprsocp_restart_info *r);
+
 /* Pieces exposed for known-answer tests of single functions. */
 /* Pi_Q(t, xi) in place on v[0..len) (kernels.jl:7585-7662 restated). */
 void oracle_soc_project(double *v, int32_t len);
@@ -51,7 +59,8 @@ double oracle_golden(double a, double b, double c, double d, double lo, double h
 /* Deterministic power-iteration start vector (standard normals + 1e-8). */
 void oracle_start_vector(int64_t len, double *out);
 /* sizeof of the ABI structs as this C compiler lays them out
- * (0 problem, 1 params, 2 result, 3 residuals, 4 scalars); -1 if unknown. */
+ * (0 problem, 1 params, 2 result, 3 residuals, 4 scalars, 5 restart info);
+ * -1 if unknown. */
 int64_t oracle_abi_sizeof(int32_t which);
 /* Thread count used by the parallel loops (1 = sequential). */
 void oracle_set_threads(int32_t nthreads);

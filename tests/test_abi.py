@@ -25,6 +25,28 @@ def test_header_declares_abi_functions():
     for fn in _abi.ABI_FUNCTIONS:
         assert "hprsocp_" + fn in names
     assert "hprsocp_time_iterations" in names
+    for fn in _abi._LOOP_SIGS:
+        assert "hprsocp_" + fn in names
+
+
+def test_restart_info_layout_matches_julia():
+    """hprsocp_restart_info has the C layout of the reference's HPRSOCP_restart
+    (structs.jl:600-623; Int = 8 bytes, Bool = 1 byte, C alignment), so a Julia
+    host passes its own restart_info by reference."""
+    julia_fields = ["restart_flag", "first_restart", "last_gap", "current_gap", "save_gap", "inner",
+                    "step", "sufficient", "necessary", "long", "ratio", "times", "weighted_norm",
+                    "best_gap", "best_kkt", "best_sigma", "best_iter", "sigma_correction_active",
+                    "sigma_correction_hold", "sigma_correction_dir"]
+    bools = {"first_restart", "sigma_correction_active"}
+    off = 0
+    for (name, ctype), jname in zip(_abi.RestartInfo._fields_, julia_fields):
+        assert name == (jname if jname != "long" else "long_restarts")
+        size = 1 if jname in bools else 8
+        off = (off + size - 1) // size * size
+        assert getattr(_abi.RestartInfo, name).offset == off, name
+        assert C.sizeof(ctype) == size, name
+        off += size
+    assert C.sizeof(_abi.RestartInfo) == (off + 7) // 8 * 8
 
 
 def test_library_exports_every_declared_symbol():
@@ -61,7 +83,8 @@ def test_struct_sizes(oracle_lib):
     fn = oracle_lib.oracle_abi_sizeof
     fn.restype = C.c_int64
     fn.argtypes = [C.c_int32]
-    for k, st in enumerate([_abi.Problem, _abi.Params, _abi.Result, _abi.Residuals, _abi.Scalars]):
+    for k, st in enumerate([_abi.Problem, _abi.Params, _abi.Result, _abi.Residuals, _abi.Scalars,
+                            _abi.RestartInfo]):
         assert C.sizeof(st) == fn(k), st.__name__
 
 
