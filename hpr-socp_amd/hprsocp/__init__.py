@@ -458,6 +458,18 @@ class Solver:
         return ms.value
 
 
+def project_cones(inp: int, out: int, n: int, cone_idx: int, ncones: int, stream: int = 0,
+                  lib: Optional[C.CDLL] = None) -> None:
+    """hprsocp_project_cones on device pointers (ints, e.g. torch tensors'
+    data_ptr()): rows [cone_idx[k], cone_idx[k+1]) of ``inp`` projected onto the
+    second-order cone into ``out``; waits for ``stream``."""
+    lib = lib or load_library()
+    rc = lib.hprsocp_project_cones(C.c_void_p(inp), C.c_void_p(out), int(n), C.c_void_p(cone_idx), int(ncones),
+                                   C.c_void_p(stream or None))
+    if rc != 0:
+        raise HPRSOCPError(rc, lib.hprsocp_project_cones_last_error().decode())
+
+
 def optimize(model: QP_info, params: Optional[HPRSOCP_parameters] = None) -> HPRSOCP_results:
     """optimize(model, params) of the reference (algorithm.jl:2538): optional
     200-iteration warm-up, then setup + solve on the GPU; returns unscaled x, y, z, w."""

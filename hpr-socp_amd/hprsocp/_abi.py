@@ -182,6 +182,13 @@ _BATCH_SIGS = {
 }
 
 
+# Stand-alone device-pointer kernels (HIP backend only).
+_KERNEL_SIGS = {
+    "project_cones": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p]),
+    "project_cones_last_error": (C.c_char_p, []),
+}
+
+
 # Measurement hooks (HIP backend only).
 _HOOK_SIGS = {
     "time_iterations": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), _f64p]),
@@ -192,7 +199,7 @@ _HOOK_SIGS = {
 
 
 def bind_hooks(lib: C.CDLL, prefix: str = "hprsocp_"):
-    for name, (res, args) in list(_HOOK_SIGS.items()) + list(_LOOP_SIGS.items()):
+    for name, (res, args) in list(_HOOK_SIGS.items()) + list(_LOOP_SIGS.items()) + list(_KERNEL_SIGS.items()):
         fn = getattr(lib, prefix + name)
         fn.restype = res
         fn.argtypes = args

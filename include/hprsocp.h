@@ -372,6 +372,26 @@ int hprsocp_time_spmv(hprsocp_handle *h, int32_t trans, int32_t reps, double *ms
 int hprsocp_probe_gather(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t slices, int32_t sorted,
                          double *ms);
 
+/* ------------------------------------------------------------ cone projection */
+/* Stand-alone projection onto a product of second-order cones, on device
+ * memory: for each k < ncones, rows [cone_idx[k], cone_idx[k+1]) of `in` are
+ * read as (t, xi) and written to the same rows of `out` as Pi_Q(t, xi): zero
+ * when |xi| <= -t, unchanged when |xi| <= t, else ((t+|xi|)/2)(1, xi/|xi|)
+ * (the reference's projection, kernels.jl:7585-7663, as used by the variable
+ * and constraint cone updates, :7748-7811 and :7895-7951).  Rows outside every
+ * cone are not touched.  in, out (n doubles, in == out allowed) and cone_idx
+ * (ncones + 1 int32, 0-based, nondecreasing, first >= 0, last <= n) are device
+ * pointers; the work is queued on `stream` (a hipStream_t; NULL = the default
+ * stream).  Long cones are split over several workgroups; the per-tile partial
+ * norms live in a scratch buffer that this call allocates on the stream and
+ * frees again.  The call waits for the stream, and returns
+ * HPRSOCP_ERR_INVALID (out untouched) for cone offsets that break the rules
+ * above.  Thread-safe; the text of the last failure on the calling thread is
+ * available from hprsocp_project_cones_last_error(). */
+int hprsocp_project_cones(const double *in, double *out, int64_t n, const int32_t *cone_idx, int32_t ncones,
+                          void *stream);
+const char *hprsocp_project_cones_last_error(void);
+
 /* ------------------------------------------------------------ CBF input */
 /* Reads a Conic Benchmark Format file into the canonical layout, as read_cbf
  * + build_from_cbf do (reference src/utils/cbf_io.jl:493-750, 899-1131,
