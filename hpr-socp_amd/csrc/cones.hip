@@ -14,7 +14,11 @@
 //   k_cone_finish   one workgroup per tile: |xi|^2 of its cone as the fixed-order
 //                   sum of the cone's partials, then projects its rows.
 // The scratch (tile offsets, partial sums, t values, a status word) is device
-// memory beyond the input and output.
+// memory beyond the input and output: allocated per call by
+// hprsocp_project_cones, a fixed file-scope device array in
+// hprsocp_project_cones_noalloc.  Every launch writes all of the scratch it
+// reads (offsets and status in k_cone_tiles, partials and t in k_cone_partial),
+// so nothing needs clearing between calls.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -166,6 +170,21 @@ hipError_t launch_cones(const double *in, double *out, int64_t n, const int32_t 
     return hipGetLastError();
 }
 
+// ---- the allocation-free entry point's scratch
+// Limits of hprsocp_project_cones_noalloc: cones per call, and tiles per call
+// (tiles = ceil(n / kConeTile) + ncones bounds the tiles a call can have).
+constexpr int32_t kNoAllocCones = 65536;
+constexpr int64_t kNoAllocTiles = 131072;
+constexpr size_t kNoAllocBytes = (size_t)kNoAllocTiles * 8 + (size_t)kNoAllocCones * 8 + ((size_t)kNoAllocCones + 1) * 4 + 4;
+// One copy per GPU, held until the process ends; calls on one GPU share it.
+static __device__ __align__(256) double scratch[(kNoAllocBytes + 7) / 8];
+
+// Device address of `scratch` on each device (the host-side name is only a
+// stand-in), looked up once per device with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
 thread_local std::string g_cone_error;
 
 int cone_fail(int code, const std::string &msg) {
@@ -212,6 +231,42 @@ int hprsocp_project_cones(const double *in, double *out, int64_t n, const int32_
     if (flag)
         return cone_fail(HPRSOCP_ERR_INVALID,
                          "project_cones: cone_idx must start at >= 0, never decrease and end at <= n");
+    return HPRSOCP_OK;
+}
+
+int hprsocp_project_cones_noalloc(const double *in, double *out, int64_t n, const int32_t *cone_idx,
+                                  int32_t ncones, void *stream) {
+    int rc = check_args(in, out, n, cone_idx, ncones);
+    if (rc != HPRSOCP_OK || ncones == 0) return rc;
+    if (ncones > kNoAllocCones)
+        return cone_fail(HPRSOCP_ERR_INVALID, "project_cones_noalloc: ncones " + std::to_string(ncones) +
+                                                  " exceeds the limit " + std::to_string(kNoAllocCones));
+    const int64_t tiles = tile_bound(n, ncones);
+    if (tiles > kNoAllocTiles)
+        return cone_fail(HPRSOCP_ERR_INVALID, "project_cones_noalloc: ceil(n / " + std::to_string(kConeTile) +
+                                                  ") + ncones = " + std::to_string(tiles) + " exceeds the limit " +
+                                                  std::to_string(kNoAllocTiles));
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return cone_fail(HPRSOCP_ERR_DEVICE, "project_cones_noalloc: no usable current device");
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void *ptr = nullptr;
+            e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr)
+                return cone_fail(HPRSOCP_ERR_DEVICE, std::string("project_cones_noalloc: hipGetSymbolAddress: ") +
+                                                         hipGetErrorString(e));
+            g_scratch_ptr[dev] = ptr;
+        }
+        base = g_scratch_ptr[dev];
+    }
+    const ConeScratch s = carve(base, kNoAllocTiles, kNoAllocCones);
+    e = launch_cones(in, out, n, cone_idx, ncones, tiles, s, (hipStream_t)stream);
+    if (e != hipSuccess)
+        return cone_fail(HPRSOCP_ERR_DEVICE, std::string("project_cones_noalloc: ") + hipGetErrorString(e));
     return HPRSOCP_OK;
 }
 

@@ -459,13 +459,14 @@ class Solver:
 
 
 def project_cones(inp: int, out: int, n: int, cone_idx: int, ncones: int, stream: int = 0,
-                  lib: Optional[C.CDLL] = None) -> None:
+                  lib: Optional[C.CDLL] = None, noalloc: bool = False) -> None:
     """hprsocp_project_cones on device pointers (ints, e.g. torch tensors'
     data_ptr()): rows [cone_idx[k], cone_idx[k+1]) of ``inp`` projected onto the
-    second-order cone into ``out``; waits for ``stream``."""
+    second-order cone into ``out``; waits for ``stream``.  ``noalloc`` calls
+    hprsocp_project_cones_noalloc instead (no allocation, no wait)."""
     lib = lib or load_library()
-    rc = lib.hprsocp_project_cones(C.c_void_p(inp), C.c_void_p(out), int(n), C.c_void_p(cone_idx), int(ncones),
-                                   C.c_void_p(stream or None))
+    fn = lib.hprsocp_project_cones_noalloc if noalloc else lib.hprsocp_project_cones
+    rc = fn(C.c_void_p(inp), C.c_void_p(out), int(n), C.c_void_p(cone_idx), int(ncones), C.c_void_p(stream or None))
     if rc != 0:
         raise HPRSOCPError(rc, lib.hprsocp_project_cones_last_error().decode())
 

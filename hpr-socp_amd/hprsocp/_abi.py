@@ -186,6 +186,7 @@ _BATCH_SIGS = {
 _KERNEL_SIGS = {
     "project_cones": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p]),
     "project_cones_last_error": (C.c_char_p, []),
+    "project_cones_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p]),
 }
 
 

@@ -391,6 +391,21 @@ int hprsocp_probe_gather(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t 
 int hprsocp_project_cones(const double *in, double *out, int64_t n, const int32_t *cone_idx, int32_t ncones,
                           void *stream);
 const char *hprsocp_project_cones_last_error(void);
+/* hprsocp_project_cones without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments and results; the scratch is a fixed device array of the library
+ * (about 1.75 MiB per GPU, held until the process ends), so a call is limited
+ * to ncones <= 65536 and ceil(n / 2048) + ncones <= 131072 (larger calls
+ * return HPRSOCP_ERR_INVALID before queuing anything).  The call only queues
+ * the work on `stream` and returns without waiting; cone offsets that break
+ * the rules of hprsocp_project_cones leave `out` untouched, with no error
+ * code (none can be read back without waiting).  Every call on one GPU uses
+ * the same scratch: calls that overlap on one GPU (different streams, or
+ * threads) race and give wrong results; calls queued on one stream are
+ * ordered and safe. */
+int hprsocp_project_cones_noalloc(const double *in, double *out, int64_t n, const int32_t *cone_idx,
+                                  int32_t ncones, void *stream);
 
 /* ------------------------------------------------------------ CBF input */
 /* Reads a Conic Benchmark Format file into the canonical layout, as read_cbf
