@@ -226,14 +226,17 @@ void launch_ddual(hipStream_t st, const DevPlan &plan, const CsrT<float> &A, con
                   const float *v, const DDualArgs &a, double *raw_buf, double *partials);
 
 // Gather-ceiling probe: synthetic entries shaped like the sliced SpMV's
-// (8-byte value + packed column/position word, one gather each from the slice
-// of block b % slices).  probe_fill writes nch chunks of kStreamNnz (random
-// columns) or kSortChunk (column-sorted chunks) entries; probe_gather runs
-// the gather phase of k_spmv_slices (random) or k_spmv_sorted (sorted) on them
-// and writes per-thread sums to out (nch x threads per block).
-void launch_probe_fill(hipStream_t st, int64_t nch, int32_t width, int sorted, uint32_t *pk, double *val);
+// (T value + packed column/position word, one gather of a T each from the
+// slice of block b % slices).  probe_fill writes nch chunks of kStreamNnz
+// (random columns) or SortSpec<T>::chunk (column-sorted chunks) entries;
+// probe_gather runs the gather phase of k_spmv_slices (random) or
+// k_spmv_sorted (sorted) on them and writes per-thread sums to out (nch x
+// threads per block).
+template <typename T>
+void launch_probe_fill(hipStream_t st, int64_t nch, int32_t width, int sorted, uint32_t *pk, T *val);
+template <typename T>
 void launch_probe_gather(hipStream_t st, int64_t nch, int32_t slices, int32_t width, int sorted, const uint32_t *pk,
-                         const double *val, const double *x, double *out);
+                         const T *val, const T *x, T *out);
 constexpr int kProbePosBits = 11;  // log2(kStreamNnz): position bits of the random-order probe
 
 }  // namespace hpr

@@ -711,9 +711,10 @@ __device__ __forceinline__ uint64_t probe_hash(uint64_t z) {  // SplitMix64 fina
 // over the wavefront; every wavefront of the launch is whole because n and the
 // grid stride are multiples of 64) — with positions an odd-multiplier
 // permutation of j.
-__global__ void k_probe_fill(int64_t n, int32_t width, int sorted, uint32_t *pk, double *val) {
-    const int C = sorted ? kSortChunk : kStreamNnz;
-    const int bits = sorted ? kSortPosBits : kProbePosBits;
+template <typename T>
+__global__ void k_probe_fill(int64_t n, int32_t width, int sorted, uint32_t *pk, T *val) {
+    const int C = sorted ? SortSpec<T>::chunk : kStreamNnz;
+    const int bits = sorted ? SortSpec<T>::pos_bits : kProbePosBits;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t j = (uint32_t)(k % C);
         const uint64_t r = probe_hash((uint64_t)k);
@@ -736,46 +737,48 @@ __global__ void k_probe_fill(int64_t n, int32_t width, int sorted, uint32_t *pk,
             pos = j;
         }
         pk[k] = (col << bits) | pos;
-        val[k] = 1.0;
+        val[k] = (T)1;
     }
 }
 
-template <int C, int NT, int BITS>
+template <typename T, int C, int NT, int BITS>
 __global__ __launch_bounds__(NT) void k_probe_gather(int32_t slices, int32_t width, const uint32_t *__restrict__ pk,
-                                                     const double *__restrict__ val, const double *__restrict__ x,
-                                                     double *__restrict__ out) {
-    __shared__ double prod[C];
+                                                     const T *__restrict__ val, const T *__restrict__ x,
+                                                     T *__restrict__ out) {
+    __shared__ T prod[C];
     const int64_t b = blockIdx.x;
-    const double *__restrict__ xs = x + (int64_t)(b % slices) * width;
+    const T *__restrict__ xs = x + (int64_t)(b % slices) * width;
     const int64_t base = b * (int64_t)C;
     constexpr uint32_t mask = (1u << BITS) - 1;
 #pragma unroll 4
     for (int k = threadIdx.x; k < C; k += NT) {
         const uint32_t w = __builtin_nontemporal_load(pk + base + k);
-        const double v = __builtin_nontemporal_load(val + base + k);
+        const T v = __builtin_nontemporal_load(val + base + k);
         prod[w & mask] = v * xs[w >> BITS];
     }
     __syncthreads();
-    double s = 0.0;
+    T s = 0;
     for (int k = 0; k < C / NT; ++k) s += prod[threadIdx.x * (C / NT) + k];
     out[b * NT + threadIdx.x] = s;
 }
 
-void launch_probe_fill(hipStream_t st, int64_t nch, int32_t width, int sorted, uint32_t *pk, double *val) {
-    const int64_t n = nch * (sorted ? kSortChunk : kStreamNnz);
+template <typename T>
+void launch_probe_fill(hipStream_t st, int64_t nch, int32_t width, int sorted, uint32_t *pk, T *val) {
+    const int64_t n = nch * (sorted ? SortSpec<T>::chunk : kStreamNnz);
     if (n <= 0) return;
     const int64_t blocks = std::min<int64_t>((n + kBlock - 1) / kBlock, 65536);
-    hipLaunchKernelGGL(k_probe_fill, dim3((unsigned)blocks), dim3(kBlock), 0, st, n, width, sorted, pk, val);
+    hipLaunchKernelGGL(k_probe_fill<T>, dim3((unsigned)blocks), dim3(kBlock), 0, st, n, width, sorted, pk, val);
 }
 
+template <typename T>
 void launch_probe_gather(hipStream_t st, int64_t nch, int32_t slices, int32_t width, int sorted, const uint32_t *pk,
-                         const double *val, const double *x, double *out) {
+                         const T *val, const T *x, T *out) {
     if (nch <= 0) return;
     if (sorted)
-        hipLaunchKernelGGL((k_probe_gather<kSortChunk, kSortBlock, kSortPosBits>), dim3((unsigned)nch),
-                           dim3(kSortBlock), 0, st, slices, width, pk, val, x, out);
+        hipLaunchKernelGGL((k_probe_gather<T, SortSpec<T>::chunk, kSortBlock, SortSpec<T>::pos_bits>),
+                           dim3((unsigned)nch), dim3(kSortBlock), 0, st, slices, width, pk, val, x, out);
     else
-        hipLaunchKernelGGL((k_probe_gather<kStreamNnz, kBlock, kProbePosBits>), dim3((unsigned)nch), dim3(kBlock),
+        hipLaunchKernelGGL((k_probe_gather<T, kStreamNnz, kBlock, kProbePosBits>), dim3((unsigned)nch), dim3(kBlock),
                            0, st, slices, width, pk, val, x, out);
 }
 
@@ -790,7 +793,10 @@ void launch_probe_gather(hipStream_t st, int64_t nch, int32_t slices, int32_t wi
     template void launch_spmv<T>(hipStream_t, const DevPlan &, const CsrT<T> &, const T *, T *, const T *,      \
                                  const T *);                                                                     \
     template void launch_spmv_tempv<T>(hipStream_t, const DevPlan &, const CsrT<T> &, const T *,                \
-                                       const TempvArgsT<T> &);
+                                       const TempvArgsT<T> &);                                                   \
+    template void launch_probe_fill<T>(hipStream_t, int64_t, int32_t, int, uint32_t *, T *);                     \
+    template void launch_probe_gather<T>(hipStream_t, int64_t, int32_t, int32_t, int, const uint32_t *,          \
+                                         const T *, const T *, T *);
 HPR_INSTANTIATE(double)
 HPR_INSTANTIATE(float)
 

@@ -21,6 +21,7 @@
 #include <atomic>
 #include <chrono>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <tuple>
 #include <string>
@@ -365,6 +366,16 @@ struct hprsocp_handle {
     double *d_ra = nullptr, *d_rya = nullptr, *d_img_n = nullptr, *d_img_m = nullptr;
     // the solve loop in progress (hprsocp_iterate)
     SolveLoop *loop = nullptr;
+    // launch-plan choices of the autotuner, one record per operator
+    struct TuneRecord {
+        std::string name, form = "plain";
+        PlanConfig cfg;
+        bool tuned = false, cached = false;
+        int32_t candidates = 0, tiles = 0;
+        float ms_default = 0, ms = 0;    // plain form: default / chosen config
+        float ms_form = 0;               // the chosen form (sliced or plain)
+    };
+    std::vector<TuneRecord> tune;
     // timing (profiled iterations)
     bool profile = false;
     std::vector<hipEvent_t> ev;
@@ -461,8 +472,7 @@ void upload_plan(hprsocp_handle *h, const HostPlan &hp, DevPlan &dp) {
 
 DevPlan make_plan(hprsocp_handle *h, const std::vector<int32_t> &rp, int32_t nrows, int32_t lin_end,
                   const std::vector<int32_t> &cidx, int32_t ncones, const std::vector<int32_t> *row_inst = nullptr,
-                  HostPlan *keep = nullptr) {
-    PlanConfig cfg;
+                  HostPlan *keep = nullptr, const PlanConfig &cfg = PlanConfig{}) {
     const int32_t *ri = (row_inst && h->nb > 1) ? row_inst->data() : nullptr;
     HostPlan hp = build_plan(rp.empty() ? nullptr : rp.data(), nrows, lin_end, cidx.empty() ? nullptr : cidx.data(),
                              ncones, cfg, ri);
@@ -963,6 +973,188 @@ void free_plan(hprsocp_handle *h, const DevPlan &p) {
     h->dfree(p.huge_buf);
 }
 
+// Autotuner of the tile plan (the reference fixes its launch shapes:
+// gpu_launch_config, kernels.jl:349-357, and the cone size classes,
+// algorithm.jl:1914-1919).  For an operator of at least kTuneMinNnz nonzeros
+// (HPRSOCP_TUNE=1: any size, HPRSOCP_TUNE=0: never) it times one SpMV over the
+// plan of each PlanConfig candidate (median of 5 HIP-event timed launches after
+// a warm-up), coordinate-wise around the defaults: nonzeros per tile, lane
+// width divisor, HUGE-segment targets (when the operator has split cones) and
+// the CSR-stream path; then the combination of the per-coordinate winners.  A
+// candidate replaces the current best only when it is at least 2% faster, and
+// candidates are visited in a fixed order, so near-ties keep the earlier one.
+// Results are cached per process by the operator's shape: the log2 histogram
+// of its row lengths (linear rows and cone rows apart), the log2 histogram of
+// its cone sizes and the instance count.  The choice changes only launch shapes
+// (and with them the summation order of the row sums), never the arithmetic.
+constexpr int64_t kTuneMinNnz = 1 << 21;
+
+std::mutex g_tune_mu;
+std::map<std::string, PlanConfig> g_tune_cache;
+
+std::string tune_key(const std::vector<int32_t> &rp, int32_t nrows, int32_t lin_end, const std::vector<int32_t> &cidx,
+                     int32_t ncones, int32_t nb) {
+    std::map<std::tuple<int, int>, int64_t> hist;  // (class, log2 bucket) -> count
+    auto l2 = [](int64_t v) {
+        int b = 0;
+        while ((1ll << b) <= v && b < 62) ++b;
+        return b;
+    };
+    for (int32_t i = 0; i < nrows; ++i) hist[{i < lin_end ? 0 : 1, l2(rp[i + 1] - rp[i])}]++;
+    for (int32_t k = 0; k < ncones; ++k) hist[{2, l2(cidx[k + 1] - cidx[k])}]++;
+    std::string key = std::to_string(nrows) + "/" + std::to_string(lin_end) + "/" + std::to_string(ncones) + "/" +
+                      std::to_string(nb > 1 ? nb : 1) + ":";
+    for (const auto &kv : hist)
+        key += std::to_string(std::get<0>(kv.first)) + "." + std::to_string(std::get<1>(kv.first)) + "=" +
+               std::to_string(kv.second) + ";";
+    return key;
+}
+
+PlanConfig tune_plan_config(hprsocp_handle *h, const CsrView &M, const std::vector<int32_t> &rp, int32_t nrows,
+                            int32_t ncols, int32_t lin_end, const std::vector<int32_t> &cidx, int32_t ncones,
+                            const std::vector<int32_t> *row_inst, const char *name) {
+    hprsocp_handle::TuneRecord rec;
+    rec.name = name;
+    PlanConfig best;
+    const int64_t nnz = rp.empty() ? 0 : rp[nrows];
+    const char *env = getenv("HPRSOCP_TUNE");
+    const bool force = env && atoi(env) == 1, off = env && atoi(env) == 0;
+    const int32_t *ri = (row_inst && h->nb > 1) ? row_inst->data() : nullptr;
+    auto plan_of = [&](const PlanConfig &c) {
+        return build_plan(rp.data(), nrows, lin_end, cidx.empty() ? nullptr : cidx.data(), ncones, c, ri);
+    };
+    if (off || nnz == 0 || (!force && nnz < kTuneMinNnz)) {
+        rec.cfg = best;
+        rec.tiles = (int32_t)plan_of(best).tiles.size();
+        h->tune.push_back(rec);
+        return best;
+    }
+    const std::string key = tune_key(rp, nrows, lin_end, cidx, ncones, h->nb);
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            rec.cfg = it->second;
+            rec.tuned = rec.cached = true;
+            rec.tiles = (int32_t)plan_of(rec.cfg).tiles.size();
+            h->tune.push_back(rec);
+            return rec.cfg;
+        }
+    }
+    double *xin = h->dalloc<double>(ncols);
+    double *yout = h->dalloc<double>(nrows);
+    launch_fill(h->st, ncols, 1.0, xin);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto time_cfg = [&](const PlanConfig &c) {
+        HostPlan hp = plan_of(c);
+        DevPlan dp;
+        upload_plan(h, hp, dp);
+        std::vector<float> t;
+        for (int rep = 0; rep < 6; ++rep) {
+            HIPCHK(hipEventRecord(e0, h->st));
+            launch_spmv(h->st, dp, M, xin, yout);
+            HIPCHK(hipEventRecord(e1, h->st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep) t.push_back(ms);
+        }
+        free_plan(h, dp);
+        std::sort(t.begin(), t.end());
+        ++rec.candidates;
+        return t[t.size() / 2];
+    };
+    const PlanConfig def;
+    const float t_def = time_cfg(def);
+    float t_best = t_def;
+    bool has_huge = false;
+    for (int32_t k = 0; k < ncones && !has_huge; ++k) has_huge = cidx[k + 1] - cidx[k] > kTileRows;
+    // coordinate candidates; each coordinate remembers its own winner
+    struct Coord {
+        std::vector<PlanConfig> alts;
+        PlanConfig win;
+        float t;
+        bool improved = false;
+    };
+    std::vector<Coord> coords;
+    auto with = [&](auto setter) {
+        PlanConfig c = def;
+        setter(c);
+        return c;
+    };
+    coords.push_back(Coord{{with([](PlanConfig &c) { c.nnz_target = 1024; }),
+                            with([](PlanConfig &c) { c.nnz_target = 4096; })}, def, t_def});
+    coords.push_back(Coord{{with([](PlanConfig &c) { c.g_divisor = 1; }),
+                            with([](PlanConfig &c) { c.g_divisor = 4; })}, def, t_def});
+    if (has_huge)
+        coords.push_back(Coord{{with([](PlanConfig &c) { c.huge_target = 8192; c.huge_whole = 4096; }),
+                                with([](PlanConfig &c) { c.huge_target = 32768; c.huge_whole = 16384; })},
+                               def, t_def});
+    coords.push_back(Coord{{with([](PlanConfig &c) { c.stream = false; })}, def, t_def});
+    for (Coord &co : coords)
+        for (const PlanConfig &c : co.alts) {
+            const float t = time_cfg(c);
+            if (t < 0.98f * co.t) {
+                co.t = t;
+                co.win = c;
+                co.improved = true;
+            }
+            if (t < 0.98f * t_best) {
+                t_best = t;
+                best = c;
+            }
+        }
+    int nimp = 0;
+    PlanConfig comb = def;
+    for (size_t k = 0; k < coords.size(); ++k) {
+        if (!coords[k].improved) continue;
+        ++nimp;
+        const PlanConfig &w = coords[k].win;
+        if (w.nnz_target != def.nnz_target) comb.nnz_target = w.nnz_target;
+        if (w.g_divisor != def.g_divisor) comb.g_divisor = w.g_divisor;
+        if (w.huge_target != def.huge_target) {
+            comb.huge_target = w.huge_target;
+            comb.huge_whole = w.huge_whole;
+        }
+        if (w.stream != def.stream) comb.stream = w.stream;
+    }
+    if (nimp >= 2) {
+        const float t = time_cfg(comb);
+        if (t < 0.98f * t_best) {
+            t_best = t;
+            best = comb;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->dfree(xin);
+    h->dfree(yout);
+    if (h->params.verbose)
+        printf("plan autotune %s: nnz/tile %d, g divisor %d, huge %d/%d, stream %d: %.4f ms (default %.4f ms, %d "
+               "candidates)\n",
+               name, best.nnz_target, best.g_divisor, best.huge_target, best.huge_whole, (int)best.stream, t_best,
+               t_def, rec.candidates);
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tune_cache[key] = best;
+    }
+    rec.cfg = best;
+    rec.tuned = true;
+    rec.ms_default = t_def;
+    rec.ms = rec.ms_form = t_best;
+    rec.tiles = (int32_t)plan_of(best).tiles.size();
+    h->tune.push_back(rec);
+    return best;
+}
+
+hprsocp_handle::TuneRecord *tune_record(hprsocp_handle *h, const char *name) {
+    for (auto &r : h->tune)
+        if (r.name == name) return &r;
+    return nullptr;
+}
+
 // Autotuner of the XCD slicing: for an operator whose input vector outgrows
 // ~2 MB it builds the sliced forms with 4 and 8 column slices (8: one slice per
 // XCD; 4: two XCDs share a slice, halving the partial-sum traffic while the
@@ -1045,6 +1237,12 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
     const bool sliced = forms[pick].sliced;
     const CsrView bestM = forms[pick].M;
     const DevPlan bestP = forms[pick].plan;
+    if (hprsocp_handle::TuneRecord *rec = tune_record(h, name)) {
+        if (sliced)
+            rec->form = std::string(bestM.s_sorted ? "sorted" : "slices") + std::to_string(bestM.nslices);
+        rec->ms_form = forms[pick].t;
+        if (rec->ms == 0) rec->ms = forms[0].t;
+    }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     h->dfree(xin);
@@ -1601,7 +1799,9 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
                     break;
                 }
         h->q_struct_diag = sd;
-        h->plan_Q = make_plan(h, Qh.rp, n, n, {}, 0, &h->h_col_inst, &h->hp_Q);
+        const PlanConfig qcfg = sd ? PlanConfig{}
+                                   : tune_plan_config(h, h->Q, Qh.rp, n, n, n, {}, 0, &h->h_col_inst, "Q");
+        h->plan_Q = make_plan(h, Qh.rp, n, n, {}, 0, &h->h_col_inst, &h->hp_Q, qcfg);
     }
     HostCsr F, FT;
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
@@ -1625,12 +1825,18 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
         h->FT.c0 = contig_c0(h, FT);
         h->Fdiag = p->Q_diag ? h->upload<double>(p->Q_diag, n) : nullptr;
         h->Ftmp = h->dalloc<double>(F.nrows);
-        h->plan_F = make_plan(h, F.rp, F.nrows, F.nrows, {}, 0);
-        h->plan_FT = make_plan(h, FT.rp, n, n, {}, 0);
+        const PlanConfig fcfg = tune_plan_config(h, h->F, F.rp, F.nrows, n, F.nrows, {}, 0, nullptr, "F");
+        const PlanConfig ftcfg = tune_plan_config(h, h->FT, FT.rp, n, F.nrows, n, {}, 0, nullptr, "F^T");
+        h->plan_F = make_plan(h, F.rp, F.nrows, F.nrows, {}, 0, nullptr, nullptr, fcfg);
+        h->plan_FT = make_plan(h, FT.rp, n, n, {}, 0, nullptr, nullptr, ftcfg);
     }
     // plans: A rows (linear | constraint cones), A^T rows (box | variable cones)
-    h->plan_A = make_plan(h, A.rp, m, h->m1, h->h_con_idx, h->ncon, &h->h_row_inst, &h->hp_A);
-    h->plan_AT = make_plan(h, AT.rp, n, h->n1, h->h_var_idx, h->nvar, &h->h_col_inst, &h->hp_AT);
+    const PlanConfig acfg =
+        tune_plan_config(h, h->A, A.rp, m, n, h->m1, h->h_con_idx, h->ncon, &h->h_row_inst, "A");
+    const PlanConfig atcfg =
+        tune_plan_config(h, h->AT, AT.rp, n, m, h->n1, h->h_var_idx, h->nvar, &h->h_col_inst, "A^T");
+    h->plan_A = make_plan(h, A.rp, m, h->m1, h->h_con_idx, h->ncon, &h->h_row_inst, &h->hp_A, acfg);
+    h->plan_AT = make_plan(h, AT.rp, n, h->n1, h->h_var_idx, h->nvar, &h->h_col_inst, &h->hp_AT, atcfg);
     if (h->nb > 1) {
         h->act_A = alloc_active(h, h->plan_A);
         h->act_AT = alloc_active(h, h->plan_AT);
@@ -2247,50 +2453,64 @@ int hprsocp_time_spmv(hprsocp_handle *h, int32_t trans, int32_t reps, double *ms
 // Gather-ceiling probe: nnz synthetic entries (rounded down to whole chunks)
 // gathered from `slices` slices of `width` doubles, in random or column-sorted
 // chunk order; median time of 5 launches after a warm-up.
-int hprsocp_probe_gather(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t slices, int32_t sorted, double *ms) {
-    ABI_TRY(h, {
-        const int C = sorted ? kSortChunk : kStreamNnz;
-        const int bits = sorted ? kSortPosBits : kProbePosBits;
-        if (!ms || slices < 1 || width < 1 || nnz < C || (int64_t)width > (1ll << (32 - bits)))
-            throw Err{HPRSOCP_ERR_INVALID, "probe_gather: need nnz >= one chunk, 1 <= width <= 2^(32-position bits), slices >= 1"};
-        const int NT = sorted ? kSortBlock : kBlock;
-        const int64_t nch = nnz / C;
-        if (nch > (int64_t)INT32_MAX) throw Err{HPRSOCP_ERR_INVALID, "probe_gather: too many chunks"};
-        void *pk = nullptr, *val = nullptr, *x = nullptr, *out = nullptr;
-        auto release = [&]() {
-            (void)hipFree(pk);
-            (void)hipFree(val);
-            (void)hipFree(x);
-            (void)hipFree(out);
-        };
-        if (hipMalloc(&pk, (size_t)(nch * C) * 4) != hipSuccess || hipMalloc(&val, (size_t)(nch * C) * 8) != hipSuccess ||
-            hipMalloc(&x, (size_t)slices * width * 8) != hipSuccess || hipMalloc(&out, (size_t)(nch * NT) * 8) != hipSuccess) {
-            release();
-            throw Err{HPRSOCP_ERR_ALLOC, "probe_gather alloc"};
-        }
-        launch_fill(h->st, (int64_t)slices * width, 1.0, (double *)x);
-        launch_probe_fill(h->st, nch, width, sorted, (uint32_t *)pk, (double *)val);
-        launch_probe_gather(h->st, nch, slices, width, sorted, (const uint32_t *)pk, (const double *)val,
-                            (const double *)x, (double *)out);
-        hipEvent_t e0, e1;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        std::vector<float> t(5);
-        for (int r = 0; r < 5; ++r) {
-            HIPCHK(hipEventRecord(e0, h->st));
-            launch_probe_gather(h->st, nch, slices, width, sorted, (const uint32_t *)pk, (const double *)val,
-                                (const double *)x, (double *)out);
-            HIPCHK(hipEventRecord(e1, h->st));
-            HIPCHK(hipEventSynchronize(e1));
-            HIPCHK(hipEventElapsedTime(&t[r], e0, e1));
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        h->sync();
+}  // extern "C"
+
+template <typename T>
+static void probe_gather_t(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t slices, int32_t sorted, double *ms) {
+    const int C = sorted ? SortSpec<T>::chunk : kStreamNnz;
+    const int bits = sorted ? SortSpec<T>::pos_bits : kProbePosBits;
+    if (!ms || slices < 1 || width < 1 || nnz < C || (int64_t)width > (1ll << (32 - bits)))
+        throw Err{HPRSOCP_ERR_INVALID, "probe_gather: need nnz >= one chunk, 1 <= width <= 2^(32-position bits), slices >= 1"};
+    const int NT = sorted ? kSortBlock : kBlock;
+    const int64_t nch = nnz / C;
+    if (nch > (int64_t)INT32_MAX) throw Err{HPRSOCP_ERR_INVALID, "probe_gather: too many chunks"};
+    void *pk = nullptr, *val = nullptr, *x = nullptr, *out = nullptr;
+    auto release = [&]() {
+        (void)hipFree(pk);
+        (void)hipFree(val);
+        (void)hipFree(x);
+        (void)hipFree(out);
+    };
+    if (hipMalloc(&pk, (size_t)(nch * C) * 4) != hipSuccess || hipMalloc(&val, (size_t)(nch * C) * sizeof(T)) != hipSuccess ||
+        hipMalloc(&x, (size_t)slices * width * sizeof(T)) != hipSuccess ||
+        hipMalloc(&out, (size_t)(nch * NT) * sizeof(T)) != hipSuccess) {
         release();
-        std::sort(t.begin(), t.end());
-        *ms = t[2];
-    });
+        throw Err{HPRSOCP_ERR_ALLOC, "probe_gather alloc"};
+    }
+    if constexpr (sizeof(T) == 8) launch_fill(h->st, (int64_t)slices * width, 1.0, (double *)x);
+    else HIPCHK(hipMemsetAsync(x, 0, (size_t)slices * width * sizeof(T), h->st));  // values do not matter
+    launch_probe_fill<T>(h->st, nch, width, sorted, (uint32_t *)pk, (T *)val);
+    launch_probe_gather<T>(h->st, nch, slices, width, sorted, (const uint32_t *)pk, (const T *)val, (const T *)x,
+                           (T *)out);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    std::vector<float> t(5);
+    for (int r = 0; r < 5; ++r) {
+        HIPCHK(hipEventRecord(e0, h->st));
+        launch_probe_gather<T>(h->st, nch, slices, width, sorted, (const uint32_t *)pk, (const T *)val,
+                               (const T *)x, (T *)out);
+        HIPCHK(hipEventRecord(e1, h->st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&t[r], e0, e1));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->sync();
+    release();
+    std::sort(t.begin(), t.end());
+    *ms = t[2];
+}
+
+extern "C" {
+
+int hprsocp_probe_gather(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t slices, int32_t sorted, double *ms) {
+    ABI_TRY(h, { probe_gather_t<double>(h, nnz, width, slices, sorted, ms); });
+    return 0;
+}
+
+int hprsocp_probe_gather32(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t slices, int32_t sorted, double *ms) {
+    ABI_TRY(h, { probe_gather_t<float>(h, nnz, width, slices, sorted, ms); });
     return 0;
 }
 
@@ -3431,6 +3651,33 @@ int hprsocp_get_scaled_A(hprsocp_handle *h, double *val) {
     return 0;
 }
 
+int hprsocp_plan_info(hprsocp_handle *h, char *buf, int64_t cap, int64_t *len) {
+    ABI_TRY(h, {
+        std::string j = "{\"operators\": [";
+        char b[512];
+        for (size_t k = 0; k < h->tune.size(); ++k) {
+            const auto &r = h->tune[k];
+            snprintf(b, sizeof b,
+                     "%s{\"name\": \"%s\", \"form\": \"%s\", \"tuned\": %s, \"cached\": %s, "
+                     "\"nnz_target\": %d, \"g_divisor\": %d, \"huge_target\": %d, \"huge_whole\": %d, "
+                     "\"stream\": %d, \"tiles\": %d, \"candidates\": %d, \"ms_default\": %.5f, "
+                     "\"ms\": %.5f, \"ms_form\": %.5f}",
+                     k ? ", " : "", r.name.c_str(), r.form.c_str(), r.tuned ? "true" : "false",
+                     r.cached ? "true" : "false", r.cfg.nnz_target, r.cfg.g_divisor, r.cfg.huge_target,
+                     r.cfg.huge_whole, (int)r.cfg.stream, r.tiles, r.candidates, r.ms_default, r.ms, r.ms_form);
+            j += b;
+        }
+        j += "]}";
+        if (len) *len = (int64_t)j.size();
+        if (buf && cap > 0) {
+            const size_t c = std::min<size_t>((size_t)cap - 1, j.size());
+            memcpy(buf, j.data(), c);
+            buf[c] = 0;
+        }
+    });
+    return 0;
+}
+
 int hprsocp_get_dims(hprsocp_handle *h, int32_t *m, int32_t *n, int64_t *nnz) {
     *m = h->m;
     *n = h->n;
@@ -3752,6 +3999,10 @@ const char *hprsocp_batch_last_error(const hprsocp_batch *b) {
 }
 
 int32_t hprsocp_batch_size(const hprsocp_batch *b) { return b->h->nb; }
+
+int hprsocp_batch_plan_info(hprsocp_batch *b, char *buf, int64_t cap, int64_t *len) {
+    return hprsocp_plan_info(b->h, buf, cap, len);
+}
 
 int hprsocp_batch_dims(const hprsocp_batch *b, int32_t i, int32_t *m, int32_t *n) {
     if (i < 0 || i >= b->h->nb) return HPRSOCP_ERR_INVALID;

@@ -412,6 +412,10 @@ class Solver:
         self._check(self._fn("iterate")(self.h, int(k), _abi.ITERATE_RESET if reset else 0, C.byref(r)))
         return self._results(r)
 
+    def plan_info(self) -> dict:
+        """The autotuner's launch-plan choice for each operator (hprsocp_plan_info)."""
+        return _plan_info(self.lib.hprsocp_plan_info, self.h, self._check)
+
     # restart / sigma seams (check_restart, update_sigma!, do_restart!)
     def init_restart(self) -> _abi.RestartInfo:
         r = _abi.RestartInfo()
@@ -450,12 +454,23 @@ class Solver:
         self._check(self.lib.hprsocp_time_spmv(self.h, int(bool(trans)), int(reps), C.byref(ms)))
         return ms.value
 
-    def probe_gather(self, nnz: int, width: int, slices: int = 8, sorted_chunks: bool = False) -> float:
-        """Median ms of the synthetic gather-ceiling kernel (see include/hprsocp.h)."""
+    def probe_gather(self, nnz: int, width: int, slices: int = 8, sorted_chunks: bool = False,
+                     fp32: bool = False) -> float:
+        """Median ms of the synthetic gather-ceiling kernel (see include/hprsocp.h);
+        ``fp32`` probes float entries and gathers (the FP32 operators)."""
         ms = C.c_double()
-        self._check(self.lib.hprsocp_probe_gather(self.h, int(nnz), int(width), int(slices),
-                                                   int(bool(sorted_chunks)), C.byref(ms)))
+        fn = self.lib.hprsocp_probe_gather32 if fp32 else self.lib.hprsocp_probe_gather
+        self._check(fn(self.h, int(nnz), int(width), int(slices), int(bool(sorted_chunks)), C.byref(ms)))
         return ms.value
+
+
+def _plan_info(fn, h, check) -> dict:
+    import json
+    n = C.c_int64()
+    check(fn(h, None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    check(fn(h, buf, n.value + 1, C.byref(n)))
+    return json.loads(buf.value.decode())
 
 
 def project_cones(inp: int, out: int, n: int, cone_idx: int, ncones: int, stream: int = 0,
@@ -609,6 +624,10 @@ class BatchSolver:
                 self.h, i, _abi.ptr_f64(x), _abi.ptr_f64(y) if m else None, _abi.ptr_f64(z), _abi.ptr_f64(w)))
             out.append(HPRSOCP_results.from_c(res[i], x, y, z, w))
         return out
+
+    def plan_info(self) -> dict:
+        """The autotuner's launch-plan choice for the packed operators."""
+        return _plan_info(self.lib.hprsocp_batch_plan_info, self.h, self._check)
 
     def time_iterations(self, iters: int, per_kernel: bool = False):
         """(ms per iteration, per-launch-group ms or None) of `iters` lockstep iterations."""

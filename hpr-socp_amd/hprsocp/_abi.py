@@ -161,6 +161,7 @@ _SIGS = {
 # Entry points of the HIP backend only (the oracle runs its loop in one piece).
 _LOOP_SIGS = {
     "iterate": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(Result)]),
+    "plan_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)]),
 }
 ITERATE_RESET = 1
 
@@ -179,6 +180,7 @@ _BATCH_SIGS = {
     "batch_solve": (C.c_int, [C.c_void_p, C.POINTER(Result)]),
     "batch_get_solution": (C.c_int, [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _f64p]),
     "batch_time_iterations": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), _f64p]),
+    "batch_plan_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)]),
 }
 
 
@@ -196,6 +198,7 @@ _HOOK_SIGS = {
     "probe_copy": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double)]),
     "time_spmv": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "probe_gather": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    "probe_gather32": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
 }
 
 

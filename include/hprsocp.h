@@ -327,6 +327,16 @@ int hprsocp_get_scalars(hprsocp_handle *h, hprsocp_scalars *out);
 int hprsocp_set_scalars(hprsocp_handle *h, double sigma, double lambda_A, double lambda_Q);
 /* Scaled A values in CSR order (A_nnz entries). */
 int hprsocp_get_scaled_A(hprsocp_handle *h, double *val);
+/* Launch-plan choices of the autotuner for every operator of the handle, as a
+ * JSON object {"operators": [{name, form (plain | slicesP | sortedP), tuned,
+ * cached, nnz_target, g_divisor, huge_target, huge_whole, stream, tiles,
+ * candidates, ms_default, ms, ms_form}, ...]}: the tile-plan configuration
+ * chosen by the timed search (the reference's fixed launch shapes,
+ * kernels.jl:349-357, and cone size classes, algorithm.jl:1914-1919), the SpMV
+ * time of the default and the chosen plan, and the SpMV form kept by the XCD
+ * slicing tuner.  Writes at most cap bytes (NUL-terminated) to buf and the
+ * full length to *len (either may be NULL). */
+int hprsocp_plan_info(hprsocp_handle *h, char *buf, int64_t cap, int64_t *len);
 /* Dimensions: m, n, nnz(A). */
 int hprsocp_get_dims(hprsocp_handle *h, int32_t *m, int32_t *n, int64_t *nnz);
 
@@ -371,6 +381,11 @@ int hprsocp_time_spmv(hprsocp_handle *h, int32_t trans, int32_t reps, double *ms
  * measurement hook.) */
 int hprsocp_probe_gather(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t slices, int32_t sorted,
                          double *ms);
+/* The same gather ceiling for the FP32 operators: 8-byte entries (4-byte
+ * value, 4-byte packed word), one float gathered per entry, and the FP32
+ * sorted layout's chunks of 16384 entries (sorted = 1).  (Measurement hook.) */
+int hprsocp_probe_gather32(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t slices, int32_t sorted,
+                           double *ms);
 
 /* ------------------------------------------------------------ cone projection */
 /* Stand-alone projection onto a product of second-order cones, on device
@@ -439,6 +454,8 @@ const char *hprsocp_batch_last_error(const hprsocp_batch *b);
 int32_t hprsocp_batch_size(const hprsocp_batch *b);
 /* m and n of instance i */
 int hprsocp_batch_dims(const hprsocp_batch *b, int32_t i, int32_t *m, int32_t *n);
+/* hprsocp_plan_info of the batch's packed operators */
+int hprsocp_batch_plan_info(hprsocp_batch *b, char *buf, int64_t cap, int64_t *len);
 /* per-instance estimate_eigenvalues (algorithm.jl:2434-2467) */
 int hprsocp_batch_estimate_lambda(hprsocp_batch *b);
 /* per-instance process_initial_points! (algorithm.jl:2612-2677); x0s / y0s hold
