@@ -2424,6 +2424,35 @@ int hprsocp_time_spmv(hprsocp_handle *h, int32_t trans, int32_t reps, double *ms
     ABI_TRY(h, {
         if (reps < 1 || !ms) throw Err{HPRSOCP_ERR_INVALID, "time_spmv: reps >= 1 and an output are required"};
         if (h->nb != 1) throw Err{HPRSOCP_ERR_UNSUPPORTED, "time_spmv applies to single-problem handles"};
+        if (trans & 2) {  // the FP32 operator of the FP32 forms
+            if (!h->fp32 && !h->delta)
+                throw Err{HPRSOCP_ERR_STATE, "time_spmv: FP32 operators exist only in the FP32 precisions"};
+            const bool t1 = trans & 1;
+            const CsrT<float> &M32 = t1 ? h->v32.AT : h->v32.A;
+            const DevPlan &pl = t1 ? h->plan_AT : h->plan_A;
+            const int64_t lin = t1 ? h->m : h->n, lout = t1 ? h->n : h->m;
+            float *din = h->dalloc<float>(lin), *dout = h->dalloc<float>(lout);
+            launch_spmv<float>(h->st, pl, M32, din, dout);  // warm-up
+            h->sync();
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            std::vector<float> t((size_t)reps);
+            for (int r = 0; r < reps; ++r) {
+                HIPCHK(hipEventRecord(e0, h->st));
+                launch_spmv<float>(h->st, pl, M32, din, dout);
+                HIPCHK(hipEventRecord(e1, h->st));
+                HIPCHK(hipEventSynchronize(e1));
+                HIPCHK(hipEventElapsedTime(&t[r], e0, e1));
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            h->dfree(din);
+            h->dfree(dout);
+            std::sort(t.begin(), t.end());
+            *ms = t[t.size() / 2];
+            return 0;
+        }
         const CsrView &M = trans ? h->AT : h->A;
         const DevPlan &pl = trans ? h->plan_AT : h->plan_A;
         double *din = trans ? h->tmp_m : h->tmp_n;

@@ -448,10 +448,12 @@ class Solver:
                                                       kern if per_kernel else None))
         return ms.value, list(kern) if per_kernel else None
 
-    def time_spmv(self, trans: bool = False, reps: int = 20) -> float:
-        """Median ms of the iteration's A x (or A^T y) SpMV on resident operands."""
+    def time_spmv(self, trans: bool = False, reps: int = 20, fp32: bool = False) -> float:
+        """Median ms of the iteration's A x (or A^T y) SpMV on resident operands
+        (``fp32``: the FP32 operator of the FP32 precisions)."""
         ms = C.c_double()
-        self._check(self.lib.hprsocp_time_spmv(self.h, int(bool(trans)), int(reps), C.byref(ms)))
+        self._check(self.lib.hprsocp_time_spmv(self.h, int(bool(trans)) | (2 if fp32 else 0), int(reps),
+                                                C.byref(ms)))
         return ms.value
 
     def probe_gather(self, nnz: int, width: int, slices: int = 8, sorted_chunks: bool = False,

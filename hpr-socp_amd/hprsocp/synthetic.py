@@ -284,7 +284,15 @@ def make_robust_regression(rng, d: int, lam: float = 0.1):
     Returns (model, warm_x) with warm_x the point built from x* (x+- from its
     sign split, r = D x* - y)."""
     h = d // 2
-    cols = np.concatenate([rng.choice(d, 10, replace=False) for _ in range(h)])
+    # 10 distinct columns per row: uniform draws, redrawn for the few rows with a repeat
+    cols = rng.integers(0, d, size=(h, 10))
+    while True:
+        srt = np.sort(cols, axis=1)
+        bad = np.nonzero(np.any(srt[:, 1:] == srt[:, :-1], axis=1))[0]
+        if bad.size == 0:
+            break
+        cols[bad] = rng.integers(0, d, size=(bad.size, 10))
+    cols = cols.ravel()
     rows = np.repeat(np.arange(h), 10)
     D = sp.csr_matrix((rng.standard_normal(h * 10), (rows, cols)), shape=(h, d))
     D.sum_duplicates()

@@ -366,6 +366,8 @@ int hprsocp_probe_copy(hprsocp_handle *h, int64_t bytes, double *ms);
 
 /* Median HIP-event time (ms) of `reps` launches of the FP64 SpMV A x
  * (trans = 0) or A^T y (trans = 1) exactly as the FP64 iteration runs it
+ * (trans | 2: the FP32 operator of the FP32 precisions, as their iterations
+ * run it)
  * (sliced or plain form chosen by the autotuner), on device-resident
  * operands.  Single-problem handles.  (No counterpart in the reference;
  * measurement hook.) */
