@@ -1,11 +1,12 @@
-"""The Julia binding written out in the repository's integration notes (module
-HPRSOCPHip) passes hprsocp_problem, hprsocp_params and hprsocp_result to ccall
+"""The Julia binding julia/HPRSOCPHip.jl (module HPRSOCPHip) passes
+hprsocp_problem, hprsocp_params, hprsocp_result and hprsocp_residuals to ccall
 as isbits structs that repeat the C layout field by field (Julia lays out
 isbits structs with C alignment rules).  No Julia toolchain exists here, so the
 shim itself cannot run; this test parses its struct definitions and checks
 them against the C layout: the same fields in the same order at the same
 offsets, and the same size, as the ctypes mirror (whose sizes test_abi checks
-against the compiled library)."""
+against the compiled library).  It also checks that a Q operator without a
+factored form is rejected, never passed on as an empty Q."""
 import ctypes as C
 import os
 import re
@@ -16,7 +17,7 @@ import pytest
 from hprsocp import _abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DOC = os.path.join(ROOT, "INTEGRATION.md")
+DOC = os.path.join(ROOT, "julia", "HPRSOCPHip.jl")
 
 # Julia type -> (size, alignment) on x86-64 / aarch64 Linux
 JULIA_TYPES = {"Cint": (4, 4), "Int32": (4, 4), "Int64": (8, 8), "Cdouble": (8, 8), "Float64": (8, 8),
@@ -60,7 +61,7 @@ def c_struct(cls):
 
 
 @pytest.mark.parametrize("jname,cls", [("CProblem", _abi.Problem), ("CParams", _abi.Params),
-                                       ("CResult", _abi.Result)])
+                                       ("CResult", _abi.Result), ("CResiduals", _abi.Residuals)])
 def test_julia_structs_match_the_c_layout(jname, cls):
     jf, jsize = julia_struct(jname)
     cf, csize = c_struct(cls)
@@ -74,9 +75,25 @@ def test_parser_sees_a_changed_field(tmp_path, monkeypatch):
     CParams.save_filename no longer matches the C layout."""
     text = open(DOC).read()
     assert "auto_save::Cint; save_filename::Cstring;" in text
-    bad = tmp_path / "INTEGRATION.md"
+    bad = tmp_path / "HPRSOCPHip.jl"
     bad.write_text(text.replace("auto_save::Cint; save_filename::Cstring;", "auto_save::Cint;"))
     monkeypatch.setattr(sys.modules[__name__], "DOC", str(bad))
     jf, jsize = julia_struct("CParams")
     cf, csize = c_struct(_abi.Params)
     assert jf != cf and jsize != csize
+
+
+def test_operator_q_is_factored_or_rejected():
+    """An AbstractQOperatorCPU Q goes to the backend only in the factored form
+    F'F + Diagonal(d) (HPRSOCP_Q_FACTORED); without a hip_factored method the
+    shim raises an error.  A SparseMatrixCSC Q goes as CSR; nothing else is
+    accepted."""
+    text = open(DOC).read()
+    body = text[text.index("function problem_arrays"):text.index("ptr_or_null(v::Vector)")]
+    sparse_branch, rest = body.split("elseif Q isa AbstractQOperatorCPU", 1)
+    op_branch, other = rest.split("else", 1)
+    assert "if Q isa SparseMatrixCSC" in sparse_branch and "HPRSOCP_Q_CSR" in sparse_branch
+    assert "hasmethod(hip_factored" in op_branch and "error(" in op_branch
+    assert "HPRSOCP_Q_FACTORED" in op_branch and "hip_factored(Q)" in op_branch
+    assert other.lstrip().startswith('error("HPRSOCPHip: unsupported Q type')
+    assert "const HPRSOCP_Q_CSR = Cint(0)" in text and "const HPRSOCP_Q_FACTORED = Cint(1)" in text
