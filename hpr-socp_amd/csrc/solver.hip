@@ -30,6 +30,8 @@
 #include "../../include/hprsocp.h"
 #include "kernels.h"
 #include "plan.h"
+#include "ctl.h"
+#include "ctlk.h"
 #include "golden.h"
 #include "scale_rules.h"
 #include "npz.h"
@@ -366,6 +368,18 @@ struct hprsocp_handle {
     double *d_ra = nullptr, *d_rya = nullptr, *d_img_n = nullptr, *d_img_m = nullptr;
     // the solve loop in progress (hprsocp_iterate)
     SolveLoop *loop = nullptr;
+    // device controller of batch solves (window-edge steps queued on the stream)
+    bool dctl = false;
+    ctl::CtlInst *ci = nullptr;        // device, nb
+    ctl::InstConst *ic = nullptr;      // device, nb
+    int32_t *ctl_status = nullptr;     // device, nb
+    int32_t *noflag = nullptr;         // device, nb: !flag
+    int32_t *ctl_status_h = nullptr;   // pinned, 2 x nb (one per event in flight)
+    hipEvent_t ctl_ev[2] = {nullptr, nullptr};
+    Tile *stage_h[3] = {nullptr, nullptr, nullptr};  // pinned staging of compacted tile lists
+    hipEvent_t stage_ev = nullptr;
+    bool stage_pending = false;
+    std::vector<hipGraphExec_t> graphs_retired;      // replaced while possibly still running
     // launch-plan choices of the autotuner, one record per operator
     struct TuneRecord {
         std::string name, form = "plain";
@@ -387,6 +401,17 @@ struct hprsocp_handle {
         for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
         graphs.clear();
         graph_seen.clear();
+    }
+    // drops the captured windows while the stream may still run one of them;
+    // they are destroyed after the next full synchronisation (free_retired_graphs)
+    void retire_graphs() {
+        for (auto &kv : graphs) graphs_retired.push_back(kv.second);
+        graphs.clear();
+        graph_seen.clear();
+    }
+    void free_retired_graphs() {
+        for (auto g : graphs_retired) (void)hipGraphExecDestroy(g);
+        graphs_retired.clear();
     }
 
     InstScal &I(int i) { return isc_h[i]; }
@@ -424,6 +449,13 @@ struct hprsocp_handle {
         if (st) (void)hipStreamSynchronize(st);
         free_loop(loop);
         clear_graphs();
+        free_retired_graphs();
+        if (ctl_status_h) (void)hipHostFree(ctl_status_h);
+        for (auto &e : ctl_ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &b : stage_h)
+            if (b) (void)hipHostFree(b);
+        if (stage_ev) (void)hipEventDestroy(stage_ev);
         for (void *p : allocs) (void)hipFree(p);
         if (hscal) (void)hipHostFree(hscal);
         if (isc_h) (void)hipHostFree(isc_h);
@@ -505,6 +537,31 @@ void compact_plan(hprsocp_handle *h, const HostPlan &hp, DevPlan &act, const std
     if (!t.empty())
         HIPCHK(hipMemcpyAsync(act.tiles, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, h->st));
     h->sync();  // t is a host temporary
+}
+
+// compact_plan without a synchronisation: the kept tiles go through pinned
+// staging buffer `which` (one per plan) and an in-stream copy, so the running
+// window still reads the old list and the next window the new one.  A staging
+// buffer is rewritten only after the previous staged copies have landed.
+void compact_plan_staged(hprsocp_handle *h, const HostPlan &hp, DevPlan &act, const std::vector<char> &keep,
+                         int which) {
+    if (!h->stage_h[which])
+        HIPCHK(hipHostMalloc((void **)&h->stage_h[which], sizeof(Tile) * std::max<size_t>(hp.tiles.size(), 1)));
+    if (h->stage_pending) {
+        HIPCHK(hipEventSynchronize(h->stage_ev));
+        h->stage_pending = false;
+    }
+    Tile *t = h->stage_h[which];
+    int32_t n = 0, first_huge = 0;
+    for (size_t k = 0; k < hp.tiles.size(); ++k) {
+        const Tile &x = hp.tiles[k];
+        if (!keep[x.inst]) continue;
+        t[n++] = x;
+        if ((int32_t)k < hp.first_huge) ++first_huge;
+    }
+    act.ntiles = n;
+    act.first_huge = first_huge;
+    if (n > 0) HIPCHK(hipMemcpyAsync(act.tiles, t, sizeof(Tile) * n, hipMemcpyHostToDevice, h->st));
 }
 
 double *slot(hprsocp_handle *h, int k) { return h->dscal + k; }
@@ -1513,7 +1570,9 @@ void delta_prec(hprsocp_handle *h, int f32) {
             h->I(i).fp32 = f32;
             change = true;
         }
-    if (change) upload_isc(h);
+    if (!change) return;
+    if (h->dctl) launch_isc_fp32(h->st, h->isc, h->nb, f32);  // sigma, inner0, active live on the device
+    else upload_isc(h);
 }
 
 // Delta form: FP64 state -> deltas from the anchors, the anchors' images
@@ -2291,7 +2350,10 @@ static void run_partial_updates(hprsocp_handle *h, int64_t L) {
     }
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
-        if (h->graphs.size() >= 64) h->clear_graphs();
+        if (h->graphs.size() >= 64) {
+            if (h->dctl) h->retire_graphs();  // the stream may still run one of them
+            else h->clear_graphs();
+        }
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
         HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
@@ -2326,7 +2388,16 @@ int hprsocp_update(hprsocp_handle *h, int64_t inner, int32_t to_check) {
 }
 
 // compute_residuals! (algorithm.jl:113-196) for every instance; r has nb entries.
-static void residuals_impl(hprsocp_handle *h, int64_t iter, hprsocp_residuals *r) {
+// Constants of instance i that the residual block needs.
+static ctl::InstConst inst_const(hprsocp_handle *h, int i) {
+    const InstHost &H = h->ih[i];
+    const InstRange &R = h->rng[i];
+    return ctl::InstConst{H.norm_b_org, H.norm_c_org, H.obj_const, R.m() > 0, R.lin1 > R.lin0, R.con1 > R.con0};
+}
+
+// The residual reductions of compute_residuals! (algorithm.jl:113-196) into the
+// SL_RN / SL_RM / SL_BV slots, no host synchronisation.
+static void residuals_enqueue(hprsocp_handle *h, int64_t iter) {
     hipStream_t st = h->st;
     delta_join(h);
     const int32_t m = h->m, n = h->n;
@@ -2352,30 +2423,16 @@ static void residuals_impl(hprsocp_handle *h, int64_t iter, hprsocp_residuals *r
     if (iter == 0)
         launch_bounds_violation(st, h->seg_n, h->nvar, h->var_idx, h->cinst_var, h->x_bar, h->l, h->u, h->col_norm,
                                 h->isc, h->dx, region(h, 2), slot(h, SL_BV), SL_COUNT);
+}
+
+static void residuals_impl(hprsocp_handle *h, int64_t iter, hprsocp_residuals *r) {
+    const bool skip_q = !h->has_quad;
+    residuals_enqueue(h, iter);
     read_slots(h);
     for (int i = 0; i < h->nb; ++i) {
-        const double bs = h->I(i).b_scale, cs = h->I(i).c_scale;
-        const InstHost &H = h->ih[i];
-        const bool hm = h->rng[i].m() > 0;
         const double *v = h->hscal + (int64_t)i * SL_COUNT;
-        const double *rn = v + SL_RN, *rm = v + SL_RM;
-        hprsocp_residuals &R = r[i];
-        R.primal_obj = bs * cs * (rn[0] + 0.5 * rn[1]) + H.obj_const;
-        R.dual_obj = bs * cs * (-0.5 * rn[1] + rn[2]) + H.obj_const;
-        if (hm) R.dual_obj += bs * cs * (rm[0] + rm[1]);
-        R.rel_gap = fabs(R.primal_obj - R.dual_obj) / (1.0 + jmax(fabs(R.primal_obj), fabs(R.dual_obj)));
-        const double qx_inf = skip_q ? 0.0 : rn[5];
-        R.err_Rd = rn[3] / (1.0 + jmax(jmax(H.norm_c_org, rn[4]), qx_inf));
-        if (hm) {
-            const double denom = 1.0 + jmax(H.norm_b_org, rm[4]);
-            R.err_Rp = jmax(rm[2], rm[3]) / denom;
-            R.err_Rp_linear = h->rng[i].lin1 > h->rng[i].lin0 ? rm[2] / denom : 0.0;
-            R.err_Rp_soc = h->rng[i].con1 > h->rng[i].con0 ? rm[3] / denom : 0.0;
-        } else {
-            R.err_Rp = R.err_Rp_linear = R.err_Rp_soc = 0.0;
-        }
-        if (iter == 0) R.err_Rp = jmax(R.err_Rp, v[SL_BV]);
-        R.kkt = jmax(jmax(R.err_Rp, R.err_Rd), R.rel_gap);
+        ctl::residuals(v + SL_RN, v + SL_RM, v[SL_BV], h->I(i).b_scale, h->I(i).c_scale, inst_const(h, i), skip_q,
+                       iter == 0, r[i]);
     }
 }
 
@@ -2543,6 +2600,24 @@ int hprsocp_probe_gather32(hprsocp_handle *h, int64_t nnz, int32_t width, int32_
     return 0;
 }
 
+int hprsocp_probe_ctl(hprsocp_handle *h, const double *in, int32_t n, double *dev_out, double *host_out) {
+    ABI_TRY(h, {
+        if (n < 0 || (n > 0 && (!in || !dev_out || !host_out)))
+            throw Err{HPRSOCP_ERR_INVALID, "hprsocp_probe_ctl: n < 0 or a null array"};
+        if (n == 0) return 0;
+        double *din = h->upload(in, n);
+        double *dout = h->dalloc<double>(5 * (int64_t)n);
+        launch_ctl_probe(h->st, din, n, dout);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(dev_out, dout, sizeof(double) * 5 * (size_t)n, hipMemcpyDeviceToHost, h->st));
+        h->sync();
+        h->dfree(din);
+        h->dfree(dout);
+        for (int32_t k = 0; k < n; ++k) ctl_probe_eval(in[k], host_out + 5 * (int64_t)k);
+    });
+    return 0;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------ M-norm
@@ -2594,30 +2669,7 @@ void mnorm_enqueue(hprsocp_handle *h, int base) {
 
 // M-norm of instance i from its 8 reduced dots d (host copy)
 double mnorm_finish(hprsocp_handle *h, int i, const double *d, const MnormCtx &c) {
-    const bool q = h->has_quad;
-    const bool hm = h->rng[i].m() > 0;
-    const double sig = c.sigma[i];
-    double M1 = 0.0, M2 = (1.0 / sig) * d[0], M3 = 0.0;
-    if (q) M2 -= 2.0 * d[1];
-    if (hm) {
-        M1 = sig * c.lambda_A[i] * d[7];
-        if (q) M1 -= 2.0 * sig * d[2];
-        M2 += 2.0 * d[3];
-        if (q) {
-            if (h->Q_is_diag) {
-                M3 = d[4];
-                M1 += sig * d[5];
-            } else {
-                M3 = (sig * sig) / (1.0 + sig * c.lambda_Q[i]) * d[4];
-                M1 += sig * c.lambda_Q[i] * d[5];
-            }
-        }
-    } else if (!h->Q_is_diag && q) {
-        M1 = sig * c.lambda_Q[i] * d[5];
-    }
-    M2 += jmax(M1, 0.0);
-    const double Mn = jmax(M2, 0.0) + jmax(M3, 0.0);
-    return sqrt(Mn);
+    return ctl::mnorm(d, c.sigma[i], c.lambda_A[i], c.lambda_Q[i], h->has_quad, h->rng[i].m() > 0, h->Q_is_diag);
 }
 
 // ------------------------------------------------------------------ restart
@@ -2625,82 +2677,11 @@ double mnorm_finish(hprsocp_handle *h, int i, const double *d, const MnormCtx &c
 using Restart = hprsocp_restart_info;
 
 // initialize_restart, algorithm.jl:1886-1908
-void init_restart(Restart &r) {
-    memset(&r, 0, sizeof(r));
-    r.first_restart = 1;
-    r.last_gap = r.current_gap = r.save_gap = INFINITY;
-    r.weighted_norm = r.best_gap = r.best_kkt = INFINITY;
-    r.best_sigma = 1.0;
-}
+void init_restart(Restart &r) { ctl::init_restart(r); }
 
 // check_restart, algorithm.jl:1016-1099
 void check_restart(Restart &r, int64_t iter, int64_t check_iter, double sigma, double kkt) {
-    r.restart_flag = 0;
-    if (r.first_restart) {
-        if (iter == check_iter) {
-            r.first_restart = 0;
-            r.restart_flag = 1;
-            r.weighted_norm = r.current_gap;
-            r.best_gap = r.current_gap;
-            r.best_kkt = kkt;
-            r.best_sigma = sigma;
-        }
-        return;
-    }
-    if (iter % check_iter != 0) return;
-    if (r.current_gap < 0) r.current_gap = 1e-6;
-    if (r.current_gap <= 0.36 * r.last_gap) {
-        r.sufficient++;
-        r.restart_flag = 1;
-    }
-    if (r.current_gap <= 0.8 * r.last_gap && r.current_gap > 1.00 * r.save_gap) {
-        r.necessary++;
-        r.restart_flag = 2;
-    }
-    const double fact = 0.15;
-    if (((double)r.inner >= jmax(500.0, fact * (double)iter) ||
-         (r.inner >= 10000 && r.current_gap <= 0.75 * r.last_gap)) &&
-        r.current_gap > 0.9 * r.save_gap) {
-        r.long_restarts++;
-        r.restart_flag = 3;
-    }
-    // should_refresh_best_sigma, algorithm.jl:682-693
-    const bool improved = kkt < r.best_kkt;
-    const bool compat = r.current_gap < r.best_gap && kkt <= 1.01 * r.best_kkt;
-    const bool periodic = r.best_iter >= 10 && kkt <= 1.2 * r.best_kkt;
-    if (improved || compat || periodic) {
-        r.best_gap = r.current_gap;
-        r.best_kkt = kkt;
-        r.best_sigma = sigma;
-        r.best_iter = 0;
-    }
-    r.save_gap = r.current_gap;
-}
-
-double f_dev(double x, double a, double b, double c, double d) { return a * x + b / x + c * x * x / (1 + d * x); }
-
-// golden, kernels.jl:6272-6305
-double golden_scalar(double ap, double bp, double cp, double dp, double lo, double hi, double tol) {
-    const double phi = (sqrt(5.0) - 1.0) / 2.0;
-    double a = lo, b = hi, c = b - phi * (b - a), d = a + phi * (b - a);
-    double fc = f_dev(c, ap, bp, cp, dp), fd = f_dev(d, ap, bp, cp, dp);
-    for (int i = 0; i < 200; ++i) {
-        if (fd < fc) {
-            a = c;
-            c = d;
-            fc = fd;
-            d = a + phi * (b - a);
-            fd = f_dev(d, ap, bp, cp, dp);
-        } else {
-            b = d;
-            d = c;
-            fd = fc;
-            c = b - phi * (b - a);
-            fc = f_dev(c, ap, bp, cp, dp);
-        }
-        if ((b - a) < tol) break;
-    }
-    return 0.5 * (a + b);
+    ctl::check_restart(r, iter, check_iter, sigma, kkt);
 }
 
 // unified_golden_Q_diag (unified_operations.jl:411-449) for the listed
@@ -2734,18 +2715,14 @@ std::vector<double> golden_q_diag(hprsocp_handle *h, const std::vector<int> &ids
     return r;
 }
 
-// update_sigma!, algorithm.jl:729-990, for every active instance with a restart
-void update_sigma(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<hprsocp_residuals> &RS,
-                  const std::vector<char> &active) {
-    std::vector<int> ids;
-    for (int i = 0; i < h->nb; ++i)
-        if (active[i] && R[i].restart_flag >= 1) ids.push_back(i);
-    if (ids.empty()) return;
+// The dots update_sigma! needs (algorithm.jl:781-829), for every instance, into
+// the SL_SIG slots: s[0] |x_bar - last_x|^2, s[7] |y_bar - last_y|^2 and, with
+// a quadratic term, the dots of dw = w_bar - last_w, dQw, ATdy = ATy_bar -
+// last_ATy and QATdy.  No host synchronisation.
+void sigma_dots_enqueue(hprsocp_handle *h) {
     hipStream_t st = h->st;
     const int32_t m = h->m, n = h->n;
-    const bool skip_q = !h->has_quad, any_m = m > 0, qd = h->Q_is_diag;
-    const double SMIN = 1e-12, SMAX = 1e12, THETA_MAX = 1.0, IMB = 1e6, FLOOR = 2.0, THR = 1e-8, EPS = 1e-16;
-    const double LOG100 = log(100.0);
+    const bool skip_q = !h->has_quad, any_m = m > 0;
     if (!skip_q) {
         launch_axpby(st, n, 1.0, h->w_bar, -1.0, h->last_w, h->dw);
         qmap(h, h->dw, h->dqw);
@@ -2756,8 +2733,6 @@ void update_sigma(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<
         if (!skip_q) {
             launch_axpby(st, n, 1.0, h->aty_bar, -1.0, h->last_aty, h->atdy);
             qmap(h, h->atdy, h->qatdy);
-        } else {
-            launch_spmv(st, act64(h) ? h->act_AT : h->plan_AT, h->AT, h->dy, h->atdy);
         }
     }
     // n-side dots: b=|dx|^2, <dQw,ATdy>, |dQw|^2, <dw,dQw>, <ATdy,QATdy>
@@ -2771,6 +2746,17 @@ void update_sigma(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<
     launch_reduce(st, h->seg_n, tn.data(), (int)tn.size(), region(h, 0), slot(h, SL_SIG), SL_COUNT);
     RedTerm ty{h->dy, h->dy, nullptr, 0};
     launch_reduce(st, h->seg_m, &ty, 1, region(h, 1), slot(h, SL_SIG + 7), SL_COUNT);
+}
+
+// update_sigma!, algorithm.jl:729-990, for every active instance with a restart
+void update_sigma(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<hprsocp_residuals> &RS,
+                  const std::vector<char> &active) {
+    std::vector<int> ids;
+    for (int i = 0; i < h->nb; ++i)
+        if (active[i] && R[i].restart_flag >= 1) ids.push_back(i);
+    if (ids.empty()) return;
+    const bool skip_q = !h->has_quad, qd = h->Q_is_diag;
+    sigma_dots_enqueue(h);
     read_slots(h);
     // model sigma per instance
     const size_t K = ids.size();
@@ -2781,39 +2767,17 @@ void update_sigma(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<
         const int i = ids[k];
         const double *d = h->hscal + (int64_t)i * SL_COUNT + SL_SIG;
         const bool has_m = h->rng[i].m() > 0;
-        const double dd_x = d[0], dd_y = has_m ? d[7] : 0.0;
-        double a = 0.0, b = dd_x, c = 0.0, dl = 0.0;
-        if (has_m) {
-            if (!skip_q) a = h->I(i).lambda_A * dd_y - 2.0 * d[1];
-            else a = h->I(i).lambda_A * dd_y;
-        }
-        if (!skip_q) {
-            if (qd) {
-                const double nq = sqrt(d[2]);
-                a += nq * nq;
-            } else {
-                a += h->I(i).lambda_Q * d[3];
-                if (has_m) {
-                    c = d[4];
-                    dl = h->I(i).lambda_Q;
-                }
-            }
-        }
-        a = jmax(a, EPS);
-        b = jmax(b, EPS);
+        double a, b, c, dl;
+        ctl::sigma_terms(d, skip_q, has_m, qd, h->I(i).lambda_A, h->I(i).lambda_Q, a, b, c, dl);
         av[k] = a;
         bv[k] = b;
         cv[k] = c;
         dv[k] = dl;
-        if (skip_q || !has_m) {
-            sm[k] = sqrt(b / a);
-        } else if (qd) {
+        sm[k] = ctl::sigma_model(a, b, c, dl, skip_q, has_m, qd);
+        if (!skip_q && has_m && qd) {  // the diagonal-Q search, over device vectors below
             gold_ids.push_back(i);
             ga.push_back(a);
             gb.push_back(b);
-            sm[k] = NAN;  // filled below
-        } else {
-            sm[k] = golden_scalar(a, b, c, dl, 1e-12, 1e12, 1e-13);
         }
     }
     if (!gold_ids.empty()) {
@@ -2825,68 +2789,16 @@ void update_sigma(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<
     bool changed = false;
     for (size_t k = 0; k < K; ++k) {
         const int i = ids[k];
-        Restart &r = R[i];
-        const hprsocp_residuals &res = RS[i];
         const double sigma_old = h->I(i).sigma;
-        const double sref = r.best_sigma, ck = res.kkt, cg = r.current_gap, bk = r.best_kkt, bg = r.best_gap;
-        const bool ref_ok = (ck <= 10.0 * bk) && (cg <= 10.0 * bg);
-        double pr = sqrt((ck / bk) * (cg / bg));
-        pr = jclamp(pr, 1.0, 10.0);
-        double sbase = sm[k];
-        if (ref_ok) {
-            const double theta = jclamp(exp(-0.3 * (pr - 1.0)), 0.0, THETA_MAX);
-            sbase = exp(theta * log(sm[k]) + (1.0 - theta) * log(sref));
-            sbase = jclamp(sbase, SMIN, SMAX);
-        }
-        const double ep = jmax(res.err_Rp, EPS), ed = jmax(res.err_Rd, EPS);
-        const double es = jmin(ep, ed), el = jmax(ep, ed), ir = el / es;
-        const bool both = el <= THR;
-        const bool one = (es <= 0.5 * THR) && (el >= 2.0 * THR);
-        const bool extreme = ir >= IMB;
-        const bool trig = !both && (one || extreme);
-        const bool keep = !both && ((es <= 0.8 * THR) || (ir >= 0.1 * IMB));
-        const int cdir = ed >= ep ? 1 : -1;
-        if (both) {
-            r.sigma_correction_active = 0;
-            r.sigma_correction_hold = -1;
-            r.sigma_correction_dir = 0;
-        } else if (trig || (r.sigma_correction_active && keep)) {
-            r.sigma_correction_active = 1;
-            r.sigma_correction_hold = 0;
-            r.sigma_correction_dir = cdir;
-        } else {
-            r.sigma_correction_active = 0;
-            r.sigma_correction_hold = -1;
-            r.sigma_correction_dir = 0;
-        }
-        double beta = 0.0;
-        if (!both) {
-            if (one) beta = 0.5;
-            if (extreme) beta = 1.0;
-            else if (r.sigma_correction_active) beta = 0.5;
-        }
-        double kappa = 1.0;
-        if (beta > 0.0) {
-            double rk = ed / ep;
-            if (r.sigma_correction_active) {
-                if (r.sigma_correction_dir > 0) rk = jmax(rk, FLOOR);
-                else if (r.sigma_correction_dir < 0) rk = jmin(rk, 1.0 / FLOOR);
-            }
-            kappa = exp(beta * jclamp(log(rk), -LOG100, LOG100));
-        }
-        double sn = sbase * kappa;
-        if (r.sigma_correction_active) {
-            if (r.sigma_correction_dir > 0) sn = jmax(sn, sigma_old * 1.5);
-            else if (r.sigma_correction_dir < 0) sn = jmin(sn, sigma_old / 1.5);
-        }
-        sn = jclamp(sn, SMIN, SMAX);
+        const double sn = ctl::sigma_next(R[i], RS[i], sm[k], sigma_old);
         h->I(i).sigma = sn;
-        r.best_iter += 1;
         if (fabs(sigma_old - sn) > 1e-15) changed = true;
     }
     upload_isc(h);
     if (qd && !skip_q && changed) update_q_factors(h);
 }
+
+void restart_copies_enqueue(hprsocp_handle *h);
 
 // do_restart!, algorithm.jl:1130-1166, for the active instances with a restart flag
 bool do_restart(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<char> &active) {
@@ -2896,8 +2808,25 @@ bool do_restart(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<ch
         any = any || h->flag_h[i];
     }
     if (!any) return false;
+    HIPCHK(hipMemcpyAsync(h->flag, h->flag_h, sizeof(int32_t) * h->nb, hipMemcpyHostToDevice, h->st));
+    restart_copies_enqueue(h);
+    h->sync();  // the flags' pinned mirror may change at the next check
+    for (int i = 0; i < h->nb; ++i) {
+        if (!h->flag_h[i]) continue;
+        Restart &r = R[i];
+        r.last_gap = r.current_gap;
+        r.save_gap = INFINITY;
+        r.times += 1;
+        r.inner = 0;
+    }
+    return true;
+}
+
+// The copies of do_restart! (algorithm.jl:1130-1166) for the instances whose
+// device flag is set: x, last_x <- x_bar; w, last_w <- w_bar; y, last_y <- y_bar;
+// ATy, last_ATy, ATy_bar <- A^T y_bar (QP).
+void restart_copies_enqueue(hprsocp_handle *h) {
     hipStream_t st = h->st;
-    HIPCHK(hipMemcpyAsync(h->flag, h->flag_h, sizeof(int32_t) * h->nb, hipMemcpyHostToDevice, st));
     const SegPlan &sn = h->seg_n, &sm = h->seg_m;
     launch_copy_flagged(st, sn, h->x_bar, h->x, h->flag);
     launch_copy_flagged(st, sn, h->x_bar, h->last_x, h->flag);
@@ -2915,16 +2844,6 @@ bool do_restart(hprsocp_handle *h, std::vector<Restart> &R, const std::vector<ch
             launch_copy_flagged(st, sn, h->tmp_n2, h->aty, h->flag);
         }
     }
-    h->sync();  // the flags' pinned mirror may change at the next check
-    for (int i = 0; i < h->nb; ++i) {
-        if (!h->flag_h[i]) continue;
-        Restart &r = R[i];
-        r.last_gap = r.current_gap;
-        r.save_gap = INFINITY;
-        r.times += 1;
-        r.inner = 0;
-    }
-    return true;
 }
 
 // collect_results! (algorithm.jl:1222-1256) for instance i from the scaled
@@ -3175,11 +3094,295 @@ void begin_loop(hprsocp_handle *h) {
     S.restarts32.assign(nb, 0);
     S.iter32.assign(nb, 0);
     S.live = true;
+    // batches in FP64 or the FP32 delta form with no diagonal quadratic term take
+    // their window-edge steps on the device (HPRSOCP_DEVICE_CONTROL=0: on the host)
+    const char *env = getenv("HPRSOCP_DEVICE_CONTROL");
+    h->dctl = nb > 1 && !h->fp32 && !(h->has_quad && h->Q_is_diag) && !p.auto_save && !(env && atoi(env) == 0);
+    if (h->dctl) {
+        if (!h->ci) {
+            h->ci = h->dalloc<ctl::CtlInst>(nb);
+            h->ic = h->dalloc<ctl::InstConst>(nb);
+            h->ctl_status = h->dalloc<int32_t>(nb);
+            h->noflag = h->dalloc<int32_t>(nb);
+            HIPCHK(hipHostMalloc((void **)&h->ctl_status_h, sizeof(int32_t) * 2 * nb));
+            for (auto &e : h->ctl_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&h->stage_ev, hipEventDisableTiming));
+        }
+        std::vector<ctl::CtlInst> C((size_t)nb);
+        std::vector<ctl::InstConst> K((size_t)nb);
+        for (int i = 0; i < nb; ++i) {
+            memset(&C[i], 0, sizeof(C[i]));
+            ctl::init_restart(C[i].R);
+            C[i].R.best_sigma = h->I(i).sigma;
+            C[i].first4 = C[i].first6 = 1;
+            C[i].status = HPRSOCP_STATUS_CONTINUE;
+            K[i] = inst_const(h, i);
+        }
+        HIPCHK(hipMemcpyAsync(h->ci, C.data(), sizeof(ctl::CtlInst) * nb, hipMemcpyHostToDevice, h->st));
+        HIPCHK(hipMemcpyAsync(h->ic, K.data(), sizeof(ctl::InstConst) * nb, hipMemcpyHostToDevice, h->st));
+        h->sync();
+    }
+}
+
+// A restart makes the body after a check a full update (to_check,
+// algorithm.jl:2895-2900).  In a batch the update launches serve every
+// instance, so the instances without a restart get a full update there too;
+// their own solves run a partial one and keep the bar vectors of the previous
+// full update.  Only a residual evaluation at a max_iter that is neither a
+// check nor a print step reads those before the next full update; for that
+// case the bar vectors are saved before the update and restored afterwards for
+// the instances whose restart flag is 0.  y_bar is written by full updates only
+// in LPs and in the delta form (FP64 QP updates write it every iteration).
+static void bars_save(hprsocp_handle *h) {
+    hipStream_t st = h->st;
+    HIPCHK(hipMemcpyAsync(h->tmp_n, h->x_bar, sizeof(double) * h->n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(h->tmp_n2, h->z_bar, sizeof(double) * h->n, hipMemcpyDeviceToDevice, st));
+    if (h->m > 0) {
+        HIPCHK(hipMemcpyAsync(h->tmp_m, h->s, sizeof(double) * h->m, hipMemcpyDeviceToDevice, st));
+        if (!h->has_quad || h->delta)
+            HIPCHK(hipMemcpyAsync(h->rp, h->y_bar, sizeof(double) * h->m, hipMemcpyDeviceToDevice, st));
+    }
+}
+static void bars_restore(hprsocp_handle *h) {
+    hipStream_t st = h->st;
+    if (!h->noflag) h->noflag = h->dalloc<int32_t>(h->nb);
+    launch_flag_not(st, h->flag, h->noflag, h->nb);
+    launch_copy_flagged(st, h->seg_n, h->tmp_n, h->x_bar, h->noflag);
+    launch_copy_flagged(st, h->seg_n, h->tmp_n2, h->z_bar, h->noflag);
+    if (h->m > 0) {
+        launch_copy_flagged(st, h->seg_m, h->tmp_m, h->s, h->noflag);
+        if (!h->has_quad || h->delta) launch_copy_flagged(st, h->seg_m, h->rp, h->y_bar, h->noflag);
+    }
+}
+
+// compact_active for the device-controlled loop: staged copies in stream
+// order, the captured windows retired (the stream may still run them).
+static void compact_active_staged(hprsocp_handle *h, const std::vector<char> &keep) {
+    compact_plan_staged(h, h->hp_A, h->act_A, keep, 0);
+    compact_plan_staged(h, h->hp_AT, h->act_AT, keep, 1);
+    if (h->nnzQ > 0) compact_plan_staged(h, h->hp_Q, h->act_Q, keep, 2);
+    HIPCHK(hipEventRecord(h->stage_ev, h->st));
+    h->stage_pending = true;
+    h->use_act = true;
+    h->retire_graphs();
+}
+
+// Print steps, and the loop bodies that start with a residual evaluation:
+// checks, print steps and the last iteration (the time-limit print is
+// CtlArgs::time_up).
+static bool is_print(const hprsocp_params &p, int64_t t) {
+    return p.print_frequency == -1 ? rem_zero(t, print_step(t)) : t % p.print_frequency == 0;
+}
+static bool is_event(const hprsocp_params &p, int64_t t) {
+    return t % p.check_iter == 0 || t == p.max_iter || is_print(p, t);
+}
+
+// The solve loop of a batch with its window-edge steps on the device
+// (k_ctl_event): per event the residual reductions (and at checks the sigma
+// dots), the controller and the flag-driven restart copies are queued behind
+// the previous window, the controller's status vector is copied to pinned
+// memory, and the next window is queued before the host reads the previous
+// event's status; the stream never waits for the host.  A check body always
+// runs a full update followed by the M-norm reductions a restart would need
+// (the controller consumes them only after a restart); full and partial
+// updates give the same iterate, so every instance follows its host-controlled
+// trajectory bit for bit in FP64.  Retirements seen one event late only delay
+// the tile compaction: retired instances' tiles already skip their work.
+bool run_loop_dev(hprsocp_handle *h) {
+    const hprsocp_params &p = h->params;
+    SolveLoop &S = *h->loop;
+    const int nb = h->nb;
+    const int64_t check_iter = p.check_iter;
+    hipStream_t st = h->st;
+    std::vector<int64_t> ev_iter;   // events whose status the host has read,
+    std::vector<double> ev_time;    // and when it read them
+    std::vector<std::pair<int64_t, int64_t>> windows;  // partial windows (first body, length), FP32 in the delta form
+    std::vector<double> t_ret(nb, 0.0);
+    // retired graphs, tagged with the event after which they were retired
+    std::vector<std::pair<int64_t, size_t>> marks;
+    auto poll = [&](int64_t e, int64_t it) {
+        const int slot = (int)(e & 1);
+        HIPCHK(hipEventSynchronize(h->ctl_ev[slot]));
+        const double t = now_s() - S.t_start;
+        ev_iter.push_back(it);
+        ev_time.push_back(t);
+        const int32_t *sv = h->ctl_status_h + (int64_t)slot * nb;
+        bool retired = false;
+        for (int i = 0; i < nb; ++i) {
+            if (!S.active[i] || sv[i] == HPRSOCP_STATUS_CONTINUE) continue;
+            S.active[i] = 0;
+            h->I(i).active = 0;
+            --S.nactive;
+            t_ret[i] = t;
+            retired = true;
+        }
+        // graphs retired after event e-1 ran at the latest in the window before event e
+        size_t done = 0;
+        while (!marks.empty() && marks.front().first <= e - 1) {
+            done = marks.front().second;
+            marks.erase(marks.begin());
+        }
+        if (done > 0) {
+            for (size_t k = 0; k < done; ++k) (void)hipGraphExecDestroy(h->graphs_retired[k]);
+            h->graphs_retired.erase(h->graphs_retired.begin(), h->graphs_retired.begin() + (int64_t)done);
+            for (auto &mk : marks) mk.second -= done;
+        }
+        return retired;
+    };
+    CtlArgs a{};
+    a.ci = h->ci;
+    a.isc = h->isc;
+    a.ic = h->ic;
+    a.flag = h->flag;
+    a.status = h->ctl_status;
+    a.slots = h->dscal;
+    a.slot_stride = SL_COUNT;
+    a.sl_rn = SL_RN;
+    a.sl_rm = SL_RM;
+    a.sl_bv = SL_BV;
+    a.sl_mn0 = SL_MN0;
+    a.sl_mn1 = SL_MN1;
+    a.sl_sig = SL_SIG;
+    a.nb = nb;
+    a.check_iter = check_iter;
+    a.max_iter = p.max_iter;
+    a.stoptol = p.stoptol;
+    a.skip_q = !h->has_quad;
+    a.q_diag = h->Q_is_diag;
+    int64_t t = S.iter, prev_t = 0, e = 0;
+    bool last = false;
+    while (true) {
+        const bool check = t % check_iter == 0;
+        last = t >= p.max_iter;
+        int64_t N = t + 1;
+        while (N < p.max_iter && !is_event(p, N)) ++N;
+        h->kstep = 0;  // the controller sets InstScal::inner0 to the updates since the restart
+        residuals_enqueue(h, t);
+        if (check) sigma_dots_enqueue(h);
+        a.iter = t;
+        a.is_check = check;
+        a.next_is_check = !last && N % check_iter == 0;
+        a.time_up = now_s() - S.t_start > p.time_limit;
+        launch_ctl_event(st, a);
+        if (check) {
+            restart_copies_enqueue(h);
+            h->images_valid = false;  // delta form: anchors or sigma may have changed
+            h->fp32_dirty = true;
+        }
+        const int slot = (int)(e & 1);
+        HIPCHK(hipMemcpyAsync(h->ctl_status_h + (int64_t)slot * nb, h->ctl_status, sizeof(int32_t) * nb,
+                              hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(h->ctl_ev[slot], st));
+        if (!last) {  // bodies t .. N-1
+            // to_check (algorithm.jl:2895-2900): the body before a check or print
+            // step is a full update, the body before a max_iter that is neither is
+            // not, so that event's residuals read the bar vectors of the last full
+            // update
+            const bool n_full = N % check_iter == 0 || is_print(p, N);
+            int64_t b = t;
+            if (check) {  // the update a restart needs, bar vectors kept as above
+                if (!n_full) bars_save(h);
+                do_update(h, 1);
+                if (!n_full) bars_restore(h);
+                mnorm_enqueue(h, SL_MN0);
+                b = t + 1;
+            }
+            const int64_t b_end = n_full ? N - 1 : N;  // partial bodies b .. b_end-1
+            for (int64_t w = b; w < b_end;) {
+                const int64_t L = std::min<int64_t>(b_end - w, check_iter);
+                run_partial_updates(h, L);
+                if (h->delta) windows.emplace_back(w, L);
+                w += L;
+            }
+            if (n_full && b < N) do_update(h, 1);
+            if (N % check_iter == 0) mnorm_enqueue(h, SL_MN1);
+        }
+        HIPCHK(hipGetLastError());
+        if (e > 0 && poll(e - 1, prev_t) && S.nactive > 0) {
+            compact_active_staged(h, S.active);
+            marks.emplace_back(e, h->graphs_retired.size());
+        }
+        if (S.nactive == 0) break;
+        if (last || a.time_up) {
+            poll(e, t);
+            break;
+        }
+        prev_t = t;
+        t = N;
+        ++e;
+        if (p.verbose && check && t > 0 && (t / check_iter) % 20 == 0) {
+            printf("iter %lld: %d active (device control), %.2f s\n", (long long)t, S.nactive, now_s() - S.t_start);
+            fflush(stdout);
+        }
+    }
+    h->sync();
+    const double t_end = now_s() - S.t_start;
+    std::vector<ctl::CtlInst> C((size_t)nb);
+    HIPCHK(hipMemcpy(C.data(), h->ci, sizeof(ctl::CtlInst) * nb, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h->isc_h, h->isc, sizeof(InstScal) * nb, hipMemcpyDeviceToHost));
+    auto time_at = [&](int64_t it) {
+        const auto k = std::lower_bound(ev_iter.begin(), ev_iter.end(), it) - ev_iter.begin();
+        return k < (int64_t)ev_iter.size() ? ev_time[k] : t_end;
+    };
+    for (int i = 0; i < nb; ++i) {
+        const ctl::CtlInst &c = C[i];
+        if (c.status == HPRSOCP_STATUS_CONTINUE)
+            throw Err{HPRSOCP_ERR_STATE, "device control: instance " + std::to_string(i) + " did not terminate"};
+        const double tt = t_ret[i] > 0 ? t_ret[i] : t_end;
+        const hprsocp_residuals &r = c.rs_end;
+        hprsocp_result &o = S.out[i];
+        memset(&o, 0, sizeof(o));
+        o.iter = c.iter_end;
+        o.time = tt;
+        o.power_time = h->power_time;
+        o.setup_time = h->setup_time;
+        o.residuals = r.kkt;
+        o.primal_obj = r.primal_obj;
+        o.dual_obj = r.dual_obj;
+        o.gap = r.rel_gap;
+        o.err_Rp = r.err_Rp;
+        o.err_Rp_linear = r.err_Rp_linear;
+        o.err_Rp_soc = r.err_Rp_soc;
+        o.err_Rd = r.err_Rd;
+        o.status = c.status;
+        o.time_4 = c.first4 ? tt : time_at(c.it4);
+        o.iter_4 = c.it4 == 0 ? c.iter_end : c.it4;
+        o.time_6 = c.first6 ? tt : time_at(c.it6);
+        o.iter_6 = c.it6 == 0 ? c.iter_end : c.it6;
+        o.sigma = c.sigma_end;
+        o.lambda_max_A = h->I(i).lambda_A;
+        o.lambda_max_Q = h->I(i).lambda_Q;
+        o.restarts = (int32_t)c.restarts_end;
+        int64_t n32 = 0;
+        for (const auto &w : windows)
+            if (w.first < c.iter_end) n32 += w.second;
+        o.iter_fp32 = n32;
+        S.rs[i] = r;
+        S.R[i] = c.R;
+    }
+    if (p.verbose) {
+        printf("batch done: %d instances, %lld iterations, %.3f s (device control)\n", nb, (long long)t, t_end);
+        fflush(stdout);
+    }
+    h->use_act = false;
+    h->clear_graphs();
+    h->free_retired_graphs();
+    h->dctl = false;
+    for (int i = 0; i < nb; ++i) {  // the handle stays usable after the solve
+        h->I(i).active = 1;
+        h->I(i).fp32 = 0;
+    }
+    upload_isc(h);
+    h->images_valid = false;
+    S.nactive = 0;
+    S.live = false;
+    return true;
 }
 
 // Runs loop bodies S.iter .. stop-1 (or to termination).  Returns true when
 // every instance has terminated; S.out then holds the results.
 bool run_loop(hprsocp_handle *h, int64_t stop) {
+    if (h->dctl) return run_loop_dev(h);
     const hprsocp_params &p = h->params;
     SolveLoop &S = *h->loop;
     const double t_start = S.t_start;
@@ -3386,7 +3589,14 @@ bool run_loop(hprsocp_handle *h, int64_t stop) {
         } else {
             for (int i = 0; i < nb; ++i)
                 if (active[i] && h->I(i).fp32 && !h->delta) ++iter32[i];
+            // a full update only for the restarted instances when the next residual
+            // evaluation is a max_iter that is neither a check nor a print step
+            bool keep = nb > 1 && restarted;
+            for (int64_t q = nx; keep && q <= p.max_iter; ++q)
+                if (q % check_iter == 0 || is_print(p, q)) keep = false;
+            if (keep) bars_save(h);
             do_update(h, 1);
+            if (keep) bars_restore(h);
         }
         const bool need_cur = (nx % check_iter) == 0;
         if (restarted) {

@@ -465,6 +465,16 @@ class Solver:
         self._check(fn(self.h, int(nnz), int(width), int(slices), int(bool(sorted_chunks)), C.byref(ms)))
         return ms.value
 
+    def probe_ctl(self, xs) -> "tuple[np.ndarray, np.ndarray]":
+        """The control arithmetic of batch solves evaluated at xs on the device
+        and on the host (see include/hprsocp.h): two (len(xs), 5) arrays."""
+        x = np.ascontiguousarray(xs, dtype=np.float64)
+        dev = np.zeros((x.size, 5))
+        host = np.zeros((x.size, 5))
+        ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        self._check(self.lib.hprsocp_probe_ctl(self.h, ptr(x), x.size, ptr(dev), ptr(host)))
+        return dev, host
+
 
 def _plan_info(fn, h, check) -> dict:
     import json

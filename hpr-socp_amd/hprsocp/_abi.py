@@ -199,6 +199,7 @@ _HOOK_SIGS = {
     "time_spmv": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "probe_gather": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "probe_gather32": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    "probe_ctl": (C.c_int, [C.c_void_p, _f64p, C.c_int32, _f64p, _f64p]),
 }
 
 

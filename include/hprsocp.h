@@ -388,6 +388,14 @@ int hprsocp_probe_gather(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t 
  * sorted layout's chunks of 16384 entries (sorted = 1).  (Measurement hook.) */
 int hprsocp_probe_gather32(hprsocp_handle *h, int64_t nnz, int32_t width, int32_t slices, int32_t sorted,
                            double *ms);
+/* The control arithmetic of batch solves (residual block, M-norm, restart
+ * test, sigma update) runs on the device or on the host, from the same source
+ * with no fused multiply-add and its own exp / log; this hook evaluates the
+ * same functions of in[0..n) in both places: for each x, out[5k..5k+4] =
+ * exp(x), log|x|, sqrt|x|, a general-Q golden-section search and the sigma
+ * update of a fixed restart state.  dev_out and host_out are host arrays of 5n
+ * doubles; they agree bit for bit.  (Measurement hook.) */
+int hprsocp_probe_ctl(hprsocp_handle *h, const double *in, int32_t n, double *dev_out, double *host_out);
 
 /* ------------------------------------------------------------ cone projection */
 /* Stand-alone projection onto a product of second-order cones, on device
