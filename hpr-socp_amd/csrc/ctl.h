@@ -343,8 +343,11 @@ struct CtlInst {
     hprsocp_residuals rs_end;  // at termination
     double mn_sig[2], mn_lamA[2], mn_lamQ[2];
     double sigma_end;
+    double sm;  // the sigma model between the two phases of a check with a golden-section search
     int64_t it4, it6, last_event, iter_end, restarts_end;
     int32_t pend[2], first4, first6, status;
+    int32_t st_pend;  // termination status between the two phases
+    int32_t gold;     // the sigma model comes from this instance's golden-section walk
 };
 
 }  // namespace ctl

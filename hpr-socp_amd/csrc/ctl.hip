@@ -11,6 +11,7 @@
 
 #include "ctl.h"
 #include "ctlk.h"
+#include "kernels.h"
 
 namespace hpr {
 
@@ -18,34 +19,55 @@ __global__ __launch_bounds__(256) void k_ctl_event(CtlArgs a) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.nb; i += gridDim.x * blockDim.x) {
         ctl::CtlInst &C = a.ci[i];
         InstScal &S = a.isc[i];
-        a.flag[i] = 0;
-        if (!S.active) {
-            a.status[i] = C.status;
-            continue;
-        }
         hprsocp_restart_info &R = C.R;
-        R.inner += a.iter - C.last_event;  // updates since the previous event
-        C.last_event = a.iter;
         const double *v = a.slots + (int64_t)i * a.slot_stride;
         const ctl::InstConst &H = a.ic[i];
         const bool skip_q = a.skip_q != 0, qd = a.q_diag != 0, hm = H.has_m != 0;
-        ctl::residuals(v + a.sl_rn, v + a.sl_rm, v[a.sl_bv], S.b_scale, S.c_scale, H, skip_q, a.iter == 0, C.rs);
-        if (a.is_check) {  // deferred M-norms (perform_iteration_step!, algorithm.jl:2731-2741)
-            if (C.pend[0])
-                R.last_gap = ctl::jmin(
-                    ctl::mnorm(v + a.sl_mn0, C.mn_sig[0], C.mn_lamA[0], C.mn_lamQ[0], !skip_q, hm, qd), R.last_gap);
-            if (C.pend[1]) R.current_gap = ctl::mnorm(v + a.sl_mn1, C.mn_sig[1], C.mn_lamA[1], C.mn_lamQ[1], !skip_q, hm, qd);
-            C.pend[0] = C.pend[1] = 0;
+        if (a.phase != 2) {
+            a.flag[i] = 0;
+            if (!S.active) {
+                a.status[i] = C.status;
+                continue;
+            }
+            R.inner += a.iter - C.last_event;  // updates since the previous event
+            C.last_event = a.iter;
+            ctl::residuals(v + a.sl_rn, v + a.sl_rm, v[a.sl_bv], S.b_scale, S.c_scale, H, skip_q, a.iter == 0, C.rs);
+            if (a.is_check) {  // deferred M-norms (perform_iteration_step!, algorithm.jl:2731-2741)
+                if (C.pend[0])
+                    R.last_gap = ctl::jmin(
+                        ctl::mnorm(v + a.sl_mn0, C.mn_sig[0], C.mn_lamA[0], C.mn_lamQ[0], !skip_q, hm, qd), R.last_gap);
+                if (C.pend[1])
+                    R.current_gap = ctl::mnorm(v + a.sl_mn1, C.mn_sig[1], C.mn_lamA[1], C.mn_lamQ[1], !skip_q, hm, qd);
+                C.pend[0] = C.pend[1] = 0;
+            }
+            int32_t st = HPRSOCP_STATUS_CONTINUE;  // check_break, algorithm.jl:1169-1187
+            if (C.rs.kkt < a.stoptol) st = HPRSOCP_STATUS_OPTIMAL;
+            else if (a.iter == a.max_iter) st = HPRSOCP_STATUS_MAX_ITER;
+            else if (a.time_up) st = HPRSOCP_STATUS_TIME_LIMIT;
+            C.st_pend = st;
+            ctl::check_restart(R, a.iter, a.check_iter, S.sigma, C.rs.kkt);
+            C.gold = 0;
+            if (R.restart_flag >= 1) {  // update_sigma!, from this event's dots
+                double sa, sb, sc, sd;
+                ctl::sigma_terms(v + a.sl_sig, skip_q, hm, qd, S.lambda_A, S.lambda_Q, sa, sb, sc, sd);
+                if (!skip_q && hm && qd) {  // unified_golden_Q_diag over the device vectors, between the phases
+                    gw_init(a.gw[i], sa, sb, 1e-12, 1e12, 1e-13, kGoldenL);
+                    C.gold = 1;
+                } else {
+                    C.sm = ctl::sigma_model(sa, sb, sc, sd, skip_q, hm, qd);
+                }
+            }
+            if (a.phase == 1) continue;
+        } else if (!S.active) {
+            continue;
         }
-        int32_t st = HPRSOCP_STATUS_CONTINUE;  // check_break, algorithm.jl:1169-1187
-        if (C.rs.kkt < a.stoptol) st = HPRSOCP_STATUS_OPTIMAL;
-        else if (a.iter == a.max_iter) st = HPRSOCP_STATUS_MAX_ITER;
-        else if (a.time_up) st = HPRSOCP_STATUS_TIME_LIMIT;
-        ctl::check_restart(R, a.iter, a.check_iter, S.sigma, C.rs.kkt);
-        if (R.restart_flag >= 1) {  // update_sigma!, from this event's dots
-            double sa, sb, sc, sd;
-            ctl::sigma_terms(v + a.sl_sig, skip_q, hm, qd, S.lambda_A, S.lambda_Q, sa, sb, sc, sd);
-            const double sm = ctl::sigma_model(sa, sb, sc, sd, skip_q, hm, qd);
+        if (R.restart_flag >= 1) {
+            double sm = C.sm;
+            if (C.gold) {
+                sm = gw_result(a.gw[i]);
+                a.gw[i].active = 0;
+                C.gold = 0;
+            }
             S.sigma = ctl::sigma_next(R, C.rs, sm, S.sigma);
         }
         if (R.restart_flag > 0) {  // do_restart!: the copies run next, flag-driven
@@ -63,6 +85,7 @@ __global__ __launch_bounds__(256) void k_ctl_event(CtlArgs a) {
             C.it6 = a.iter;
             C.first6 = 0;
         }
+        const int32_t st = C.st_pend;
         if (st != HPRSOCP_STATUS_CONTINUE) {
             C.status = st;
             C.iter_end = a.iter;
@@ -94,6 +117,24 @@ void launch_ctl_event(hipStream_t st, const CtlArgs &a) {
     if (a.nb <= 0) return;
     const int blocks = (a.nb + 255) / 256;
     hipLaunchKernelGGL(k_ctl_event, dim3((unsigned)blocks), dim3(256), 0, st, a);
+}
+
+static_assert(kGoldenMaxPts <= kMaxGolden, "a walk's points fit its row of xs");
+
+__global__ void k_gold_step(GoldenWalk *gw, int32_t nb, const double *slots, int32_t stride, int32_t sl_gold,
+                            double *xs, int32_t *np) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    GoldenWalk &g = gw[i];
+    if (np[i] > 0) gw_consume(g, slots + (int64_t)i * stride + sl_gold);
+    np[i] = g.active && !gw_done(g) ? gw_next_points(g, xs + (int64_t)i * kMaxGolden) : 0;
+}
+
+void launch_gold_step(hipStream_t st, GoldenWalk *gw, int32_t nb, const double *slots, int32_t stride,
+                      int32_t sl_gold, double *xs, int32_t *np) {
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_gold_step, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, gw, nb, slots, stride,
+                       sl_gold, xs, np);
 }
 
 // InstScal::fp32 of every instance, leaving the other fields (which the

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "ctl.h"
+#include "golden.h"
 #include "hotpath.h"
 
 namespace hpr {
@@ -21,9 +22,21 @@ struct CtlArgs {
     int64_t iter, check_iter, max_iter;
     double stoptol;
     int32_t is_check, next_is_check, time_up, skip_q, q_diag;
+    // 0: the whole step; with a diagonal Q the step runs in two phases around the
+    // golden-section passes: 1 up to the sigma model (walks started in gw), 2 from
+    // the sigma update on
+    int32_t phase;
+    GoldenWalk *gw;  // nb
 };
 
 void launch_ctl_event(hipStream_t st, const CtlArgs &a);
+// One pass of the instances' golden-section walks: consume the g values of the
+// previous pass (slots[i * stride + sl_gold ..], when np[i] > 0), then write the
+// next points to xs[i * kMaxGolden ..] and their number to np[i] (0: no search
+// or done).  kGoldenPasses evaluations between kGoldenPasses + 1 steps finish
+// every walk and leave np = 0.
+void launch_gold_step(hipStream_t st, GoldenWalk *gw, int32_t nb, const double *slots, int32_t stride,
+                      int32_t sl_gold, double *xs, int32_t *np);
 void launch_isc_fp32(hipStream_t st, InstScal *isc, int32_t nb, int32_t f32);
 void launch_flag_not(hipStream_t st, const int32_t *flag, int32_t *nflag, int32_t nb);
 

@@ -89,9 +89,20 @@ struct ResidMArgs {
 };
 // ax to original units (per-instance b_scale)
 void launch_ax_to_org(hipStream_t st, const SegPlan &sp, double *ax, const double *row_norm, const InstScal *isc);
+// Constraint cones cut into pieces of at most kConePiece rows (one wavefront
+// each), so a long cone is summed by many wavefronts: piece p covers rows
+// [row[p], min(row[p] + kConePiece, end of its cone)) of cone[p]; the pieces of
+// cone c are first[c] .. first[c+1]-1, and `multi` lists the cones with more
+// than one piece (their partial sums go through part[]).
+constexpr int kConePiece = 2048;
+struct ConePieces {
+    const int32_t *cone = nullptr, *row = nullptr, *first = nullptr, *multi = nullptr;
+    int32_t npieces = 0, nmulti = 0;
+    double *part = nullptr;
+};
 void launch_cone_rp_stats(hipStream_t st, int32_t ncones, const int32_t *con_idx, const int32_t *cone_inst,
-                          const double *ax_org, const double *rhs, const double *row_norm, const InstScal *isc,
-                          double *cone_t, double *cone_ns);
+                          const ConePieces &pc, const double *ax_org, const double *rhs, const double *row_norm,
+                          const InstScal *isc, double *cone_t, double *cone_ns);
 // writes Rp and reduces per instance out[0]=<y_bar,s>, out[1]=<y_bar,rhs>, out[2]=max|Rp_lin|,
 // out[3]=max|Rp_soc|, out[4]=max|Ax_org|.
 void launch_resid_m(hipStream_t st, const SegPlan &sp, const ResidMArgs &a, double *partials, double *out,
@@ -104,9 +115,10 @@ void launch_bounds_violation(hipStream_t st, const SegPlan &sp, int32_t nvar, co
 
 // ---- golden section on diagonal Q (unified_golden_Q_diag) ----------------
 // per instance i: out[i*ostride + k] = sum_j atdy[j] * qatdy[j] / (1 + xs[i*kMaxGolden + k] * dq[j])
+// for k < min(npts, np_inst[i]) (np_inst null: npts; instances with 0 points are skipped)
 void launch_golden_eval(hipStream_t st, const SegPlan &sp, const double *atdy, const double *qatdy,
-                        const double *dq, const double *xs, int npts, double *partials, double *out,
-                        int ostride);
+                        const double *dq, const double *xs, int npts, const int32_t *np_inst, double *partials,
+                        double *out, int ostride);
 
 // ---- power iteration ------------------------------------------------------
 struct PowerState {

@@ -183,37 +183,60 @@ void launch_ax_to_org(hipStream_t st, const SegPlan &sp, double *ax, const doubl
     if (sp.nchunks > 0) hipLaunchKernelGGL(k_ax_to_org, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, ax, row_norm, isc);
 }
 
-// one wavefront per cone: t and ||xi|| of (A x)_org - rhs_org
-__global__ __launch_bounds__(kBlock) void k_cone_rp_stats(int32_t ncones, const int32_t *idx, const int32_t *cinst,
-                                                          const double *ax, const double *rhs,
-                                                          const double *rn, const InstScal *isc, double *ct,
-                                                          double *cns) {
+// t and ||xi|| of (A x)_org - rhs_org per cone, one wavefront per cone piece;
+// a cone of one piece is finished here, the others by k_cone_rp_fin.
+__global__ __launch_bounds__(kBlock) void k_cone_rp_part(const int32_t *idx, const int32_t *cinst, ConePieces pc,
+                                                         const double *ax, const double *rhs, const double *rn,
+                                                         const InstScal *isc, double *ct, double *cns) {
     const int lane = threadIdx.x & 63;
     const int64_t wid = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
-    for (int64_t c = wid; c < ncones; c += nw) {
+    for (int64_t p = wid; p < pc.npieces; p += nw) {
+        const int c = pc.cone[p];
         const int s = idx[c], e = idx[c + 1];
+        const int b = pc.row[p], pe = min(b + kConePiece, e);
         const double bs = isc[cinst ? cinst[c] : 0].b_scale;
         double sq = 0.0;
-        for (int j = s + 1 + lane; j < e; j += 64) {
+        for (int j = max(b, s + 1) + lane; j < pe; j += 64) {
             const double v = ax[j] - rhs[j] * rn[j] * bs;
             sq += v * v;
         }
         sq = wave_sum(sq);
         if (lane == 0) {
-            cns[c] = sqrt(sq);
-            ct[c] = s < e ? ax[s] - rhs[s] * rn[s] * bs : 0.0;
+            if (pc.first[c + 1] - pc.first[c] == 1) {
+                cns[c] = sqrt(sq);
+                ct[c] = s < e ? ax[s] - rhs[s] * rn[s] * bs : 0.0;
+            } else {
+                pc.part[p] = sq;
+            }
         }
     }
 }
+// the cones of several pieces: their partial sums in piece order
+__global__ __launch_bounds__(kBlock) void k_cone_rp_fin(const int32_t *idx, const int32_t *cinst, ConePieces pc,
+                                                        const double *ax, const double *rhs, const double *rn,
+                                                        const InstScal *isc, double *ct, double *cns) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= pc.nmulti) return;
+    const int c = pc.multi[k];
+    const int s = idx[c];
+    const double bs = isc[cinst ? cinst[c] : 0].b_scale;
+    double sq = 0.0;
+    for (int p = pc.first[c]; p < pc.first[c + 1]; ++p) sq += pc.part[p];
+    cns[c] = sqrt(sq);
+    ct[c] = ax[s] - rhs[s] * rn[s] * bs;
+}
 void launch_cone_rp_stats(hipStream_t st, int32_t ncones, const int32_t *con_idx, const int32_t *cone_inst,
-                          const double *ax_org, const double *rhs, const double *row_norm, const InstScal *isc,
-                          double *cone_t, double *cone_ns) {
-    if (ncones <= 0) return;
-    int blocks = (int)((ncones + 3) / 4);
+                          const ConePieces &pc, const double *ax_org, const double *rhs, const double *row_norm,
+                          const InstScal *isc, double *cone_t, double *cone_ns) {
+    if (ncones <= 0 || pc.npieces <= 0) return;
+    int blocks = (pc.npieces + 3) / 4;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_cone_rp_stats, dim3(blocks), dim3(kBlock), 0, st, ncones, con_idx, cone_inst, ax_org, rhs,
+    hipLaunchKernelGGL(k_cone_rp_part, dim3(blocks), dim3(kBlock), 0, st, con_idx, cone_inst, pc, ax_org, rhs,
                        row_norm, isc, cone_t, cone_ns);
+    if (pc.nmulti > 0)
+        hipLaunchKernelGGL(k_cone_rp_fin, dim3((pc.nmulti + kBlock - 1) / kBlock), dim3(kBlock), 0, st, con_idx,
+                           cone_inst, pc, ax_org, rhs, row_norm, isc, cone_t, cone_ns);
 }
 
 __global__ __launch_bounds__(kBlock) void k_resid_m(SegPlan sp, ResidMArgs a, double *partials) {
@@ -308,11 +331,13 @@ void launch_bounds_violation(hipStream_t st, const SegPlan &sp, int32_t nvar, co
 
 // ------------------------------------------------------------ golden section
 __global__ __launch_bounds__(kBlock) void k_golden_eval(SegPlan sp, const double *atdy, const double *qatdy,
-                                                        const double *dq, const double *xs, int npts,
-                                                        double *partials) {
+                                                        const double *dq, const double *xs, int npts_all,
+                                                        const int32_t *np_inst, double *partials) {
     __shared__ double sh[4];
     __shared__ double xsh[kMaxGolden];
     const SegChunk ch = sp.chunks[blockIdx.x];
+    const int npts = np_inst ? min(npts_all, (int)np_inst[ch.inst]) : npts_all;
+    if (npts <= 0) return;  // no search on this instance (its slots are not read)
     if ((int)threadIdx.x < npts) xsh[threadIdx.x] = xs[(int64_t)ch.inst * kMaxGolden + threadIdx.x];
     __syncthreads();
     for (int k0 = 0; k0 < npts; k0 += kMaxRed) {
@@ -332,11 +357,11 @@ __global__ __launch_bounds__(kBlock) void k_golden_eval(SegPlan sp, const double
     }
 }
 void launch_golden_eval(hipStream_t st, const SegPlan &sp, const double *atdy, const double *qatdy,
-                        const double *dq, const double *xs, int npts, double *partials, double *out,
-                        int ostride) {
+                        const double *dq, const double *xs, int npts, const int32_t *np_inst, double *partials,
+                        double *out, int ostride) {
     if (sp.nchunks > 0)
         hipLaunchKernelGGL(k_golden_eval, dim3(sp.nchunks), dim3(kBlock), 0, st, sp, atdy, qatdy, dq, xs, npts,
-                           partials);
+                           np_inst, partials);
     finish_reduce(st, sp, npts, 0u, kMaxGolden, partials, out, ostride);
 }
 

@@ -332,6 +332,7 @@ struct hprsocp_handle {
     double *fact1 = nullptr, *fact2 = nullptr, *fact = nullptr, *fact_m = nullptr;
     double *raw_n = nullptr, *raw_m = nullptr, *part_n = nullptr, *part_m = nullptr;
     double *cone_t = nullptr, *cone_ns = nullptr;
+    ConePieces cone_pieces;  // constraint cones cut for the residual's cone statistics
     double *red_part = nullptr;  // reduction partials, 3 regions of red_region doubles
     int64_t red_region = 0;
     double *dscal = nullptr;     // device scalar slots, nb x SL_COUNT
@@ -374,6 +375,8 @@ struct hprsocp_handle {
     ctl::InstConst *ic = nullptr;      // device, nb
     int32_t *ctl_status = nullptr;     // device, nb
     int32_t *noflag = nullptr;         // device, nb: !flag
+    GoldenWalk *gwalk = nullptr;       // device, nb: golden-section walks of the diagonal-Q sigma update
+    int32_t *gnp = nullptr;            // device, nb: points of each walk's current pass
     int32_t *ctl_status_h = nullptr;   // pinned, 2 x nb (one per event in flight)
     hipEvent_t ctl_ev[2] = {nullptr, nullptr};
     Tile *stage_h[3] = {nullptr, nullptr, nullptr};  // pinned staging of compacted tile lists
@@ -1942,6 +1945,27 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
     h->part_m = h->dalloc<double>(h->plan_A.nslots);
     h->cone_t = h->dalloc<double>(h->ncon);
     h->cone_ns = h->dalloc<double>(h->ncon);
+    {  // pieces of at most kConePiece rows per constraint cone (at least one per cone)
+        std::vector<int32_t> pcone, prow, first(1, 0), multi;
+        for (int32_t c = 0; c < h->ncon; ++c) {
+            const int32_t s0 = h->h_con_idx[c], e0 = h->h_con_idx[c + 1];
+            const int32_t np = std::max<int32_t>(1, (e0 - s0 + kConePiece - 1) / kConePiece);
+            for (int32_t k = 0; k < np; ++k) {
+                pcone.push_back(c);
+                prow.push_back(s0 + k * kConePiece);
+            }
+            first.push_back((int32_t)pcone.size());
+            if (np > 1) multi.push_back(c);
+        }
+        ConePieces &pc = h->cone_pieces;
+        pc.npieces = (int32_t)pcone.size();
+        pc.nmulti = (int32_t)multi.size();
+        pc.cone = h->upload<int32_t>(pcone.data(), pc.npieces);
+        pc.row = h->upload<int32_t>(prow.data(), pc.npieces);
+        pc.first = h->upload<int32_t>(first.data(), (int64_t)first.size());
+        pc.multi = h->upload<int32_t>(multi.data(), pc.nmulti);
+        pc.part = h->dalloc<double>(pc.npieces);
+    }
 
     phase("upload + plans + alloc");
     do_scaling(h);
@@ -2414,8 +2438,8 @@ static void residuals_enqueue(hprsocp_handle *h, int64_t iter) {
     if (m > 0) {
         launch_spmv(st, act ? h->act_A : h->plan_A, h->A, h->x_bar, h->ax);
         launch_ax_to_org(st, h->seg_m, h->ax, h->row_norm, h->isc);
-        launch_cone_rp_stats(st, h->ncon, h->con_idx, h->cinst_con, h->ax, h->rhs, h->row_norm, h->isc, h->cone_t,
-                             h->cone_ns);
+        launch_cone_rp_stats(st, h->ncon, h->con_idx, h->cinst_con, h->cone_pieces, h->ax, h->rhs, h->row_norm,
+                             h->isc, h->cone_t, h->cone_ns);
         ResidMArgs am{h->y_bar, h->s, h->rhs, h->AL, h->AU, h->row_norm, h->ax, h->rp, h->row_cone, h->con_idx,
                       h->cone_t, h->cone_ns, h->isc, h->m1};
         launch_resid_m(st, h->seg_m, am, region(h, 1), slot(h, SL_RM), SL_COUNT);
@@ -2684,34 +2708,44 @@ void check_restart(Restart &r, int64_t iter, int64_t check_iter, double sigma, d
     ctl::check_restart(r, iter, check_iter, sigma, kkt);
 }
 
-// unified_golden_Q_diag (unified_operations.jl:411-449) for the listed
-// instances at once: every evaluation of g(x) = sum_j ATdy_j * QATdy_j / (1 + x d_j)
-// of all their searches runs in one batched device pass.
+// The golden-section walks of the instances whose walk is active, on the
+// device: kGoldenPasses evaluations of g(x) = sum_j ATdy_j * QATdy_j / (1 + x d_j)
+// at each walk's points (unified_golden_Q_diag, unified_operations.jl:411-449),
+// one batched pass over the searching instances' vectors per 2^L - 1 points.
+void golden_passes_enqueue(hprsocp_handle *h) {
+    if (!h->gwalk) {
+        h->gwalk = h->dalloc<GoldenWalk>(h->nb);  // zeroed: no walk active
+        h->gnp = h->dalloc<int32_t>(h->nb);
+    }
+    double *xd = region(h, 2);  // nb * kMaxGolden points (region 2 is free here)
+    for (int p = 0;; ++p) {
+        launch_gold_step(h->st, h->gwalk, h->nb, h->dscal, SL_COUNT, SL_GOLD, xd, h->gnp);
+        if (p == kGoldenPasses) break;
+        launch_golden_eval(h->st, h->seg_n, h->atdy, h->qatdy, h->diag_Q, xd, (1 << kGoldenL) - 1, h->gnp,
+                           region(h, 0), slot(h, SL_GOLD), SL_COUNT);
+    }
+}
+
+// unified_golden_Q_diag for the listed instances at once, from the host.
 std::vector<double> golden_q_diag(hprsocp_handle *h, const std::vector<int> &ids, const std::vector<double> &a,
                                   const std::vector<double> &b, double lo, double hi, double tol) {
-    std::vector<GoldenWalk> walks;
-    for (size_t k = 0; k < ids.size(); ++k) walks.emplace_back(a[k], b[k], lo, hi, tol, 5);
     const int nb = h->nb;
-    std::vector<double> xs((size_t)nb * kMaxGolden, 1.0);
-    while (true) {
-        int npts = 0;
-        for (size_t k = 0; k < ids.size(); ++k) {
-            if (walks[k].done()) continue;
-            const std::vector<double> &p = walks[k].next_points();
-            std::copy(p.begin(), p.end(), xs.begin() + (int64_t)ids[k] * kMaxGolden);
-            npts = std::max(npts, (int)p.size());
-        }
-        if (npts == 0) break;
-        double *xd = region(h, 2);  // nb * kMaxGolden points (region 2 is free here)
-        HIPCHK(hipMemcpyAsync(xd, xs.data(), sizeof(double) * xs.size(), hipMemcpyHostToDevice, h->st));
-        launch_golden_eval(h->st, h->seg_n, h->atdy, h->qatdy, h->diag_Q, xd, npts, region(h, 0), slot(h, SL_GOLD),
-                           SL_COUNT);
-        read_slots(h);
-        for (size_t k = 0; k < ids.size(); ++k)
-            if (!walks[k].done()) walks[k].consume(h->hscal + (int64_t)ids[k] * SL_COUNT + SL_GOLD);
+    std::vector<GoldenWalk> W((size_t)nb);
+    memset(W.data(), 0, sizeof(GoldenWalk) * W.size());
+    for (size_t k = 0; k < ids.size(); ++k) gw_init(W[ids[k]], a[k], b[k], lo, hi, tol, kGoldenL);
+    if (!h->gwalk) {
+        h->gwalk = h->dalloc<GoldenWalk>(nb);
+        h->gnp = h->dalloc<int32_t>(nb);
     }
+    HIPCHK(hipMemcpyAsync(h->gwalk, W.data(), sizeof(GoldenWalk) * nb, hipMemcpyHostToDevice, h->st));
+    golden_passes_enqueue(h);
+    HIPCHK(hipMemcpyAsync(W.data(), h->gwalk, sizeof(GoldenWalk) * nb, hipMemcpyDeviceToHost, h->st));
+    h->sync();
     std::vector<double> r(ids.size());
-    for (size_t k = 0; k < ids.size(); ++k) r[k] = walks[k].result();
+    for (size_t k = 0; k < ids.size(); ++k) {
+        if (!gw_done(W[ids[k]])) throw Err{HPRSOCP_ERR_STATE, "golden-section search did not finish"};
+        r[k] = gw_result(W[ids[k]]);
+    }
     return r;
 }
 
@@ -3097,7 +3131,7 @@ void begin_loop(hprsocp_handle *h) {
     // batches in FP64 or the FP32 delta form with no diagonal quadratic term take
     // their window-edge steps on the device (HPRSOCP_DEVICE_CONTROL=0: on the host)
     const char *env = getenv("HPRSOCP_DEVICE_CONTROL");
-    h->dctl = nb > 1 && !h->fp32 && !(h->has_quad && h->Q_is_diag) && !p.auto_save && !(env && atoi(env) == 0);
+    h->dctl = nb > 1 && !h->fp32 && !p.auto_save && !(env && atoi(env) == 0);
     if (h->dctl) {
         if (!h->ci) {
             h->ci = h->dalloc<ctl::CtlInst>(nb);
@@ -3107,6 +3141,10 @@ void begin_loop(hprsocp_handle *h) {
             HIPCHK(hipHostMalloc((void **)&h->ctl_status_h, sizeof(int32_t) * 2 * nb));
             for (auto &e : h->ctl_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&h->stage_ev, hipEventDisableTiming));
+        }
+        if (h->has_quad && h->Q_is_diag && !h->gwalk) {
+            h->gwalk = h->dalloc<GoldenWalk>(nb);
+            h->gnp = h->dalloc<int32_t>(nb);
         }
         std::vector<ctl::CtlInst> C((size_t)nb);
         std::vector<ctl::InstConst> K((size_t)nb);
@@ -3249,6 +3287,8 @@ bool run_loop_dev(hprsocp_handle *h) {
     a.stoptol = p.stoptol;
     a.skip_q = !h->has_quad;
     a.q_diag = h->Q_is_diag;
+    a.gw = h->gwalk;
+    const bool gold = h->has_quad && h->Q_is_diag;  // checks run the golden-section passes between two phases
     int64_t t = S.iter, prev_t = 0, e = 0;
     bool last = false;
     while (true) {
@@ -3263,7 +3303,17 @@ bool run_loop_dev(hprsocp_handle *h) {
         a.is_check = check;
         a.next_is_check = !last && N % check_iter == 0;
         a.time_up = now_s() - S.t_start > p.time_limit;
-        launch_ctl_event(st, a);
+        if (check && gold) {
+            a.phase = 1;
+            launch_ctl_event(st, a);
+            golden_passes_enqueue(h);
+            a.phase = 2;
+            launch_ctl_event(st, a);
+            update_q_factors(h);  // the factors of the new sigma
+        } else {
+            a.phase = 0;
+            launch_ctl_event(st, a);
+        }
         if (check) {
             restart_copies_enqueue(h);
             h->images_valid = false;  // delta form: anchors or sigma may have changed

@@ -1,8 +1,9 @@
 """The backend's batched golden-section search (hpr-socp_amd/csrc/golden.h) visits
 exactly the points of the reference's sequential loop (unified_golden_Q_diag,
 unified_operations.jl:371-449) and returns the same sigma bit for bit, for every
-lookahead depth; the answer is the minimiser of a dense scan.  Host C++ only:
-compiled here with g++."""
+lookahead depth; the answer is the minimiser of a dense scan.  The header is
+shared with the device; here only its host side is compiled (hipcc
+--offload-host-only) and run."""
 import os
 import subprocess
 import tempfile
@@ -41,7 +42,9 @@ def driver():
     exe = os.path.join(tmp, "g")
     with open(src, "w") as f:
         f.write(DRIVER)
-    subprocess.run(["g++", "-O2", "-std=c++17", "-I", CSRC, "-o", exe, src], check=True)
+    hipcc = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc")
+    subprocess.run([hipcc, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-I", CSRC, "-o", exe, src],
+                   check=True)
     return exe
 
 

@@ -11,8 +11,10 @@ Checks:
   * in FP64 every instance of a device-controlled batch follows its
     host-controlled trajectory bit for bit: status, iteration, milestone
     iterations, restarts, sigma, residuals and solution arrays are equal, for
-    LP, sparse-Q QP and portfolio batches, at a KKT stop, at max_iter (a print
-    step, not a check) and at a time limit;
+    LP, sparse-Q QP, portfolio, diagonal-Q QP and robust-regression batches
+    (the diagonal-Q sigma update runs its golden-section walks on the device
+    between two controller phases), at a KKT stop, at max_iter (a print step,
+    not a check) and at a time limit;
   * in the FP32 delta form (where the device loop adds an FP64 step after
     every check) instances agree to rounding: iteration counts within two check
     windows, objectives to 1e-7 relative.
@@ -43,7 +45,19 @@ def portfolio_models():
     return synthetic.make_c3(count=5, n_lo=200, n_hi=2500)
 
 
-MODELS = {"lp": lp_models, "csr": csr_models, "portfolio": portfolio_models}
+def diag_models():
+    return [synthetic.make_small_qp(seed=40 + k, n1=30 + 25 * k, var_sizes=(3, 4 + k), m1=8 + 4 * k,
+                                    con_sizes=(3, 5), density=0.25, q="diag") for k in range(4)]
+
+
+def regression_models():
+    """Robust regressions (diagonal Q, one SOC of d+1 rows split over many
+    tiles), d in [2000, 6000], cold starts."""
+    return synthetic.make_c4(count=4, seed=4100, d_lo=2000, d_hi=6000)[0]
+
+
+MODELS = {"lp": lp_models, "csr": csr_models, "portfolio": portfolio_models, "diag": diag_models,
+          "regression": regression_models}
 
 
 def batch_solve(models, device, monkeypatch, warm=None, **kw):
@@ -90,7 +104,7 @@ def test_control_arithmetic_is_the_same_on_device_and_host(gpu_lib):
     np.testing.assert_array_equal(host[:, 2], np.sqrt(np.abs(xs)))
 
 
-@pytest.mark.parametrize("kind", ["lp", "csr", "portfolio"])
+@pytest.mark.parametrize("kind", ["lp", "csr", "portfolio", "diag", "regression"])
 def test_fp64_device_control_is_the_host_trajectory(kind, monkeypatch, gpu_lib):
     models = MODELS[kind]()
     print(f"[case] device vs host control, fp64 {kind} batch of {len(models)}", flush=True)
@@ -100,7 +114,7 @@ def test_fp64_device_control_is_the_host_trajectory(kind, monkeypatch, gpu_lib):
     assert_same(rd, rh, kind)
 
 
-@pytest.mark.parametrize("kind,max_iter", [("lp", 537), ("lp", 1060), ("csr", 1060)])
+@pytest.mark.parametrize("kind,max_iter", [("lp", 537), ("lp", 1060), ("csr", 1060), ("diag", 1060)])
 def test_fp64_device_control_max_iter(kind, max_iter, monkeypatch, gpu_lib):
     """A max_iter that is neither a check nor a print step: its residuals read
     the bar vectors of the last full update (the update before it is partial,
@@ -139,7 +153,7 @@ def test_fp64_device_control_warm_start(monkeypatch, gpu_lib):
     assert_same(rd, rh, "warm")
 
 
-@pytest.mark.parametrize("kind", ["lp", "portfolio"])
+@pytest.mark.parametrize("kind", ["lp", "portfolio", "regression"])
 def test_delta_device_control_agrees_to_rounding(kind, monkeypatch, gpu_lib):
     models = MODELS[kind]()
     print(f"[case] device vs host control, fp32_delta {kind} batch of {len(models)}", flush=True)
