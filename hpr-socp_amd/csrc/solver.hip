@@ -2346,6 +2346,21 @@ static void compact_active(hprsocp_handle *h) {
     h->clear_graphs();
 }
 
+// Stream capture under rocprofv3's kernel tracing has ended in a host SIGSEGV
+// inside the HIP runtime on this ROCm image, with no capturable-API misuse in
+// the captured code (kernel launches only).  A process that rocprofv3 starts
+// (it sets ROCPROFILER_LIBRARY_CTOR / ROCP_TOOL_LIBRARIES) therefore launches
+// every window eagerly: the same kernels with the same arguments, so the same
+// results, and a kernel trace that covers every update.
+static bool capture_refused() {
+    static const bool refused = [] {
+        const bool r = getenv("ROCPROFILER_LIBRARY_CTOR") || getenv("ROCP_TOOL_LIBRARIES");
+        if (r) fprintf(stderr, "hprsocp: running under rocprofv3: HIP graph capture off, windows launched eagerly\n");
+        return r;
+    }();
+    return refused;
+}
+
 // L consecutive partial updates.  With params.use_graph the run is captured
 // once per (first step, length, precision) into a HIP graph and replayed: the
 // kernels read sigma, the Halpern counter base and the retirement mask from
@@ -2359,7 +2374,7 @@ static void compact_active(hprsocp_handle *h) {
 static void run_partial_updates(hprsocp_handle *h, int64_t L) {
     const int fp = h->delta ? 4 : (h->n32 > 0 ? 1 : 0) | (h->n64 > 0 ? 2 : 0);
     const auto key = std::make_tuple(h->kstep, (int32_t)L, fp);
-    bool eager = !h->params.use_graph || L < 2;
+    bool eager = !h->params.use_graph || L < 2 || capture_refused();
     if (!eager && h->nb > 1 && !h->graphs.count(key)) eager = h->graph_seen[key]++ == 0;
     if (eager) {
         for (int64_t k = 0; k < L; ++k) do_update(h, 0);
@@ -4022,7 +4037,11 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
                 h->ev.resize(6);
                 for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
             }
-            h->profile = true;
+            struct ProfileOn {  // event records in do_update_t only for these eager launches
+                hprsocp_handle *h;
+                explicit ProfileOn(hprsocp_handle *hh) : h(hh) { h->profile = true; }
+                ~ProfileOn() { h->profile = false; }
+            } guard(h);
             double acc[6] = {0, 0, 0, 0, 0, 0};
             set_inner(h, std::vector<int64_t>(h->nb, 0));
             for (int k = 0; k < iters; ++k) {
@@ -4034,7 +4053,6 @@ int hprsocp_time_iterations(hprsocp_handle *h, int32_t iters, double *ms_per_ite
                     acc[j] += t;
                 }
             }
-            h->profile = false;
             for (int j = 0; j < 6; ++j) ms_kernel[j] = iters > 0 ? acc[j] / iters : 0.0;
         }
         if (h->n32 > 0 && !h->delta) promote_state(h);  // (the delta form's mirrors hold differences)
