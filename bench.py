@@ -256,6 +256,10 @@ def compact_plan(info):
                 rec.update({"spmv_ms_default": o["ms_default"], "spmv_ms": o["ms"], "candidates": o["candidates"]})
         if o["form"] != "plain":
             rec["spmv_ms_form"] = o["ms_form"]
+        if o.get("block"):
+            rec["sorted_block"] = o["block"]
+        if o.get("form32"):
+            rec.update({"fp32_layout": o["form32"], "fp32_block": o["block32"], "fp32_spmv_ms": o["ms32"]})
         out[o["name"]] = rec
     return out
 
@@ -462,6 +466,27 @@ def bench_batch(args, name="c3"):
                 if prec == bprecs[0]:
                     out["plan"] = compact_plan(b.plan_info())
             b.close()
+            rec["control"] = "device"
+            if prec == "fp64" and tol == tols[0]:
+                # the same solve with the window-edge steps on the host, for comparison
+                saved = os.environ.get("HPRSOCP_DEVICE_CONTROL")
+                os.environ["HPRSOCP_DEVICE_CONTROL"] = "0"
+                try:
+                    t0 = time.time()
+                    with hprsocp.BatchSolver(models, p) as bh:
+                        bh.estimate_lambda()
+                        if any(w is not None for w in warms):
+                            bh.set_initial(warms, None)
+                        t1 = time.time()
+                        rh = bh.solve()
+                        t2 = time.time()
+                finally:
+                    if saved is None:
+                        del os.environ["HPRSOCP_DEVICE_CONTROL"]
+                    else:
+                        os.environ["HPRSOCP_DEVICE_CONTROL"] = saved
+                rec["host_control"] = {"solves_per_s": round(N / (t2 - t0), 3), "solve_s": round(t2 - t1, 3),
+                                       "same_iterations": [r.iter for r in rh] == [r.iter for r in rs]}
             rt[prec] = rec
             log(f"[{name} batch {prec} @{key}] {rec}")
             if prec == "fp64":

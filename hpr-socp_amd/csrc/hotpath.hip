@@ -396,9 +396,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv_slices(CsrT<T> M, const T *__re
 // and every row sums its products in the row-major order, so the partial sums
 // are bit-identical to k_spmv_slices.  Row ranges live in registers: thread t
 // owns rows t, t + kSortBlock, ...
-template <typename T>
-__global__ __launch_bounds__(kSortBlock) void k_spmv_sorted(CsrT<T> M, const T *__restrict__ x) {
+template <typename T, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_spmv_sorted(CsrT<T> M, const T *__restrict__ x) {
     using Spec = SortSpec<T>;
+    static_assert(Spec::rows % BLOCK == 0, "rows of a slice tile per thread");
     __shared__ T prod[Spec::chunk];
     const int P = M.nslices;
     const int p = blockIdx.x % P;
@@ -411,12 +412,12 @@ __global__ __launch_bounds__(kSortBlock) void k_spmv_sorted(CsrT<T> M, const T *
     const int32_t tnnz = base[1] - k0;
     const uint16_t *off = M.s_off + (int64_t)p * M.nrows + S.rb;
     const T *__restrict__ xs = x + (int64_t)M.s_ncols * p / P;
-    constexpr int RPT = Spec::rows / kSortBlock;
+    constexpr int RPT = Spec::rows / BLOCK;
     T acc[RPT];
     int32_t ra[RPT], re[RPT];
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
-        const int r = tid + j * kSortBlock;
+        const int r = tid + j * BLOCK;
         acc[j] = T(0);
         ra[j] = r < nr ? (int32_t)off[r] : 0;
         re[j] = r < nr ? (r + 1 < nr ? (int32_t)off[r + 1] : tnnz) : 0;
@@ -429,7 +430,7 @@ __global__ __launch_bounds__(kSortBlock) void k_spmv_sorted(CsrT<T> M, const T *
     for (int c0 = 0; c0 < tnnz; c0 += chunk) {
         const int c1 = min(c0 + chunk, tnnz);
 #pragma unroll 4
-        for (int k = c0 + tid; k < c1; k += kSortBlock) {
+        for (int k = c0 + tid; k < c1; k += BLOCK) {
             const uint32_t w = __builtin_nontemporal_load(pk + k);
             const T v = __builtin_nontemporal_load(val + k);
             prod[w & pmask] = v * xs[w >> pbits];
@@ -447,7 +448,7 @@ __global__ __launch_bounds__(kSortBlock) void k_spmv_sorted(CsrT<T> M, const T *
     T *out = M.s_part + (int64_t)p * M.nrows + S.rb;
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
-        const int r = tid + j * kSortBlock;
+        const int r = tid + j * BLOCK;
         if (r < nr) out[r] = acc[j];
     }
 }
@@ -560,8 +561,11 @@ static void run_tiles(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, con
                       const Op &op, typename Op::V *raw_buf, double *partials) {
     if (plan.ntiles == 0) return;
     if (SRC == SRC_CSR && M.nslices > 1) {
-        if (M.s_ntiles > 0 && M.s_sorted)
-            hipLaunchKernelGGL((k_spmv_sorted<T>), dim3(M.s_ntiles * M.nslices), dim3(kSortBlock), 0, st, M, v);
+        if (M.s_ntiles > 0 && M.s_sorted && M.s_block == 1024)
+            hipLaunchKernelGGL((k_spmv_sorted<T, 1024>), dim3(M.s_ntiles * M.nslices), dim3(1024), 0, st, M, v);
+        else if (M.s_ntiles > 0 && M.s_sorted)
+            hipLaunchKernelGGL((k_spmv_sorted<T, kSortBlock>), dim3(M.s_ntiles * M.nslices), dim3(kSortBlock), 0, st,
+                               M, v);
         else if (M.s_ntiles > 0)
             hipLaunchKernelGGL((k_spmv_slices<T>), dim3(M.s_ntiles * M.nslices), dim3(kBlock), 0, st, M, v);
         hipLaunchKernelGGL((k_tile<T, Op, SRC_SLICES, CONES>), dim3(plan.ntiles), dim3(kBlock), 0, st, plan.tiles, M,

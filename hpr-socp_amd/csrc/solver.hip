@@ -391,6 +391,10 @@ struct hprsocp_handle {
         int32_t candidates = 0, tiles = 0;
         float ms_default = 0, ms = 0;    // plain form: default / chosen config
         float ms_form = 0;               // the chosen form (sliced or plain)
+        int32_t block = 0;               // sorted form: threads of the sorted-slice kernel (FP64)
+        std::string form32;              // FP32 copy: "own" sorted layout, "fp64" layout, or "" (none)
+        int32_t block32 = 0;             // FP32 copy: threads of its sorted-slice kernel
+        float ms32 = 0;                  // FP32 copy: the chosen layout and block
     };
     std::vector<TuneRecord> tune;
     // timing (profiled iterations)
@@ -1215,6 +1219,25 @@ hprsocp_handle::TuneRecord *tune_record(hprsocp_handle *h, const char *name) {
     return nullptr;
 }
 
+// Threads of the sorted-slice kernel for one sorted operator: 512 (each thread
+// sums more rows of a chunk) or 1024 (twice the waves to hide the gathers);
+// both sum every row in the same order.  `time` times one SpMV of the operator
+// as it stands; 1024 is kept when it is more than 2% faster.
+// HPRSOCP_SORT_BLOCK=<512|1024> forces the choice.
+template <typename T, class Time>
+float pick_sort_block(CsrT<T> &M, float t512, Time time) {
+    if (!M.s_sorted) return t512;
+    if (const char *env = getenv("HPRSOCP_SORT_BLOCK")) {
+        M.s_block = atoi(env) == 1024 ? 1024 : kSortBlock;
+        return t512;
+    }
+    M.s_block = 1024;
+    const float t1024 = time(M);
+    if (t1024 < 0.98f * t512) return t1024;
+    M.s_block = kSortBlock;
+    return t512;
+}
+
 // Autotuner of the XCD slicing: for an operator whose input vector outgrows
 // ~2 MB it builds the sliced forms with 4 and 8 column slices (8: one slice per
 // XCD; 4: two XCDs share a slice, halving the partial-sum traffic while the
@@ -1278,10 +1301,15 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
         CsrView S = M;
         if (!build_slices(h, S, hm, dval, P, cand.second)) continue;
         DevPlan splan = make_plan(h, {}, hm.nrows, lin_end, cidx, ncones, row_inst);
-        const float t = env ? 0.0f : time_one(S, splan);
+        float t = env ? 0.0f : time_one(S, splan);
+        if (env) {
+            if (const char *eb = getenv("HPRSOCP_SORT_BLOCK")) S.s_block = atoi(eb) == 1024 ? 1024 : kSortBlock;
+        } else {
+            t = pick_sort_block(S, t, [&](const CsrView &V) { return time_one(V, splan); });
+        }
         if (h->params.verbose)
-            printf("XCD slicing autotune %s: %d slices%s %.4f ms (plain %.4f ms)\n", name, P,
-                   cand.second ? " column-sorted" : "", t, forms[0].t);
+            printf("XCD slicing autotune %s: %d slices%s (%d threads) %.4f ms (plain %.4f ms)\n", name, P,
+                   cand.second ? " column-sorted" : "", S.s_block, t, forms[0].t);
         forms.push_back(Form{S, splan, t, (P >= 8 ? 0 : 2) + (cand.second ? 0 : 1), true});
     }
     float tmin = INFINITY;
@@ -1300,6 +1328,7 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
     if (hprsocp_handle::TuneRecord *rec = tune_record(h, name)) {
         if (sliced)
             rec->form = std::string(bestM.s_sorted ? "sorted" : "slices") + std::to_string(bestM.nslices);
+        rec->block = sliced && bestM.s_sorted ? bestM.s_block : 0;
         rec->ms_form = forms[pick].t;
         if (rec->ms == 0) rec->ms = forms[0].t;
     }
@@ -1444,7 +1473,8 @@ IterView<double> view64(hprsocp_handle *h) {
 // and row registers.
 static_assert(SortSpec<double>::chunk <= SortSpec<float>::chunk && SortSpec<double>::rows <= SortSpec<float>::rows,
               "the FP32 sorted-slice kernel must hold the FP64 layout's chunks");
-CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const HostCsr *hm, const DevPlan *plan) {
+CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const HostCsr *hm, const DevPlan *plan,
+                       const char *name) {
     CsrT<float> f;
     f.rowptr = M.rowptr;
     f.col = M.col;
@@ -1458,6 +1488,7 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
     f.nslices = M.nslices;
     f.s_sorted = M.s_sorted;
     f.s_pos_bits = M.s_pos_bits;
+    f.s_block = kSortBlock;
     f.s_ncols = M.s_ncols;
     f.s_base = M.s_base;
     f.s_off = M.s_off;
@@ -1468,38 +1499,55 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
     launch_demote(h->st, nnz, M.s_val, sv);
     f.s_val = sv;
     f.s_part = h->dalloc<float>((int64_t)M.nslices * M.nrows);
-    if (!hm || !plan || !build_slices<float>(h, g, *hm, f.val, M.nslices, true)) return f;
+    hprsocp_handle::TuneRecord *rec = tune_record(h, name);
+    if (!hm || !plan || !build_slices<float>(h, g, *hm, f.val, M.nslices, true)) {
+        if (rec) rec->form32 = "fp64";
+        return f;
+    }
     const char *force = getenv("HPRSOCP_FP32_OWN_LAYOUT");
     bool own = !force || atoi(force) != 0;
-    if (!force && !getenv("HPRSOCP_XCD_SLICES")) {
-        float *xin = h->dalloc<float>(hm->ncols), *yout = h->dalloc<float>(M.nrows);
-        hipEvent_t e0, e1;
+    const bool timed = !force && !getenv("HPRSOCP_XCD_SLICES");
+    float *xin = nullptr, *yout = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto time_one = [&](const CsrT<float> &V) {
+        std::vector<float> t;
+        for (int rep = 0; rep < 6; ++rep) {
+            HIPCHK(hipEventRecord(e0, h->st));
+            launch_spmv<float>(h->st, *plan, V, xin, yout);
+            HIPCHK(hipEventRecord(e1, h->st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep) t.push_back(ms);
+        }
+        std::sort(t.begin(), t.end());
+        return t[t.size() / 2];
+    };
+    float tpick = 0;
+    if (timed) {
+        xin = h->dalloc<float>(hm->ncols);
+        yout = h->dalloc<float>(M.nrows);
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        auto time_one = [&](const CsrT<float> &V) {
-            std::vector<float> t;
-            for (int rep = 0; rep < 6; ++rep) {
-                HIPCHK(hipEventRecord(e0, h->st));
-                launch_spmv<float>(h->st, *plan, V, xin, yout);
-                HIPCHK(hipEventRecord(e1, h->st));
-                HIPCHK(hipEventSynchronize(e1));
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                if (rep) t.push_back(ms);
-            }
-            std::sort(t.begin(), t.end());
-            return t[t.size() / 2];
-        };
         const float tf = time_one(f), tg = time_one(g);
         if (h->params.verbose)
             printf("FP32 column-sorted layout: %.4f ms with %d-entry chunks, %.4f ms with the FP64 layout\n", tg,
                    SortSpec<float>::chunk, tf);
         own = tg < tf;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        h->dfree(xin);
-        h->dfree(yout);
+        tpick = own ? tg : tf;
     }
+    CsrT<float> &keep = own ? g : f;
+    if (timed) tpick = pick_sort_block(keep, tpick, time_one);
+    else if (const char *eb = getenv("HPRSOCP_SORT_BLOCK")) keep.s_block = atoi(eb) == 1024 ? 1024 : kSortBlock;
+    if (rec) {
+        rec->form32 = own ? "own" : "fp64";
+        rec->block32 = keep.s_block;
+        rec->ms32 = tpick;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    h->dfree(xin);
+    h->dfree(yout);
     if (!own) {
         free_slices(h, g);
         return f;
@@ -1518,12 +1566,12 @@ struct HostOps {
 void setup_fp32(hprsocp_handle *h, const HostOps &ho) {
     IterView<float> &v = h->v32;
     const int64_t n = h->n, m = h->m;
-    v.A = csr_to_f32(h, h->A, h->nnzA, ho.A, &h->plan_A);
-    v.AT = csr_to_f32(h, h->AT, h->nnzA, ho.AT, &h->plan_AT);
-    if (h->nnzQ > 0) v.Q = csr_to_f32(h, h->Q, h->nnzQ, ho.Q, &h->plan_Q);
+    v.A = csr_to_f32(h, h->A, h->nnzA, ho.A, &h->plan_A, "A");
+    v.AT = csr_to_f32(h, h->AT, h->nnzA, ho.AT, &h->plan_AT, "A^T");
+    if (h->nnzQ > 0) v.Q = csr_to_f32(h, h->Q, h->nnzQ, ho.Q, &h->plan_Q, "Q");
     if (h->q_kind == HPRSOCP_Q_FACTORED) {
-        v.F = csr_to_f32(h, h->F, h->nnzF, ho.F, &h->plan_F);
-        v.FT = csr_to_f32(h, h->FT, h->nnzF, ho.FT, &h->plan_FT);
+        v.F = csr_to_f32(h, h->F, h->nnzF, ho.F, &h->plan_F, "F");
+        v.FT = csr_to_f32(h, h->FT, h->nnzF, ho.FT, &h->plan_FT, "F^T");
         v.Ftmp = h->dalloc<float>(h->Frows);
         if (h->Fdiag) v.Fdiag = h->dalloc<float>(n);
     }
@@ -3958,17 +4006,19 @@ int hprsocp_get_scaled_A(hprsocp_handle *h, double *val) {
 int hprsocp_plan_info(hprsocp_handle *h, char *buf, int64_t cap, int64_t *len) {
     ABI_TRY(h, {
         std::string j = "{\"operators\": [";
-        char b[512];
+        char b[1024];
         for (size_t k = 0; k < h->tune.size(); ++k) {
             const auto &r = h->tune[k];
             snprintf(b, sizeof b,
                      "%s{\"name\": \"%s\", \"form\": \"%s\", \"tuned\": %s, \"cached\": %s, "
                      "\"nnz_target\": %d, \"g_divisor\": %d, \"huge_target\": %d, \"huge_whole\": %d, "
                      "\"stream\": %d, \"tiles\": %d, \"candidates\": %d, \"ms_default\": %.5f, "
-                     "\"ms\": %.5f, \"ms_form\": %.5f}",
+                     "\"ms\": %.5f, \"ms_form\": %.5f, \"block\": %d, \"form32\": \"%s\", \"block32\": %d, "
+                     "\"ms32\": %.5f}",
                      k ? ", " : "", r.name.c_str(), r.form.c_str(), r.tuned ? "true" : "false",
                      r.cached ? "true" : "false", r.cfg.nnz_target, r.cfg.g_divisor, r.cfg.huge_target,
-                     r.cfg.huge_whole, (int)r.cfg.stream, r.tiles, r.candidates, r.ms_default, r.ms, r.ms_form);
+                     r.cfg.huge_whole, (int)r.cfg.stream, r.tiles, r.candidates, r.ms_default, r.ms, r.ms_form, r.block,
+                     r.form32.c_str(), r.block32, r.ms32);
             j += b;
         }
         j += "]}";

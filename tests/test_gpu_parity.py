@@ -35,6 +35,10 @@ MODELS = {
                                                   q="diag"),
     "huge_noq": lambda: synthetic.make_small_qp(seed=11, n1=50, var_sizes=(1500,), m1=5,
                                                 con_sizes=(600,), density=0.6, q="none"),
+    # a constraint cone of 5000 rows: the residual's cone statistics sum it in
+    # three pieces of at most 2048 rows
+    "long_cone": lambda: synthetic.make_small_qp(seed=12, n1=120, var_sizes=(3, 5), m1=20,
+                                                 con_sizes=(5000, 4), density=0.01, q="none"),
     "c1_tiny": lambda: synthetic.make_c1(scale=0.002),
     "c2_tiny": lambda: synthetic.make_c2(scale=0.004),
 }
@@ -153,7 +157,7 @@ def test_iteration_parity(name, oracle_lib, gpu_lib):
 
 
 @pytest.mark.parametrize("name", ["c0", "noq_small", "qp_diag", "qp_csr", "qp_factored",
-                                  "huge_cones", "c1_tiny", "c2_tiny"])
+                                  "huge_cones", "long_cone", "c1_tiny", "c2_tiny"])
 def test_solve_parity(name, oracle_lib, gpu_lib):
     """Same algorithm, same stoptol: the GPU and the oracle solve to the same point
     (objective to 1e-6 relative); both are within 1e-5 of a 1e-9 oracle solve."""
@@ -428,3 +432,25 @@ def test_sorted_slices_multi_chunk_rows_fp32(precision, monkeypatch, gpu_lib):
     assert a[2].iter == b[2].iter
     np.testing.assert_array_equal(a[2].x, b[2].x)
     np.testing.assert_array_equal(a[2].y, b[2].y)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_delta"])
+@pytest.mark.parametrize("name", ["c1_small", "dense_rows"])
+def test_sorted_kernel_block_sizes_agree(name, precision, monkeypatch, gpu_lib):
+    """The sorted-slice kernel runs with 512 or 1024 threads per workgroup (the
+    autotuner times both per operator): each row still sums its products in
+    row-major order, so SpMVs and solves are the same bit for bit."""
+    print(f"[case] sorted kernel 512 vs 1024 threads {name} {precision}", flush=True)
+    solve = name != "dense_rows"
+    runs = []
+    for blk in ("512", "1024"):
+        monkeypatch.setenv("HPRSOCP_SORT_BLOCK", blk)
+        runs.append(_sliced_run(name, 8 if solve else 3, True, monkeypatch, precision, solve=solve, max_iter=1500))
+    a, b = runs
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    if solve:
+        print(f"[result] {name} {precision}: iter {a[2].iter} / {b[2].iter}", flush=True)
+        assert a[2].iter == b[2].iter and a[2].primal_obj == b[2].primal_obj
+        np.testing.assert_array_equal(a[2].x, b[2].x)
+        np.testing.assert_array_equal(a[2].y, b[2].y)
