@@ -3290,6 +3290,10 @@ static bool is_event(const hprsocp_params &p, int64_t t) {
 // trajectory bit for bit in FP64.  Retirements seen one event late only delay
 // the tile compaction: retired instances' tiles already skip their work.
 bool run_loop_dev(hprsocp_handle *h) {
+    struct DevControlOff {  // the host owns the per-instance scalars again, also after an error
+        hprsocp_handle *h;
+        ~DevControlOff() { h->dctl = false; }
+    } off{h};
     const hprsocp_params &p = h->params;
     SolveLoop &S = *h->loop;
     const int nb = h->nb;
@@ -3480,7 +3484,7 @@ bool run_loop_dev(hprsocp_handle *h) {
     h->use_act = false;
     h->clear_graphs();
     h->free_retired_graphs();
-    h->dctl = false;
+    h->dctl = false;  // before upload_isc below
     for (int i = 0; i < nb; ++i) {  // the handle stays usable after the solve
         h->I(i).active = 1;
         h->I(i).fp32 = 0;
