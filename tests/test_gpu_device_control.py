@@ -168,3 +168,17 @@ def test_delta_device_control_agrees_to_rounding(kind, monkeypatch, gpu_lib):
         assert 0 < d.iter_fp32 < d.iter
         assert abs(d.primal_obj - h.primal_obj) <= 1e-7 * (1 + abs(h.primal_obj))
 
+
+
+@pytest.mark.parametrize("kind,check_iter,print_frequency,max_iter", [
+    ("lp", 30, 7, 480), ("diag", 30, 7, 481), ("lp", 1, -1, 120), ("csr", 45, 10, 450)])
+def test_fp64_device_control_event_schedules(kind, check_iter, print_frequency, max_iter, monkeypatch, gpu_lib):
+    """Other check and print schedules (every body a check; print steps between
+    checks; max_iter on a check, on a print step, and on neither): the device
+    loop's events and full updates follow the host loop's exactly."""
+    models = MODELS[kind]()
+    kw = dict(stoptol=1e-12, check_iter=check_iter, print_frequency=print_frequency, max_iter=max_iter)
+    print(f"[case] device vs host control, {kind} {kw}", flush=True)
+    rd = batch_solve(models, True, monkeypatch, **kw)
+    rh = batch_solve(models, False, monkeypatch, **kw)
+    assert_same(rd, rh, f"schedule {check_iter}/{print_frequency}/{max_iter}")
