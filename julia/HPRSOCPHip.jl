@@ -4,6 +4,8 @@
 # Included from the HPRSOCP module (include("HPRSOCPHip.jl")), it adds
 #   * HPRSOCPHip.optimize_hip(model, params): optimize() (src/algorithm.jl:2538)
 #     on the HIP backend, one call of hprsocp_optimize;
+#   * HPRSOCPHip.optimize_batch_hip(models, params): optimize() of every model,
+#     the instances iterated together (hprsocp_batch_*);
 #   * HPRSOCPHip.HipWorkspace and the seam functions hip_update!,
 #     hip_residuals!, hip_mnorm, hip_update_sigma!, hip_do_restart!,
 #     hip_iterate!: the per-step seams of solve() (src/algorithm.jl:2765-2905),
@@ -52,14 +54,13 @@ mutable struct CParams          # field order of hprsocp_params
     CParams() = new()            # filled by hprsocp_default_params
 end
 
-mutable struct CResult
+struct CResult                  # isbits: Ref{CResult}() for one result, Vector{CResult} for a batch
     iter::Int64; iter_4::Int64; iter_6::Int64
     time::Cdouble; time_4::Cdouble; time_6::Cdouble; power_time::Cdouble; setup_time::Cdouble
     primal_obj::Cdouble; dual_obj::Cdouble; residuals::Cdouble; gap::Cdouble
     err_Rp::Cdouble; err_Rp_linear::Cdouble; err_Rp_soc::Cdouble; err_Rd::Cdouble
     sigma::Cdouble; lambda_max_A::Cdouble; lambda_max_Q::Cdouble
     status::Cint; restarts::Cint; iter_fp32::Int64
-    CResult() = new()
 end
 
 mutable struct CResiduals       # hprsocp_residuals, mirrors HPRSOCP_residuals
@@ -183,7 +184,7 @@ function optimize_hip(model::QP_info_cpu, p::HPRSOCP_parameters; precision::Symb
     a = problem_arrays(model)
     m, n = size(model.A)
     x = zeros(n); y = zeros(m); z = zeros(n); w = zeros(n)
-    res = CResult()
+    res = Ref{CResult}()
     GC.@preserve a model x y z w p begin
         prob = cproblem(model, a)
         rc = ccall((:hprsocp_optimize, lib), Cint,
@@ -192,7 +193,53 @@ function optimize_hip(model::QP_info_cpu, p::HPRSOCP_parameters; precision::Symb
                    prob, cparams(p; precision), res, x, y, z, w)
         check(rc)
     end
-    return results(res, x, y, z, w)
+    return results(res[], x, y, z, w)
+end
+
+# ---------------------------------------------------------------- batch
+# optimize() of every model (each instance follows algorithm.jl:2538-2581), the
+# instances packed block-diagonally and iterated together on the backend.
+# x0s / y0s: optional per-instance initial points (`nothing` entries: cold start).
+function optimize_batch_hip(models::AbstractVector{<:QP_info_cpu}, p::HPRSOCP_parameters;
+                            precision::Symbol = :fp64, x0s = nothing, y0s = nothing)
+    k = length(models)
+    arrs = [problem_arrays(md) for md in models]
+    res = Vector{CResult}(undef, k)
+    out = HPRSOCP_results[]
+    br = Ref{Ptr{Cvoid}}(C_NULL)
+    bcheck(rc) = rc == 0 || error("hprsocp: " *
+        unsafe_string(ccall((:hprsocp_batch_last_error, lib), Cstring, (Ptr{Cvoid},), br[])))
+    GC.@preserve arrs models p x0s y0s begin
+        # one Ref per CProblem, kept alive while create copies the problems
+        refs = [Ref(cproblem(models[i], arrs[i])) for i in 1:k]
+        GC.@preserve refs begin
+            ptrs = [Base.unsafe_convert(Ptr{CProblem}, r) for r in refs]
+            check(ccall((:hprsocp_batch_create, lib), Cint,
+                        (Ptr{Ptr{CProblem}}, Cint, Ref{CParams}, Ref{Ptr{Cvoid}}),
+                        ptrs, k, cparams(p; precision), br))
+        end
+        try
+            bcheck(ccall((:hprsocp_batch_estimate_lambda, lib), Cint, (Ptr{Cvoid},), br[]))
+            if x0s !== nothing || y0s !== nothing
+                pick(v, i) = (v === nothing || v[i] === nothing) ? Ptr{Cdouble}(C_NULL) : pointer(v[i])
+                xp = [pick(x0s, i) for i in 1:k]; yp = [pick(y0s, i) for i in 1:k]
+                bcheck(ccall((:hprsocp_batch_set_initial, lib), Cint,
+                             (Ptr{Cvoid}, Ptr{Ptr{Cdouble}}, Ptr{Ptr{Cdouble}}), br[], xp, yp))
+            end
+            bcheck(ccall((:hprsocp_batch_solve, lib), Cint, (Ptr{Cvoid}, Ptr{CResult}), br[], res))
+            for i in 1:k
+                m, n = size(models[i].A)
+                x = zeros(n); y = zeros(m); z = zeros(n); w = zeros(n)
+                bcheck(ccall((:hprsocp_batch_get_solution, lib), Cint,
+                             (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+                             br[], i - 1, x, y, z, w))
+                push!(out, results(res[i], x, y, z, w))
+            end
+        finally
+            ccall((:hprsocp_batch_destroy, lib), Cvoid, (Ptr{Cvoid},), br[])
+        end
+    end
+    return out
 end
 
 # ---------------------------------------------------------------- seams
@@ -245,10 +292,10 @@ hip_do_restart!(ws::HipWorkspace, restart_info::HPRSOCP_restart) =
 
 # k bodies of the solve loop on the backend (algorithm.jl:2806-2904)
 function hip_iterate!(ws::HipWorkspace, k::Integer; reset::Bool = false)
-    res = CResult()
+    res = Ref{CResult}()
     check(ccall((:hprsocp_iterate, lib), Cint, (Ptr{Cvoid}, Int64, Cint, Ref{CResult}),
                 ws.h, k, reset ? ITERATE_RESET : Cint(0), res), ws.h)
-    return res
+    return res[]
 end
 
 function hip_solution(ws::HipWorkspace)
