@@ -97,3 +97,17 @@ def test_operator_q_is_factored_or_rejected():
     assert "HPRSOCP_Q_FACTORED" in op_branch and "hip_factored(Q)" in op_branch
     assert other.lstrip().startswith('error("HPRSOCPHip: unsupported Q type')
     assert "const HPRSOCP_Q_CSR = Cint(0)" in text and "const HPRSOCP_Q_FACTORED = Cint(1)" in text
+
+
+def test_batch_binding_uses_contiguous_results():
+    """optimize_batch_hip passes hprsocp_batch_solve a Vector{CResult}, which is one
+    contiguous C array only when CResult is an isbits (immutable) struct; it
+    creates, solves, reads every instance and destroys the batch."""
+    text = open(DOC).read()
+    assert re.search(r"^struct CResult\b", text, flags=re.M), "CResult must be immutable (isbits)"
+    body = text[text.index("function optimize_batch_hip"):text.index("# ---------------------------------------------------------------- seams")]
+    for sym in ("hprsocp_batch_create", "hprsocp_batch_estimate_lambda", "hprsocp_batch_set_initial",
+                "hprsocp_batch_solve", "hprsocp_batch_get_solution", "hprsocp_batch_destroy"):
+        assert f":{sym}" in body, sym
+    assert "Vector{CResult}(undef, k)" in body and "Ptr{CResult}" in body
+    assert "finally" in body  # the batch is destroyed on every path
