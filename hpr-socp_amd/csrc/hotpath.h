@@ -97,6 +97,9 @@ struct CsrT {
     int32_t s_sorted = 0;
     int32_t s_pos_bits = 0;
     int32_t s_block = kSortBlock;  // threads of the sorted-slice kernel: 512 or 1024 (tuned per operator)
+    // s_fused = 1 (sorted layout only): one workgroup per slice tile sums all
+    // slices in order and s_part[row] holds the finished row sums.
+    int32_t s_fused = 0;
     int32_t s_ncols = 0;
     const int32_t *s_base = nullptr;
     const uint16_t *s_off = nullptr;

@@ -18,6 +18,32 @@
 namespace hpr {
 
 // ---------------------------------------------------------------- SpMV phase
+// s + prod[a] + prod[a+1] + ... + prod[e-1], added in index order.  The LDS
+// reads are issued four at a time ahead of their adds, so a row waits for LDS
+// once per four entries (once for a row of at most three) instead of once per
+// entry; the adds keep the sequential order, so the sum is the plain loop's.
+template <typename T>
+__device__ __forceinline__ T lds_row_sum(const T *prod, int a, int e, T s) {
+    int k = a;
+    for (; k + 4 <= e; k += 4) {
+        const T p0 = prod[k], p1 = prod[k + 1], p2 = prod[k + 2], p3 = prod[k + 3];
+        s += p0;
+        s += p1;
+        s += p2;
+        s += p3;
+    }
+    const int r = e - k;
+    if (r > 0) {
+        const T p0 = prod[k];
+        const T p1 = r > 1 ? prod[k + 1] : T(0);
+        const T p2 = r > 2 ? prod[k + 2] : T(0);
+        s += p0;
+        if (r > 1) s += p1;
+        if (r > 2) s += p2;
+    }
+    return s;
+}
+
 // acc: the tile's row sums, stored as A (T, or double for the delta form's FP32 SpMV)
 template <typename T, typename A>
 __device__ void tile_spmv_stream(const CsrT<T> &M, const T *__restrict__ v, const Tile &t, A *acc, T *prod,
@@ -34,10 +60,7 @@ __device__ void tile_spmv_stream(const CsrT<T> &M, const T *__restrict__ v, cons
     for (int k = tid; k < nnz; k += kBlock) prod[k] = val[k] * v[col[k]];
     __syncthreads();
     if (tid < nrows) {
-        const int a = rp[tid] - k0, b = rp[tid + 1] - k0;
-        T s = T(0);
-        for (int k = a; k < b; ++k) s += prod[k];
-        acc[tid] = s;
+        acc[tid] = lds_row_sum(prod, rp[tid] - k0, rp[tid + 1] - k0, T(0));
     }
 }
 
@@ -372,10 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_slices(CsrT<T> M, const T *__re
         for (int j = 0; j < RPT; ++j) {
             const int r = tid + j * kBlock;
             if (r < nr) {
-                const int a = max(rps[r], c0), e = min(rps[r + 1], c1);
-                T s = acc[j];
-                for (int k = a; k < e; ++k) s += prod[k - c0];
-                acc[j] = s;
+                acc[j] = lds_row_sum(prod, max(rps[r], c0) - c0, min(rps[r + 1], c1) - c0, acc[j]);
             }
         }
         __syncthreads();
@@ -438,10 +458,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_sorted(CsrT<T> M, const T *__res
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
-            const int a = max(ra[j], c0), e = min(re[j], c1);
-            T s = acc[j];
-            for (int k = a; k < e; ++k) s += prod[k - c0];
-            acc[j] = s;
+            acc[j] = lds_row_sum(prod, max(ra[j], c0) - c0, min(re[j], c1) - c0, acc[j]);
         }
         __syncthreads();
     }
@@ -450,6 +467,86 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_sorted(CsrT<T> M, const T *__res
     for (int j = 0; j < RPT; ++j) {
         const int r = tid + j * BLOCK;
         if (r < nr) out[r] = acc[j];
+    }
+}
+
+// Slice-fused form of k_spmv_sorted (CsrT::s_fused): one workgroup per slice
+// tile walks the slices in order 0..P-1 and carries the running row sums in
+// registers, so one finished sum per row reaches memory instead of P partials
+// that the tile kernel reads back.  Each slice's products are summed exactly as
+// in k_spmv_sorted (from zero, in row-major order) and the slice sums are added
+// in slice order starting from zero, as the tile kernel adds the partials: the
+// row sums are bit-identical to the unfused form.  The XCDs no longer own one
+// slice each; locality rests on the workgroups resident on an XCD walking the
+// slices in step (they start together and carry about equal entries per
+// slice), so its L2 holds about one or two slices of the input at a time.
+template <typename T, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_spmv_sorted_fused(CsrT<T> M, const T *__restrict__ x) {
+    using Spec = SortSpec<T>;
+    static_assert(Spec::rows % BLOCK == 0, "rows of a slice tile per thread");
+    __shared__ T prod[Spec::chunk];
+    const int P = M.nslices;
+    const int tix = blockIdx.x;
+    const SliceTile S = M.s_tiles[tix];
+    const int tid = threadIdx.x;
+    const int nr = S.re - S.rb;
+    constexpr int RPT = Spec::rows / BLOCK;
+    T tot[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) tot[j] = T(0);
+    const int pbits = M.s_pos_bits, chunk = 1 << pbits;
+    const uint32_t pmask = (uint32_t)chunk - 1u;
+    for (int p = 0; p < P; ++p) {
+        const int32_t *base = M.s_base + (int64_t)p * (M.s_ntiles + 1) + tix;
+        const int32_t k0 = base[0];
+        const int32_t tnnz = base[1] - k0;
+        const uint16_t *off = M.s_off + (int64_t)p * M.nrows + S.rb;
+        const T *__restrict__ xs = x + (int64_t)M.s_ncols * p / P;
+        T acc[RPT];
+        int32_t ra[RPT], re[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int r = tid + j * BLOCK;
+            acc[j] = T(0);
+            ra[j] = r < nr ? (int32_t)off[r] : 0;
+            re[j] = r < nr ? (r + 1 < nr ? (int32_t)off[r + 1] : tnnz) : 0;
+        }
+        const T *__restrict__ val = M.s_val + k0;
+        const uint32_t *__restrict__ pk = (const uint32_t *)M.s_col + k0;
+        for (int c0 = 0; c0 < tnnz; c0 += chunk) {
+            const int c1 = min(c0 + chunk, tnnz);
+            // FP64: eight gathers in flight per thread (measured faster than four
+            // here, while the unfused kernel and the FP32 chunks keep four)
+            if constexpr (sizeof(T) == 8) {
+#pragma unroll 8
+                for (int k = c0 + tid; k < c1; k += BLOCK) {
+                    const uint32_t w = __builtin_nontemporal_load(pk + k);
+                    const T v = __builtin_nontemporal_load(val + k);
+                    prod[w & pmask] = v * xs[w >> pbits];
+                }
+            } else {
+#pragma unroll 4
+                for (int k = c0 + tid; k < c1; k += BLOCK) {
+                    const uint32_t w = __builtin_nontemporal_load(pk + k);
+                    const T v = __builtin_nontemporal_load(val + k);
+                    prod[w & pmask] = v * xs[w >> pbits];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                acc[j] = lds_row_sum(prod, max(ra[j], c0) - c0, min(re[j], c1) - c0, acc[j]);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) tot[j] += acc[j];
+    }
+    T *out = M.s_part + S.rb;
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int r = tid + j * BLOCK;
+        if (r < nr) out[r] = tot[j];
     }
 }
 
@@ -479,8 +576,9 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
     } else if constexpr (SRC == SRC_SLICES) {
         if (tid < nrows) {
             const T *pp = M.s_part + t.rb + tid;
+            const int np = M.s_fused ? 1 : M.nslices;  // fused: s_part holds the finished sums
             T a = T(0);
-            for (int p = 0; p < M.nslices; ++p) a += pp[(int64_t)p * M.nrows];
+            for (int p = 0; p < np; ++p) a += pp[(int64_t)p * M.nrows];
             acc[tid] = a;
         }
     } else {
@@ -561,7 +659,11 @@ static void run_tiles(hipStream_t st, const DevPlan &plan, const CsrT<T> &M, con
                       const Op &op, typename Op::V *raw_buf, double *partials) {
     if (plan.ntiles == 0) return;
     if (SRC == SRC_CSR && M.nslices > 1) {
-        if (M.s_ntiles > 0 && M.s_sorted && M.s_block == 1024)
+        if (M.s_ntiles > 0 && M.s_sorted && M.s_fused && M.s_block == 1024)
+            hipLaunchKernelGGL((k_spmv_sorted_fused<T, 1024>), dim3(M.s_ntiles), dim3(1024), 0, st, M, v);
+        else if (M.s_ntiles > 0 && M.s_sorted && M.s_fused)
+            hipLaunchKernelGGL((k_spmv_sorted_fused<T, kSortBlock>), dim3(M.s_ntiles), dim3(kSortBlock), 0, st, M, v);
+        else if (M.s_ntiles > 0 && M.s_sorted && M.s_block == 1024)
             hipLaunchKernelGGL((k_spmv_sorted<T, 1024>), dim3(M.s_ntiles * M.nslices), dim3(1024), 0, st, M, v);
         else if (M.s_ntiles > 0 && M.s_sorted)
             hipLaunchKernelGGL((k_spmv_sorted<T, kSortBlock>), dim3(M.s_ntiles * M.nslices), dim3(kSortBlock), 0, st,

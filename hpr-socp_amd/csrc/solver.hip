@@ -886,9 +886,15 @@ std::vector<double> power_iteration(hprsocp_handle *h, bool onA) {
 // and builds the slice row tiles.  `sorted` selects the column-sorted layout of
 // k_spmv_sorted: tiles of <= kSortRows rows whose entries in each slice fill
 // about one chunk of kSortChunk, each chunk stored in column order as packed
-// (column offset, row-major position) words.
+// (column offset, row-major position) words.  `fused_tiles` > 0 (sorted only)
+// builds the layout of the slice-fused kernel instead: at most fused_tiles
+// contiguous row tiles of about equal nonzero counts (one workgroup each walks
+// all slices of its tile), cut earlier where a tile would exceed the row or
+// per-slice chunk limits.  Row sums do not depend on the tiling: every row's
+// products are summed slice by slice in row-major order either way.
 template <typename T>
-bool build_slices(hprsocp_handle *h, CsrT<T> &M, const HostCsr &hm, const T *dval, int32_t P, bool sorted) {
+bool build_slices(hprsocp_handle *h, CsrT<T> &M, const HostCsr &hm, const T *dval, int32_t P, bool sorted,
+                  int32_t fused_tiles = 0) {
     using Spec = SortSpec<T>;
     if (P <= 1 || hm.ci.empty()) return false;
     const int32_t R = hm.nrows;
@@ -926,21 +932,41 @@ bool build_slices(hprsocp_handle *h, CsrT<T> &M, const HostCsr &hm, const T *dva
         }
     } else {
         // <= Spec::rows rows, <= Spec::chunk entries in every slice (a single
-        // longer row forms a tile of its own; the kernel then runs several chunks)
+        // longer row forms a tile of its own; the kernel then runs several chunks);
+        // fused layout: also end a tile where the running nonzero count reaches
+        // the next multiple of nnz / fused_tiles
         std::vector<int64_t> tc((size_t)P);
-        int32_t r = 0;
-        while (r < R) {
-            for (int p = 0; p < P; ++p) tc[p] = cnt[(size_t)p * R + r];
-            int32_t e = r + 1;
-            while (e < R && e - r < Spec::rows) {
-                bool fits = true;
-                for (int p = 0; p < P && fits; ++p) fits = tc[p] + cnt[(size_t)p * R + e] <= Spec::chunk;
-                if (!fits) break;
-                for (int p = 0; p < P; ++p) tc[p] += cnt[(size_t)p * R + e];
-                ++e;
+        auto cut = [&](int32_t target) {
+            tiles.clear();
+            const double per_tile = target > 0 ? (double)nnz / target : 0.0;
+            int32_t r = 0;
+            while (r < R) {
+                for (int p = 0; p < P; ++p) tc[p] = cnt[(size_t)p * R + r];
+                int64_t goal = INT64_MAX;
+                if (target > 0) {
+                    const int64_t k = (int64_t)std::floor((double)hm.rp[r] / per_tile) + 1;
+                    goal = std::max<int64_t>((int64_t)std::llround(k * per_tile), (int64_t)hm.rp[r] + 1);
+                }
+                int32_t e = r + 1;
+                while (e < R && e - r < Spec::rows && hm.rp[e] < goal) {
+                    bool fits = true;
+                    for (int p = 0; p < P && fits; ++p) fits = tc[p] + cnt[(size_t)p * R + e] <= Spec::chunk;
+                    if (!fits) break;
+                    for (int p = 0; p < P; ++p) tc[p] += cnt[(size_t)p * R + e];
+                    ++e;
+                }
+                tiles.push_back(SliceTile{r, e});
+                r = e;
             }
-            tiles.push_back(SliceTile{r, e});
-            r = e;
+        };
+        cut(fused_tiles);
+        // the fused kernel's tiles must not outnumber its workgroup slots (a
+        // second round of a few workgroups doubles the time): lower the target
+        // by the excess until they fit, or give the layout up
+        for (int32_t target = fused_tiles; fused_tiles > 0 && (int32_t)tiles.size() > fused_tiles;) {
+            target -= (int32_t)tiles.size() - fused_tiles;
+            if (target < 1) return false;
+            cut(target);
         }
     }
     const int32_t NT = (int32_t)tiles.size();
@@ -1005,6 +1031,7 @@ bool build_slices(hprsocp_handle *h, CsrT<T> &M, const HostCsr &hm, const T *dva
     }
     M.nslices = P;
     M.s_sorted = sorted ? 1 : 0;
+    M.s_fused = sorted && fused_tiles > 0 ? 1 : 0;
     M.s_pos_bits = sorted ? Spec::pos_bits : 0;
     M.s_ncols = hm.ncols;
     M.s_base = h->upload<int32_t>(base.data(), (int64_t)base.size());
@@ -1238,6 +1265,16 @@ float pick_sort_block(CsrT<T> &M, float t512, Time time) {
     return t512;
 }
 
+// Workgroup slots of the sorted-slice kernels on the handle's device: two per
+// CU (each workgroup holds a 64 KB LDS product buffer).  The slice-fused layout
+// is cut into a multiple of this count so that every CU carries equal work.
+int32_t sorted_slots(hprsocp_handle *h) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || ncu <= 0)
+        ncu = 256;
+    return 2 * ncu;
+}
+
 // Autotuner of the XCD slicing: for an operator whose input vector outgrows
 // ~2 MB it builds the sliced forms with 4 and 8 column slices (8: one slice per
 // XCD; 4: two XCDs share a slice, halving the partial-sum traffic while the
@@ -1252,13 +1289,20 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
                  const std::vector<int32_t> *row_inst) {
     const char *env = getenv("HPRSOCP_XCD_SLICES");
     const char *env_sorted = getenv("HPRSOCP_XCD_SORTED");
+    const char *env_fused = getenv("HPRSOCP_SLICE_FUSED");
     if (hm.ci.empty()) return;
-    std::vector<std::pair<int32_t, bool>> cands;
+    struct Cand {
+        int32_t P;
+        bool sorted;
+        int32_t fused;  // slice-fused layout with fused * sorted_slots tiles (0: unfused)
+    };
+    std::vector<Cand> cands;
     if (env) {
         const int32_t P = std::max(1, atoi(env));
-        if (P > 1) cands.push_back({P, env_sorted && atoi(env_sorted) != 0});
+        const bool srt = env_sorted && atoi(env_sorted) != 0;
+        if (P > 1) cands.push_back({P, srt, srt && env_fused ? std::max(0, atoi(env_fused)) : 0});
     } else if (hm.ncols * (int64_t)sizeof(double) > (2ll << 20)) {
-        cands = {{4, false}, {8, false}, {4, true}, {8, true}};
+        cands = {{4, false, 0}, {8, false, 0}, {4, true, 0}, {8, true, 0}, {8, true, 1}, {8, true, 2}};
     }
     if (cands.empty()) return;
     double *xin = nullptr, *yout = nullptr;
@@ -1285,7 +1329,7 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
         HIPCHK(hipEventCreate(&e1));
     }
     // every candidate is timed; the fastest wins unless a form earlier in the
-    // preference order (8 sorted, 8, 4 sorted, 4, plain) is within 2% of it,
+    // preference order (8 sorted, 8, 4 sorted, 4, plain, slice-fused) is within 2% of it,
     // which keeps the choice stable between runs when forms time alike
     struct Form {
         CsrView M;
@@ -1295,11 +1339,11 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
         bool sliced;
     };
     std::vector<Form> forms;
-    forms.push_back(Form{M, plan, env ? INFINITY : time_one(M, plan), 4, false});
+    forms.push_back(Form{M, plan, env ? INFINITY : time_one(M, plan), 4, false});  // plain
     for (const auto &cand : cands) {
-        const int32_t P = cand.first;
+        const int32_t P = cand.P;
         CsrView S = M;
-        if (!build_slices(h, S, hm, dval, P, cand.second)) continue;
+        if (!build_slices(h, S, hm, dval, P, cand.sorted, cand.fused * sorted_slots(h))) continue;
         DevPlan splan = make_plan(h, {}, hm.nrows, lin_end, cidx, ncones, row_inst);
         float t = env ? 0.0f : time_one(S, splan);
         if (env) {
@@ -1308,9 +1352,12 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
             t = pick_sort_block(S, t, [&](const CsrView &V) { return time_one(V, splan); });
         }
         if (h->params.verbose)
-            printf("XCD slicing autotune %s: %d slices%s (%d threads) %.4f ms (plain %.4f ms)\n", name, P,
-                   cand.second ? " column-sorted" : "", S.s_block, t, forms[0].t);
-        forms.push_back(Form{S, splan, t, (P >= 8 ? 0 : 2) + (cand.second ? 0 : 1), true});
+            printf("XCD slicing autotune %s: %d slices%s%s (%d tiles, %d threads) %.4f ms (plain %.4f ms)\n", name,
+                   P, cand.sorted ? " column-sorted" : "", S.s_fused ? " slice-fused" : "", S.s_ntiles, S.s_block, t,
+                   forms[0].t);
+        // preference order on near-ties: 8 sorted, 8, 4 sorted, 4, plain, slice-fused
+        const int rank = cand.fused ? 4 + cand.fused : (P >= 8 ? 0 : 2) + (cand.sorted ? 0 : 1);
+        forms.push_back(Form{S, splan, t, rank, true});
     }
     float tmin = INFINITY;
     for (const Form &f : forms) tmin = std::min(tmin, f.t);
@@ -1327,7 +1374,8 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
     const DevPlan bestP = forms[pick].plan;
     if (hprsocp_handle::TuneRecord *rec = tune_record(h, name)) {
         if (sliced)
-            rec->form = std::string(bestM.s_sorted ? "sorted" : "slices") + std::to_string(bestM.nslices);
+            rec->form = std::string(bestM.s_sorted ? "sorted" : "slices") + std::to_string(bestM.nslices) +
+                        (bestM.s_fused ? "-fused" : "");
         rec->block = sliced && bestM.s_sorted ? bestM.s_block : 0;
         rec->ms_form = forms[pick].t;
         if (rec->ms == 0) rec->ms = forms[0].t;
@@ -1499,14 +1547,37 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
     launch_demote(h->st, nnz, M.s_val, sv);
     f.s_val = sv;
     f.s_part = h->dalloc<float>((int64_t)M.nslices * M.nrows);
+    f.s_fused = M.s_fused;
     hprsocp_handle::TuneRecord *rec = tune_record(h, name);
     if (!hm || !plan || !build_slices<float>(h, g, *hm, f.val, M.nslices, true)) {
-        if (rec) rec->form32 = "fp64";
+        if (rec) rec->form32 = M.s_fused ? "fp64-fused" : "fp64";
         return f;
     }
+    // the own layout also in the slice-fused tiling (as many tiles as the FP64
+    // operator's fused layout, else one per workgroup slot)
+    CsrT<float> g2 = g;
+    g2.nslices = 1;
+    g2.s_ntiles = 0;
+    {
+        CsrT<float> t = f;
+        t.nslices = 1;
+        t.s_ntiles = 0;
+        t.s_sorted = 0;
+        t.s_fused = 0;
+        t.s_base = nullptr;
+        t.s_off = nullptr;
+        t.s_col = nullptr;
+        t.s_val = nullptr;
+        t.s_tiles = nullptr;
+        t.s_part = nullptr;
+        if (build_slices<float>(h, t, *hm, f.val, M.nslices, true, M.s_fused ? M.s_ntiles : sorted_slots(h))) g2 = t;
+    }
+    const bool have_g2 = g2.s_ntiles > 0;
     const char *force = getenv("HPRSOCP_FP32_OWN_LAYOUT");
-    bool own = !force || atoi(force) != 0;
+    const char *force_fused = getenv("HPRSOCP_SLICE_FUSED");
     const bool timed = !force && !getenv("HPRSOCP_XCD_SLICES");
+    // 0: FP64 layout, 1: own layout, 2: own slice-fused layout
+    int pick = !force || atoi(force) != 0 ? (force_fused && atoi(force_fused) != 0 && have_g2 ? 2 : 1) : 0;
     float *xin = nullptr, *yout = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     auto time_one = [&](const CsrT<float> &V) {
@@ -1529,18 +1600,21 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
         yout = h->dalloc<float>(M.nrows);
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        const float tf = time_one(f), tg = time_one(g);
+        const float tf = time_one(f), tg = time_one(g), tg2 = have_g2 ? time_one(g2) : INFINITY;
         if (h->params.verbose)
-            printf("FP32 column-sorted layout: %.4f ms with %d-entry chunks, %.4f ms with the FP64 layout\n", tg,
-                   SortSpec<float>::chunk, tf);
-        own = tg < tf;
-        tpick = own ? tg : tf;
+            printf("FP32 column-sorted layout: %.4f ms with %d-entry chunks, %.4f ms slice-fused (%d tiles), %.4f ms "
+                   "with the FP64 layout%s\n",
+                   tg, SortSpec<float>::chunk, tg2, g2.s_ntiles, tf, f.s_fused ? " (slice-fused)" : "");
+        // near-ties (2%) keep the own unfused layout, then the FP64 one
+        const float tmin = std::min(tf, std::min(tg, tg2));
+        pick = tg <= 1.02f * tmin ? 1 : (tf <= 1.02f * tmin ? 0 : 2);
+        tpick = pick == 0 ? tf : (pick == 1 ? tg : tg2);
     }
-    CsrT<float> &keep = own ? g : f;
+    CsrT<float> &keep = pick == 0 ? f : (pick == 1 ? g : g2);
     if (timed) tpick = pick_sort_block(keep, tpick, time_one);
     else if (const char *eb = getenv("HPRSOCP_SORT_BLOCK")) keep.s_block = atoi(eb) == 1024 ? 1024 : kSortBlock;
     if (rec) {
-        rec->form32 = own ? "own" : "fp64";
+        rec->form32 = std::string(pick == 0 ? "fp64" : "own") + (keep.s_fused ? "-fused" : "");
         rec->block32 = keep.s_block;
         rec->ms32 = tpick;
     }
@@ -1548,13 +1622,12 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
     if (e1) (void)hipEventDestroy(e1);
     h->dfree(xin);
     h->dfree(yout);
-    if (!own) {
-        free_slices(h, g);
-        return f;
-    }
+    if (pick != 1) free_slices(h, g);
+    if (pick != 2 && have_g2) free_slices(h, g2);
+    if (pick == 0) return f;
     h->dfree(f.s_val);
     h->dfree(f.s_part);
-    return g;
+    return keep;
 }
 
 // Host structure of the operators (for the FP32 sorted layouts).
@@ -4011,6 +4084,8 @@ int hprsocp_plan_info(hprsocp_handle *h, char *buf, int64_t cap, int64_t *len) {
     ABI_TRY(h, {
         std::string j = "{\"operators\": [";
         char b[1024];
+        // forms forced by the environment are not timed (infinite times): reported as 0
+        auto fin = [](float v) { return std::isfinite(v) ? (double)v : 0.0; };
         for (size_t k = 0; k < h->tune.size(); ++k) {
             const auto &r = h->tune[k];
             snprintf(b, sizeof b,
@@ -4021,8 +4096,8 @@ int hprsocp_plan_info(hprsocp_handle *h, char *buf, int64_t cap, int64_t *len) {
                      "\"ms32\": %.5f}",
                      k ? ", " : "", r.name.c_str(), r.form.c_str(), r.tuned ? "true" : "false",
                      r.cached ? "true" : "false", r.cfg.nnz_target, r.cfg.g_divisor, r.cfg.huge_target,
-                     r.cfg.huge_whole, (int)r.cfg.stream, r.tiles, r.candidates, r.ms_default, r.ms, r.ms_form, r.block,
-                     r.form32.c_str(), r.block32, r.ms32);
+                     r.cfg.huge_whole, (int)r.cfg.stream, r.tiles, r.candidates, fin(r.ms_default), fin(r.ms),
+                     fin(r.ms_form), r.block, r.form32.c_str(), r.block32, fin(r.ms32));
             j += b;
         }
         j += "]}";

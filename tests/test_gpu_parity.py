@@ -454,3 +454,51 @@ def test_sorted_kernel_block_sizes_agree(name, precision, monkeypatch, gpu_lib):
         assert a[2].iter == b[2].iter and a[2].primal_obj == b[2].primal_obj
         np.testing.assert_array_equal(a[2].x, b[2].x)
         np.testing.assert_array_equal(a[2].y, b[2].y)
+
+
+@pytest.mark.parametrize("fused", ["1", "2"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine", "fp32_delta"])
+@pytest.mark.parametrize("name", ["c1_small", "c2_tiny", "qp_factored", "dense_rows"])
+def test_slice_fused_bit_identical(name, precision, fused, monkeypatch, gpu_lib):
+    """The slice-fused sorted kernel (one workgroup walks every slice of its row
+    tile and writes finished row sums; tiles cut to 1 or 2 per workgroup slot)
+    sums each slice's products in row-major order and the slices in slice order,
+    as the unfused kernel and the tile kernel's partial sum do: SpMVs and whole
+    solves equal the unfused sorted form bit for bit, with 512 and 1024 threads,
+    and the FP32 copies (own layout or the FP64 one) agree the same way."""
+    print(f"[case] slice-fused vs unfused sorted {name} {precision} tiles x{fused}", flush=True)
+    solve = name != "dense_rows"
+    P = 8 if solve else 3
+    monkeypatch.setenv("HPRSOCP_FP32_OWN_LAYOUT", "1")
+    monkeypatch.setenv("HPRSOCP_SLICE_FUSED", "0")
+    a = _sliced_run(name, P, True, monkeypatch, precision, solve=solve, max_iter=1500)
+    monkeypatch.setenv("HPRSOCP_SLICE_FUSED", fused)
+    runs = []
+    for blk, own in (("512", "1"), ("1024", "1"), ("512", "0")):
+        monkeypatch.setenv("HPRSOCP_SORT_BLOCK", blk)
+        monkeypatch.setenv("HPRSOCP_FP32_OWN_LAYOUT", own)
+        runs.append(_sliced_run(name, P, True, monkeypatch, precision, solve=solve, max_iter=1500))
+    for b in runs:
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+        if solve:
+            assert np.all(np.isfinite(a[2].x)) and np.all(np.isfinite(a[2].y))
+            assert a[2].iter == b[2].iter and a[2].primal_obj == b[2].primal_obj
+            np.testing.assert_array_equal(a[2].x, b[2].x)
+            np.testing.assert_array_equal(a[2].y, b[2].y)
+    if solve:
+        print(f"[result] {name} {precision}: iter {a[2].iter} status {a[2].status}", flush=True)
+
+
+def test_slice_fused_form_reported(monkeypatch, gpu_lib):
+    """A forced slice-fused layout shows in the plan report (form "sorted8-fused",
+    FP32 copy "own-fused"), and the report is valid JSON with finite times."""
+    monkeypatch.setenv("HPRSOCP_XCD_SLICES", "8")
+    monkeypatch.setenv("HPRSOCP_XCD_SORTED", "1")
+    monkeypatch.setenv("HPRSOCP_SLICE_FUSED", "1")
+    with hprsocp.Solver(_slice_model("c1_small"), params(precision="fp32_delta")) as g:
+        info = g.plan_info()
+    ops = {o["name"]: o for o in info["operators"]}
+    assert ops["A"]["form"] == "sorted8-fused" and ops["A^T"]["form"] == "sorted8-fused"
+    assert ops["A"]["form32"] == "own-fused"
+    assert all(np.isfinite(o["ms_form"]) for o in info["operators"])
