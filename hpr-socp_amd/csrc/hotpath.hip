@@ -44,6 +44,36 @@ __device__ __forceinline__ T lds_row_sum(const T *prod, int a, int e, T s) {
     return s;
 }
 
+// The row sums of a thread's RPT rows over one chunk (row j: chunk entries
+// [max(ra[j], c0), min(re[j], c1)) at prod[k - c0]), rows taken four at a time:
+// the first two products of all four rows are read before any add (short rows,
+// the common case, then cost one LDS wait per four rows), then each row goes
+// on in order.  Every row's adds keep the sequential order.
+template <typename T, int RPT>
+__device__ __forceinline__ void lds_rows_sum(const T *prod, const int32_t *ra, const int32_t *re, int c0, int c1,
+                                             T *acc) {
+    static_assert(RPT % 4 == 0, "rows in groups of four");
+#pragma unroll
+    for (int g = 0; g < RPT; g += 4) {
+        T q0[4], q1[4];
+        int a[4], e[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            a[j] = max(ra[g + j], c0) - c0;
+            e[j] = min(re[g + j], c1) - c0;
+            q0[j] = a[j] < e[j] ? prod[a[j]] : T(0);
+            q1[j] = a[j] + 1 < e[j] ? prod[a[j] + 1] : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            T sj = acc[g + j];
+            if (a[j] < e[j]) sj += q0[j];
+            if (a[j] + 1 < e[j]) sj += q1[j];
+            acc[g + j] = lds_row_sum(prod, a[j] + 2, e[j], sj);
+        }
+    }
+}
+
 // acc: the tile's row sums, stored as A (T, or double for the delta form's FP32 SpMV)
 template <typename T, typename A>
 __device__ void tile_spmv_stream(const CsrT<T> &M, const T *__restrict__ v, const Tile &t, A *acc, T *prod,
@@ -533,10 +563,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_sorted_fused(CsrT<T> M, const T 
                 }
             }
             __syncthreads();
-#pragma unroll
-            for (int j = 0; j < RPT; ++j) {
-                acc[j] = lds_row_sum(prod, max(ra[j], c0) - c0, min(re[j], c1) - c0, acc[j]);
-            }
+            lds_rows_sum<T, RPT>(prod, ra, re, c0, c1, acc);
             __syncthreads();
         }
 #pragma unroll
