@@ -58,13 +58,21 @@ PEAK_HBM_GBS = 8000.0
 # passes (FETCH_SIZE, WRITE_SIZE) over this script, each scaled by what the same
 # counter reads for the 1 GiB plain copy of hprsocp_probe_copy (read factor 2.0,
 # write factor 1.0).  Source: profiles/c1_fp64_iteration_hbm_traffic.txt.
-# Keyed by (config, precision, group, SpMV form of A that the autotuner kept,
-# kernel revision); a run whose form differs reports null instead of a stale
-# figure, and KERNEL_REV changes whenever the launch group's kernels change.
-KERNEL_REV = "rev2"
+# Keyed by (config, precision, group, SpMV form that the autotuner kept for the
+# group's operator (A for dual, A^T for w2), kernel revision); a run whose form
+# differs reports null instead of a stale figure, and KERNEL_REV changes
+# whenever the launch group's kernels change.
+KERNEL_REV = "rev3"
 MEASURED_TRAFFIC = {
-    ("c1", "fp64", "dual", "sorted8", "rev2"): 537.6e6 + 172.3e6,  # k_spmv_sorted(A) + k_tile<OpDual, slices>
+    # k_spmv_sorted_fused (A, slice-fused) + k_tile<OpDual, slices>
+    ("c1", "fp64", "dual", "sorted8-fused", "rev3"): 496.5e6 + 81.1e6,
+    # k_spmv_sorted (A, 8 slice partials) + k_tile<OpDual, slices>
+    ("c1", "fp64", "dual", "sorted8", "rev3"): 541.5e6 + 172.4e6,
+    # the same for A^T + k_tile<OpW2, slices>
+    ("c1", "fp64", "w2", "sorted8-fused", "rev3"): 552.9e6 + 64.5e6,
+    ("c1", "fp64", "w2", "sorted8", "rev3"): 508.1e6 + 120.5e6,
 }
+GROUP_OPERATOR = {"dual": "A", "w2": "A^T"}
 
 
 def log(msg):
@@ -319,7 +327,8 @@ def bench_single(args):
     dom_bytes = bytes_by[dom]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     it_bytes = sum(bytes_by.values())
-    a_form = compact_plan(plan).get("A", {}).get("form") if plan else None
+    a_form = (compact_plan(plan).get(GROUP_OPERATOR.get(dom, "A"), {}).get("form")
+              if plan else None)
     traffic = (MEASURED_TRAFFIC.get((args.config, precs[0], dom, a_form, KERNEL_REV))
                if args.scale == 1.0 else None)
 
