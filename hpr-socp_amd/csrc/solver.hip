@@ -1555,10 +1555,7 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
     }
     // the own layout also in the slice-fused tiling (as many tiles as the FP64
     // operator's fused layout, else one per workgroup slot)
-    CsrT<float> g2 = g;
-    g2.nslices = 1;
-    g2.s_ntiles = 0;
-    {
+    auto fused_own = [&](int32_t ntiles) {
         CsrT<float> t = f;
         t.nslices = 1;
         t.s_ntiles = 0;
@@ -1570,8 +1567,10 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
         t.s_val = nullptr;
         t.s_tiles = nullptr;
         t.s_part = nullptr;
-        if (build_slices<float>(h, t, *hm, f.val, M.nslices, true, M.s_fused ? M.s_ntiles : sorted_slots(h))) g2 = t;
-    }
+        if (!build_slices<float>(h, t, *hm, f.val, M.nslices, true, ntiles)) t.s_ntiles = 0;
+        return t;
+    };
+    CsrT<float> g2 = fused_own(M.s_fused ? M.s_ntiles : sorted_slots(h));
     const bool have_g2 = g2.s_ntiles > 0;
     const char *force = getenv("HPRSOCP_FP32_OWN_LAYOUT");
     const char *force_fused = getenv("HPRSOCP_SLICE_FUSED");
@@ -1600,7 +1599,12 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
         yout = h->dalloc<float>(M.nrows);
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        const float tf = time_one(f), tg = time_one(g), tg2 = have_g2 ? time_one(g2) : INFINITY;
+        // every layout with its faster block size
+        float tf = time_one(f), tg = time_one(g);
+        float tg2 = have_g2 ? time_one(g2) : INFINITY;
+        tf = pick_sort_block(f, tf, time_one);
+        tg = pick_sort_block(g, tg, time_one);
+        if (have_g2) tg2 = pick_sort_block(g2, tg2, time_one);
         if (h->params.verbose)
             printf("FP32 column-sorted layout: %.4f ms with %d-entry chunks, %.4f ms slice-fused (%d tiles), %.4f ms "
                    "with the FP64 layout%s\n",
@@ -1611,8 +1615,8 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
         tpick = pick == 0 ? tf : (pick == 1 ? tg : tg2);
     }
     CsrT<float> &keep = pick == 0 ? f : (pick == 1 ? g : g2);
-    if (timed) tpick = pick_sort_block(keep, tpick, time_one);
-    else if (const char *eb = getenv("HPRSOCP_SORT_BLOCK")) keep.s_block = atoi(eb) == 1024 ? 1024 : kSortBlock;
+    if (!timed)
+        if (const char *eb = getenv("HPRSOCP_SORT_BLOCK")) keep.s_block = atoi(eb) == 1024 ? 1024 : kSortBlock;
     if (rec) {
         rec->form32 = std::string(pick == 0 ? "fp64" : "own") + (keep.s_fused ? "-fused" : "");
         rec->block32 = keep.s_block;
