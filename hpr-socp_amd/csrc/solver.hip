@@ -1267,7 +1267,8 @@ float pick_sort_block(CsrT<T> &M, float t512, Time time) {
 
 // Workgroup slots of the sorted-slice kernels on the handle's device: two per
 // CU (each workgroup holds a 64 KB LDS product buffer).  The slice-fused layout
-// is cut into a multiple of this count so that every CU carries equal work.
+// is cut into at most one (or two) tiles per slot, so that all its workgroups
+// run in one round (or two) with about equal work.
 int32_t sorted_slots(hprsocp_handle *h) {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || ncu <= 0)
