@@ -299,6 +299,7 @@ struct hprsocp_handle {
     InstScal *isc = nullptr;          // device, nb
     InstScal *isc_h = nullptr;        // pinned host mirror
     SegPlan seg_n, seg_m;             // per-instance chunks of columns / rows
+    std::vector<int32_t> seg_n_first, seg_m_first;  // host copies of their first-chunk tables
     int32_t max_chunks = 1;
     int32_t *cinst_con = nullptr, *cinst_var = nullptr;  // instance of each cone
     int32_t *ccnt_con = nullptr, *ccnt_var = nullptr;    // cone count of that instance
@@ -1438,9 +1439,9 @@ int validate_csr(std::string &err, const char *name, int32_t nr, int32_t nc, int
 }
 
 // ------------------------------------------------------------- instances
-SegPlan make_seg(hprsocp_handle *h, bool cols) {
+SegPlan make_seg(hprsocp_handle *h, bool cols, std::vector<int32_t> &first) {
     std::vector<SegChunk> ch;
-    std::vector<int32_t> first;
+    first.clear();
     for (int i = 0; i < h->nb; ++i) {
         first.push_back((int32_t)ch.size());
         const InstRange &R = h->rng[i];
@@ -1462,8 +1463,8 @@ SegPlan make_seg(hprsocp_handle *h, bool cols) {
 // Per-instance device state: chunk plans, cone -> instance maps, scalars, slots.
 void setup_instances(hprsocp_handle *h) {
     const int nb = h->nb;
-    h->seg_n = make_seg(h, true);
-    h->seg_m = make_seg(h, false);
+    h->seg_n = make_seg(h, true, h->seg_n_first);
+    h->seg_m = make_seg(h, false, h->seg_m_first);
     h->max_chunks = std::max(1, std::max(h->seg_n.nchunks, h->seg_m.nchunks));
     std::vector<int32_t> ci((size_t)std::max(h->ncon, 1), 0), cc((size_t)std::max(h->ncon, 1), h->ncon);
     std::vector<int32_t> vi((size_t)std::max(h->nvar, 1), 0), vc((size_t)std::max(h->nvar, 1), h->nvar);
@@ -3025,37 +3026,46 @@ void restart_copies_enqueue(hprsocp_handle *h) {
 // device vectors sx, sw, sz (n) and sy (m): the maps x = b_scale D_C xbar,
 // w = b_scale D_C wbar, z = c_scale D_C^{-1} zbar, y = c_scale D_R ybar,
 // gathered from the instance's column / row ranges.  Any output may be null.
+// Only instance i's chunks are unscaled, and the copies wait on one final
+// synchronisation (stream order keeps each copy ahead of the next kernel that
+// reuses the staging vector).
 void fetch_unscaled(hprsocp_handle *h, int i, const double *sx, const double *sw, const double *sz, const double *sy,
                     double *x, double *y, double *z, double *w) {
     hipStream_t st = h->st;
     const InstRange &R = h->rng[i];
     const InstVal bsv = isc_field(h, &h->isc->b_scale), csv = isc_field(h, &h->isc->c_scale);
+    auto sub = [&](const SegPlan &sp, const std::vector<int32_t> &first) {
+        SegPlan s1 = sp;
+        s1.chunks = sp.chunks + first[i];
+        s1.nchunks = first[i + 1] - first[i];
+        return s1;
+    };
+    const SegPlan sn = sub(h->seg_n, h->seg_n_first), sm = sub(h->seg_m, h->seg_m_first);
     auto fetch_n = [&](double *dst) {
         const int32_t nbx = R.box1 - R.box0, nvr = R.var1 - R.var0;
         if (nbx > 0) HIPCHK(hipMemcpyAsync(dst, h->tmp_n + R.box0, sizeof(double) * nbx, hipMemcpyDeviceToHost, st));
         if (nvr > 0)
             HIPCHK(hipMemcpyAsync(dst + nbx, h->tmp_n + R.var0, sizeof(double) * nvr, hipMemcpyDeviceToHost, st));
-        h->sync();
     };
     if (x) {
-        launch_unscale(st, h->seg_n, sx, h->col_norm, bsv, 0, h->tmp_n);
+        launch_unscale(st, sn, sx, h->col_norm, bsv, 0, h->tmp_n);
         fetch_n(x);
     }
     if (w) {
-        launch_unscale(st, h->seg_n, sw, h->col_norm, bsv, 0, h->tmp_n);
+        launch_unscale(st, sn, sw, h->col_norm, bsv, 0, h->tmp_n);
         fetch_n(w);
     }
     if (z) {
-        launch_unscale(st, h->seg_n, sz, h->col_norm, csv, 1, h->tmp_n);
+        launch_unscale(st, sn, sz, h->col_norm, csv, 1, h->tmp_n);
         fetch_n(z);
     }
     if (y && R.m() > 0) {
-        launch_unscale(st, h->seg_m, sy, h->row_norm, csv, 0, h->tmp_m);
+        launch_unscale(st, sm, sy, h->row_norm, csv, 0, h->tmp_m);
         const int32_t nl = R.lin1 - R.lin0, nc = R.con1 - R.con0;
         if (nl > 0) HIPCHK(hipMemcpyAsync(y, h->tmp_m + R.lin0, sizeof(double) * nl, hipMemcpyDeviceToHost, st));
         if (nc > 0) HIPCHK(hipMemcpyAsync(y + nl, h->tmp_m + R.con0, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
-        h->sync();
     }
+    h->sync();
 }
 
 // The best-iterate record of compute_residuals! (algorithm.jl:176-195): at
