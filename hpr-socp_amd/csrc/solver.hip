@@ -893,9 +893,13 @@ std::vector<double> power_iteration(hprsocp_handle *h, bool onA) {
 // all slices of its tile), cut earlier where a tile would exceed the row or
 // per-slice chunk limits.  Row sums do not depend on the tiling: every row's
 // products are summed slice by slice in row-major order either way.
+// `round_tiles` > 0 (sorted, unfused): cut the tiles nnz-balanced into the
+// next multiple of round_tiles at or above the natural count, so that the
+// P workgroups per tile fill whole rounds of the device's workgroup slots
+// instead of ending on a partial round.
 template <typename T>
 bool build_slices(hprsocp_handle *h, CsrT<T> &M, const HostCsr &hm, const T *dval, int32_t P, bool sorted,
-                  int32_t fused_tiles = 0) {
+                  int32_t fused_tiles = 0, int32_t round_tiles = 0) {
     using Spec = SortSpec<T>;
     if (P <= 1 || hm.ci.empty()) return false;
     const int32_t R = hm.nrows;
@@ -968,6 +972,21 @@ bool build_slices(hprsocp_handle *h, CsrT<T> &M, const HostCsr &hm, const T *dva
             target -= (int32_t)tiles.size() - fused_tiles;
             if (target < 1) return false;
             cut(target);
+        }
+        if (fused_tiles == 0 && round_tiles > 0) {
+            const int32_t natural = (int32_t)tiles.size();
+            const int32_t want = (natural + round_tiles - 1) / round_tiles * round_tiles;
+            if (want > natural) {
+                cut(want);
+                for (int32_t target = want; (int32_t)tiles.size() > want;) {
+                    target -= (int32_t)tiles.size() - want;
+                    if (target < 1) {
+                        cut(0);  // keep the natural tiling
+                        break;
+                    }
+                    cut(target);
+                }
+            }
         }
     }
     const int32_t NT = (int32_t)tiles.size();
@@ -1293,18 +1312,22 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
     const char *env_sorted = getenv("HPRSOCP_XCD_SORTED");
     const char *env_fused = getenv("HPRSOCP_SLICE_FUSED");
     if (hm.ci.empty()) return;
+    const char *env_rounds = getenv("HPRSOCP_SLICE_ROUNDS");
     struct Cand {
         int32_t P;
         bool sorted;
         int32_t fused;  // slice-fused layout with fused * sorted_slots tiles (0: unfused)
+        bool rounds;    // unfused sorted layout cut into whole rounds of workgroup slots
     };
     std::vector<Cand> cands;
     if (env) {
         const int32_t P = std::max(1, atoi(env));
         const bool srt = env_sorted && atoi(env_sorted) != 0;
-        if (P > 1) cands.push_back({P, srt, srt && env_fused ? std::max(0, atoi(env_fused)) : 0});
+        const int32_t fz = srt && env_fused ? std::max(0, atoi(env_fused)) : 0;
+        if (P > 1) cands.push_back({P, srt, fz, srt && fz == 0 && env_rounds && atoi(env_rounds) != 0});
     } else if (hm.ncols * (int64_t)sizeof(double) > (2ll << 20)) {
-        cands = {{4, false, 0}, {8, false, 0}, {4, true, 0}, {8, true, 0}, {8, true, 1}, {8, true, 2}};
+        cands = {{4, false, 0, false}, {8, false, 0, false}, {4, true, 0, false}, {8, true, 0, false},
+                 {8, true, 1, false},  {8, true, 2, false},  {8, true, 0, true}};
     }
     if (cands.empty()) return;
     double *xin = nullptr, *yout = nullptr;
@@ -1339,13 +1362,16 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
         float t;
         int rank;
         bool sliced;
+        bool rounds = false;
     };
     std::vector<Form> forms;
     forms.push_back(Form{M, plan, env ? INFINITY : time_one(M, plan), 4, false});  // plain
     for (const auto &cand : cands) {
         const int32_t P = cand.P;
         CsrView S = M;
-        if (!build_slices(h, S, hm, dval, P, cand.sorted, cand.fused * sorted_slots(h))) continue;
+        if (!build_slices(h, S, hm, dval, P, cand.sorted, cand.fused * sorted_slots(h),
+                          cand.rounds ? std::max(1, sorted_slots(h) / P) : 0))
+            continue;
         DevPlan splan = make_plan(h, {}, hm.nrows, lin_end, cidx, ncones, row_inst);
         float t = env ? 0.0f : time_one(S, splan);
         if (env) {
@@ -1354,12 +1380,12 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
             t = pick_sort_block(S, t, [&](const CsrView &V) { return time_one(V, splan); });
         }
         if (h->params.verbose)
-            printf("XCD slicing autotune %s: %d slices%s%s (%d tiles, %d threads) %.4f ms (plain %.4f ms)\n", name,
-                   P, cand.sorted ? " column-sorted" : "", S.s_fused ? " slice-fused" : "", S.s_ntiles, S.s_block, t,
-                   forms[0].t);
-        // preference order on near-ties: 8 sorted, 8, 4 sorted, 4, plain, slice-fused
-        const int rank = cand.fused ? 4 + cand.fused : (P >= 8 ? 0 : 2) + (cand.sorted ? 0 : 1);
-        forms.push_back(Form{S, splan, t, rank, true});
+            printf("XCD slicing autotune %s: %d slices%s%s%s (%d tiles, %d threads) %.4f ms (plain %.4f ms)\n", name,
+                   P, cand.sorted ? " column-sorted" : "", S.s_fused ? " slice-fused" : "",
+                   cand.rounds ? " whole rounds" : "", S.s_ntiles, S.s_block, t, forms[0].t);
+        // preference order on near-ties: 8 sorted, 8, 4 sorted, 4, plain, slice-fused, whole rounds
+        const int rank = cand.rounds ? 7 : cand.fused ? 4 + cand.fused : (P >= 8 ? 0 : 2) + (cand.sorted ? 0 : 1);
+        forms.push_back(Form{S, splan, t, rank, true, cand.rounds});
     }
     float tmin = INFINITY;
     for (const Form &f : forms) tmin = std::min(tmin, f.t);
@@ -1377,7 +1403,7 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
     if (hprsocp_handle::TuneRecord *rec = tune_record(h, name)) {
         if (sliced)
             rec->form = std::string(bestM.s_sorted ? "sorted" : "slices") + std::to_string(bestM.nslices) +
-                        (bestM.s_fused ? "-fused" : "");
+                        (bestM.s_fused ? "-fused" : "") + (forms[pick].rounds ? "-rounds" : "");
         rec->block = sliced && bestM.s_sorted ? bestM.s_block : 0;
         rec->ms_form = forms[pick].t;
         if (rec->ms == 0) rec->ms = forms[0].t;
@@ -1557,7 +1583,7 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
     }
     // the own layout also in the slice-fused tiling (as many tiles as the FP64
     // operator's fused layout, else one per workgroup slot)
-    auto fused_own = [&](int32_t ntiles) {
+    auto fused_own = [&](int32_t ntiles, int32_t round_tiles = 0) {
         CsrT<float> t = f;
         t.nslices = 1;
         t.s_ntiles = 0;
@@ -1569,15 +1595,21 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
         t.s_val = nullptr;
         t.s_tiles = nullptr;
         t.s_part = nullptr;
-        if (!build_slices<float>(h, t, *hm, f.val, M.nslices, true, ntiles)) t.s_ntiles = 0;
+        if (!build_slices<float>(h, t, *hm, f.val, M.nslices, true, ntiles, round_tiles)) t.s_ntiles = 0;
         return t;
     };
     CsrT<float> g2 = fused_own(M.s_fused ? M.s_ntiles : sorted_slots(h));
-    const bool have_g2 = g2.s_ntiles > 0;
+    // and the unfused own layout cut into whole rounds of workgroup slots
+    CsrT<float> g3 = fused_own(0, std::max(1, sorted_slots(h) / M.nslices));
+    if (g3.s_ntiles > 0 && g3.s_ntiles == g.s_ntiles) {  // the natural tiling already fills whole rounds
+        free_slices(h, g3);
+        g3.s_ntiles = 0;
+    }
+    const bool have_g2 = g2.s_ntiles > 0, have_g3 = g3.s_ntiles > 0;
     const char *force = getenv("HPRSOCP_FP32_OWN_LAYOUT");
     const char *force_fused = getenv("HPRSOCP_SLICE_FUSED");
     const bool timed = !force && !getenv("HPRSOCP_XCD_SLICES");
-    // 0: FP64 layout, 1: own layout, 2: own slice-fused layout
+    // 0: FP64 layout, 1: own layout, 2: own slice-fused layout, 3: own layout in whole rounds
     int pick = !force || atoi(force) != 0 ? (force_fused && atoi(force_fused) != 0 && have_g2 ? 2 : 1) : 0;
     float *xin = nullptr, *yout = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1603,24 +1635,27 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
         HIPCHK(hipEventCreate(&e1));
         // every layout with its faster block size
         float tf = time_one(f), tg = time_one(g);
-        float tg2 = have_g2 ? time_one(g2) : INFINITY;
+        float tg2 = have_g2 ? time_one(g2) : INFINITY, tg3 = have_g3 ? time_one(g3) : INFINITY;
         tf = pick_sort_block(f, tf, time_one);
         tg = pick_sort_block(g, tg, time_one);
         if (have_g2) tg2 = pick_sort_block(g2, tg2, time_one);
+        if (have_g3) tg3 = pick_sort_block(g3, tg3, time_one);
         if (h->params.verbose)
-            printf("FP32 column-sorted layout: %.4f ms with %d-entry chunks, %.4f ms slice-fused (%d tiles), %.4f ms "
-                   "with the FP64 layout%s\n",
-                   tg, SortSpec<float>::chunk, tg2, g2.s_ntiles, tf, f.s_fused ? " (slice-fused)" : "");
-        // near-ties (2%) keep the own unfused layout, then the FP64 one
-        const float tmin = std::min(tf, std::min(tg, tg2));
-        pick = tg <= 1.02f * tmin ? 1 : (tf <= 1.02f * tmin ? 0 : 2);
-        tpick = pick == 0 ? tf : (pick == 1 ? tg : tg2);
+            printf("FP32 column-sorted layout: %.4f ms with %d-entry chunks (%d tiles), %.4f ms in whole rounds (%d "
+                   "tiles), %.4f ms slice-fused (%d tiles), %.4f ms with the FP64 layout%s\n",
+                   tg, SortSpec<float>::chunk, g.s_ntiles, tg3, g3.s_ntiles, tg2, g2.s_ntiles, tf,
+                   f.s_fused ? " (slice-fused)" : "");
+        // near-ties (2%) keep the own unfused layout, then the FP64 one, then the others
+        const float tmin = std::min(std::min(tf, tg), std::min(tg2, tg3));
+        pick = tg <= 1.02f * tmin ? 1 : (tf <= 1.02f * tmin ? 0 : (tg2 <= 1.02f * tmin ? 2 : 3));
+        tpick = pick == 0 ? tf : (pick == 1 ? tg : (pick == 2 ? tg2 : tg3));
     }
-    CsrT<float> &keep = pick == 0 ? f : (pick == 1 ? g : g2);
+    CsrT<float> &keep = pick == 0 ? f : (pick == 1 ? g : (pick == 2 ? g2 : g3));
     if (!timed)
         if (const char *eb = getenv("HPRSOCP_SORT_BLOCK")) keep.s_block = atoi(eb) == 1024 ? 1024 : kSortBlock;
     if (rec) {
-        rec->form32 = std::string(pick == 0 ? "fp64" : "own") + (keep.s_fused ? "-fused" : "");
+        rec->form32 = std::string(pick == 0 ? "fp64" : "own") + (keep.s_fused ? "-fused" : "") +
+                      (pick == 3 ? "-rounds" : "");
         rec->block32 = keep.s_block;
         rec->ms32 = tpick;
     }
@@ -1630,6 +1665,7 @@ CsrT<float> csr_to_f32(hprsocp_handle *h, const CsrView &M, int64_t nnz, const H
     h->dfree(yout);
     if (pick != 1) free_slices(h, g);
     if (pick != 2 && have_g2) free_slices(h, g2);
+    if (pick != 3 && have_g3) free_slices(h, g3);
     if (pick == 0) return f;
     h->dfree(f.s_val);
     h->dfree(f.s_part);

@@ -502,3 +502,25 @@ def test_slice_fused_form_reported(monkeypatch, gpu_lib):
     assert ops["A"]["form"] == "sorted8-fused" and ops["A^T"]["form"] == "sorted8-fused"
     assert ops["A"]["form32"] == "own-fused"
     assert all(np.isfinite(o["ms_form"]) for o in info["operators"])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_delta"])
+@pytest.mark.parametrize("name", ["c1_small", "qp_factored", "dense_rows"])
+def test_sorted_whole_rounds_bit_identical(name, precision, monkeypatch, gpu_lib):
+    """The unfused sorted layout cut into whole rounds of workgroup slots (more,
+    nnz-balanced row tiles) sums every row as the natural tiling does: SpMVs and
+    solves are the same bit for bit."""
+    print(f"[case] sorted slices in whole rounds vs natural tiles {name} {precision}", flush=True)
+    solve = name != "dense_rows"
+    P = 8 if solve else 3
+    monkeypatch.setenv("HPRSOCP_SLICE_FUSED", "0")
+    monkeypatch.setenv("HPRSOCP_SLICE_ROUNDS", "0")
+    a = _sliced_run(name, P, True, monkeypatch, precision, solve=solve, max_iter=1500)
+    monkeypatch.setenv("HPRSOCP_SLICE_ROUNDS", "1")
+    b = _sliced_run(name, P, True, monkeypatch, precision, solve=solve, max_iter=1500)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    if solve:
+        assert a[2].iter == b[2].iter and a[2].primal_obj == b[2].primal_obj
+        np.testing.assert_array_equal(a[2].x, b[2].x)
+        np.testing.assert_array_equal(a[2].y, b[2].y)
