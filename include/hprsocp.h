@@ -329,14 +329,15 @@ int hprsocp_set_scalars(hprsocp_handle *h, double sigma, double lambda_A, double
 int hprsocp_get_scaled_A(hprsocp_handle *h, double *val);
 /* Launch-plan choices of the autotuner for every operator of the handle, as a
  * JSON object {"operators": [{name, form (plain | slicesP | sortedP |
- * sortedP-fused), tuned, cached, nnz_target, g_divisor, huge_target,
+ * sortedP-fused | sortedP-rounds), tuned, cached, nnz_target, g_divisor, huge_target,
  * huge_whole, stream, tiles, candidates, ms_default, ms, ms_form, block,
  * form32, block32, ms32}, ...]}: the tile-plan configuration
  * chosen by the timed search (the reference's fixed launch shapes,
  * kernels.jl:349-357, and cone size classes, algorithm.jl:1914-1919), the SpMV
  * time of the default and the chosen plan, the SpMV form kept by the XCD
  * slicing tuner with its block size, and the layout of the FP32 copy (form32:
- * own | fp64, with -fused for a slice-fused layout); times of forms forced
+ * own | fp64, with -fused for a slice-fused layout and -rounds for one cut
+ * into whole rounds of workgroup slots); times of forms forced
  * through the environment read 0.  Writes at most cap bytes (NUL-terminated)
  * to buf and the full length to *len (either may be NULL). */
 int hprsocp_plan_info(hprsocp_handle *h, char *buf, int64_t cap, int64_t *len);
