@@ -597,21 +597,53 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
     if (!op.bind(t.inst)) return;  // retired instance (uniform over the workgroup)
     const int tid = threadIdx.x;
     const int nrows = t.re - t.rb;
+    // raw_in_acc: acc holds the rows' raw values (not their products) for the cone loop
+    bool raw_in_acc = false;
     if constexpr (SRC == SRC_CSR) {
         if (t.g == 0) tile_spmv_stream(M, vin, t, acc, prod, rps);
         else tile_spmv(M, vin, t, acc, sh);
-    } else if constexpr (SRC == SRC_SLICES) {
-        if (tid < nrows) {
-            const T *pp = M.s_part + t.rb + tid;
-            const int np = M.s_fused ? 1 : M.nslices;  // fused: s_part holds the finished sums
-            T a = T(0);
-            for (int p = 0; p < np; ++p) a += pp[(int64_t)p * M.nrows];
-            acc[tid] = a;
-        }
+        __syncthreads();
     } else {
-        if (tid < nrows) acc[tid] = T(0);
+        // one row per thread: the row's product stays in a register, and a row
+        // that no other thread reads (linear rows, HUGE segments) takes no LDS
+        // round trip and no barrier, so its loads of the product and of the
+        // epilogue's operands go out together
+        V a = V(0);
+        if constexpr (SRC == SRC_SLICES) {
+            if (tid < nrows) {
+                const T *pp = M.s_part + t.rb + tid;
+                const int np = M.s_fused ? 1 : M.nslices;  // fused: s_part holds the finished sums
+                T sum = T(0);
+                for (int p = 0; p < np; ++p) sum += pp[(int64_t)p * M.nrows];
+                a = sum;
+            }
+        }
+        if (!CONES || t.kind == TILE_LINEAR) {
+            if (tid < nrows) {
+                const int i = t.rb + tid;
+                const V r = op.raw(i, a, false);
+                op.finish(i, r, op.lin_proj(i, r));
+            }
+            return;
+        }
+        if (t.kind == TILE_HUGE) {
+            // HUGE segment, phase 1: raw values and a partial |xi|^2
+            double sq = 0.0;
+            if (tid < nrows) {
+                const int i = t.rb + tid;
+                const V r = op.raw(i, a, true);
+                raw_buf[i] = r;
+                if (i != huge.start[t.c0]) sq = (double)r * (double)r;
+            }
+            sq = block_sum(sq, sh);
+            if (tid == 0) partials[t.c1] = sq;
+            return;
+        }
+        // cone rows: raw values formed here, read by the cones' lane groups
+        if (tid < nrows) acc[tid] = op.raw(t.rb + tid, a, true);
+        raw_in_acc = true;
+        __syncthreads();
     }
-    __syncthreads();
     if (!CONES || t.kind == TILE_LINEAR) {
         if (tid < nrows) {
             const int i = t.rb + tid;
@@ -627,7 +659,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
             const int s = cone_idx[c], e = cone_idx[c + 1];
             double sq = 0.0, head = 0.0;
             for (int i = s + lane; i < e; i += G2) {
-                const V r = op.raw(i, acc[i - t.rb], true);
+                const V r = raw_in_acc ? acc[i - t.rb] : op.raw(i, acc[i - t.rb], true);
                 acc[i - t.rb] = r;
                 if (i == s) head = (double)r;
                 else sq += (double)r * (double)r;
