@@ -98,10 +98,13 @@ __device__ __forceinline__ double ival(InstVal v, int inst) { return v.p[(int64_
 #define GRID_STRIDE(i, n) \
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < (n); i += (int64_t)gridDim.x * kBlock)
 
+// One element per thread: each workgroup streams one contiguous block.  A
+// 1 GiB copy runs at 6.2 TB/s this way on MI355X, 4.2-5.8 TB/s with a grid
+// of 8-64 workgroups per CU striding over the array.
 static inline int ew_blocks(int64_t n) {
     int64_t b = (n + kBlock - 1) / kBlock;
     if (b < 1) b = 1;
-    if (b > 8192) b = 8192;
+    if (b > (1 << 30)) b = 1 << 30;
     return (int)b;
 }
 
