@@ -79,11 +79,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes_dims(m, n, nnz, m1, n1, Q, s=8):
+def algorithmic_bytes_dims(m, n, nnz, m1, n1, Q, s=8, fused_qw=False):
     """Compulsory HBM bytes per partial HPR iteration, per launch group (s = value bytes).
 
     Each matrix is read once per SpMV (s + 4 bytes per nonzero, 4 bytes per row
-    pointer), each vector once per kernel; gathered SpMV inputs count once."""
+    pointer), each vector once per kernel; gathered SpMV inputs count once.
+    fused_qw: the primal step runs in the F^T product's kernel (groups "Fw" and
+    "FTw_primal"; Q w is written once and not read back, w is read once)."""
     m2 = m - m1
     out = {}
     if isinstance(Q, tuple):
@@ -91,12 +93,16 @@ def algorithmic_bytes_dims(m, n, nnz, m1, n1, Q, s=8):
         mf, nnzf = F.shape[0], F.nnz
         f1 = nnzf * (s + 4) + 4 * (mf + 1) + s * (n + mf)             # F v
         f2 = nnzf * (s + 4) + 4 * (n + 1) + s * (mf + 3 * n)          # F'(.) + D v
-        out["Qw"] = f1 + f2
-        out["qp_primal"] = s * (8 * n + 3 * n)
+        if fused_qw:
+            out["Fw"] = f1
+            out["FTw_primal"] = f2 + s * 9 * n
+        else:
+            out["Qw"] = f1 + f2
+            out["qp_primal"] = s * (8 * n + 3 * n)
         out["tempv"] = f1 + f2 + s * 3 * n                            # + x_hat, qw, tempv
         out["dual"] = nnz * (s + 4) + 4 * (m + 1) + s * n + s * (4 * m + 2 * m1 + m2)
         out["w2"] = nnz * (s + 4) + 4 * (n + 1) + s * m + s * 7 * n
-        order = ["Qw", "qp_primal", "tempv", "dual", "w2"]
+        order = (["Fw", "FTw_primal"] if fused_qw else ["Qw", "qp_primal"]) + ["tempv", "dual", "w2"]
     elif Q is not None and getattr(Q, "nnz", 0) > 0:
         out["Qw"] = 0
         out["qp_primal"] = s * 12 * n + s * 2 * n1
@@ -111,10 +117,10 @@ def algorithmic_bytes_dims(m, n, nnz, m1, n1, Q, s=8):
     return out, order
 
 
-def algorithmic_bytes(model, s=8):
+def algorithmic_bytes(model, s=8, fused_qw=False):
     m, n = model.A.shape
     return algorithmic_bytes_dims(m, n, model.A.nnz, model.number_linear_con, model.number_lu_x,
-                                  model.Q, s)
+                                  model.Q, s, fused_qw)
 
 
 def make_workload(name, scale):
@@ -321,7 +327,8 @@ def bench_single(args):
     base = res[precs[0]]
     ms = base["ms_per_iter"]
     vbytes = 8 if precs[0] == "fp64" else 4
-    bytes_by, order = algorithmic_bytes(model, vbytes)
+    fused_qw = bool(plan and plan.get("qw_primal_fused"))
+    bytes_by, order = algorithmic_bytes(model, vbytes, fused_qw)
     groups = [(name, base["kern"][i]) for i, name in enumerate(order)]
     dom, dom_ms = max(groups, key=lambda t: t[1])
     dom_bytes = bytes_by[dom]
@@ -346,7 +353,7 @@ def bench_single(args):
     forms = {}
     for prec, rec in res.items():
         vb = 8 if prec == "fp64" else 4
-        bb, od = algorithmic_bytes(model, vb)
+        bb, od = algorithmic_bytes(model, vb, fused_qw)
         forms[prec] = {"iters_per_s": round(1000.0 / rec["ms_per_iter"], 2),
                        "ms_per_iter": round(rec["ms_per_iter"], 5),
                        "groups": group_roofline(bb, od, rec["kern"]),
@@ -384,7 +391,7 @@ def bench_single(args):
                    "nnz_A": int(model.A.nnz), "precision": precs[0],
                    "algorithmic_MB_per_iter": round(it_bytes / 1e6, 2),
                    "iteration_GBps": round(it_bytes / (ms * 1e-3) / 1e9, 1),
-                   "plan": compact_plan(plan) if plan else None},
+                   "plan": compact_plan(plan) if plan else None, "qw_primal_fused": fused_qw},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic, "traffic_key": [args.config, precs[0], dom, a_form, KERNEL_REV],

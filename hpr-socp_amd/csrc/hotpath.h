@@ -213,6 +213,13 @@ void launch_dual(hipStream_t st, const DevPlan &plan, const CsrT<T> &A, const in
 template <typename T>
 void launch_qp_primal(hipStream_t st, const DevPlan &plan, const int32_t *var_idx,
                       const QPPrimalArgsT<T> &a, T *raw_buf, double *partials);
+// The QP primal step fused behind the F^T product of a factored Q = F^T F + D
+// (problems without variable cones): planFT / FT / Ftmp = F w as for
+// launch_spmv; Q w = F^T Ftmp + fdiag .* w is formed per variable, stored to
+// qw and used at once.  Same values as launch_spmv followed by launch_qp_primal.
+template <typename T>
+void launch_qp_primal_fq(hipStream_t st, const DevPlan &planFT, const CsrT<T> &FT, const T *Ftmp, const T *fdiag,
+                         T *qw, const QPPrimalArgsT<T> &a);
 template <typename T>
 void launch_w2(hipStream_t st, const DevPlan &plan, const CsrT<T> &AT, const W2ArgsT<T> &a);
 // out = M v (+ diag .* v2 when diag != null); plan must be built for M.
@@ -228,6 +235,10 @@ void launch_dprimal(hipStream_t st, const DevPlan &plan, const CsrT<float> &AT, 
                     const DPrimalArgs &a, bool qp, double *raw_buf, double *partials);
 void launch_ddual(hipStream_t st, const DevPlan &plan, const CsrT<float> &A, const int32_t *con_idx,
                   const float *v, const DDualArgs &a, double *raw_buf, double *partials);
+// Delta-form QP primal step fused behind the F^T product of Q dw, as
+// launch_qp_primal_fq (dqw: Q dw, stored).
+void launch_dprimal_fq(hipStream_t st, const DevPlan &planFT, const CsrT<float> &FT, const float *Ftmp,
+                       const float *fdiag, float *dqw, const DPrimalArgs &a);
 
 // Gather-ceiling probe: synthetic entries shaped like the sliced SpMV's
 // (T value + packed column/position word, one gather of a T each from the

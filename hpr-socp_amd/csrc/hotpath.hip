@@ -322,6 +322,20 @@ struct OpQPPrimal {
     }
 };
 
+// OpQPPrimal behind the F^T product of a factored Q = F^T F + D: the tile's
+// F^T row sum plus D w is Q w, formed as the stand-alone product forms it
+// (OpStore), stored for the tempv step and used in place of the stored Q w.
+template <typename T>
+struct OpQPPrimalFq : OpQPPrimal<T> {
+    T *qw_out;
+    const T *fdiag;
+    __device__ T raw(int i, T acc, bool) const {
+        const T q = fdiag ? acc + fdiag[i] * this->a.w[i] : acc;
+        qw_out[i] = q;
+        return this->a.x[i] + this->sigma * ((-q + this->a.aty[i]) - this->a.c[i]);
+    }
+};
+
 template <typename T>
 struct OpW2 {
     using V = T;  // raw / projected values
@@ -761,6 +775,15 @@ void launch_qp_primal(hipStream_t st, const DevPlan &plan, const int32_t *var_id
                                                 partials);
 }
 template <typename T>
+void launch_qp_primal_fq(hipStream_t st, const DevPlan &planFT, const CsrT<T> &FT, const T *Ftmp, const T *fdiag,
+                         T *qw, const QPPrimalArgsT<T> &a) {
+    OpQPPrimalFq<T> op{};
+    op.a = a;
+    op.qw_out = qw;
+    op.fdiag = fdiag;
+    run_tiles<T, OpQPPrimalFq<T>, SRC_CSR, false>(st, planFT, FT, Ftmp, nullptr, op, nullptr, nullptr);
+}
+template <typename T>
 void launch_w2(hipStream_t st, const DevPlan &plan, const CsrT<T> &AT, const W2ArgsT<T> &a) {
     run_tiles<T, OpW2<T>, SRC_CSR, false>(st, plan, AT, a.ybar, nullptr, OpW2<T>{a}, nullptr, nullptr);
 }
@@ -822,6 +845,19 @@ struct OpDPrimal {
     }
 };
 
+// OpDPrimal<true> behind the F^T product of Q dw (as OpQPPrimalFq): the FP32
+// row sum plus D dw is Q dw, rounded to FP32 as the stand-alone product stores it.
+struct OpDPrimalFq : OpDPrimal<true> {
+    float *dqw_out;
+    const float *fdiag;
+    __device__ double raw(int i, double acc, bool) const {
+        const float s = (float)acc;
+        const float q = fdiag ? s + fdiag[i] * a.dw[i] : s;
+        dqw_out[i] = q;
+        return a.ra[i] + ((double)a.dx[i] + sigma * ((double)a.daty[i] - (double)q));
+    }
+};
+
 struct OpDDual {
     using V = double;
     DDualArgs a;
@@ -855,6 +891,14 @@ void launch_dprimal(hipStream_t st, const DevPlan &plan, const CsrT<float> &AT, 
         run_tiles<float, OpDPrimal<false>, SRC_CSR, true>(st, plan, AT, a.dy, var_idx, OpDPrimal<false>{a},
                                                           raw_buf, partials);
     }
+}
+void launch_dprimal_fq(hipStream_t st, const DevPlan &planFT, const CsrT<float> &FT, const float *Ftmp,
+                       const float *fdiag, float *dqw, const DPrimalArgs &a) {
+    OpDPrimalFq op{};
+    op.a = a;
+    op.dqw_out = dqw;
+    op.fdiag = fdiag;
+    run_tiles<float, OpDPrimalFq, SRC_CSR, false>(st, planFT, FT, Ftmp, nullptr, op, nullptr, nullptr);
 }
 void launch_ddual(hipStream_t st, const DevPlan &plan, const CsrT<float> &A, const int32_t *con_idx,
                   const float *v, const DDualArgs &a, double *raw_buf, double *partials) {
@@ -955,6 +999,8 @@ void launch_probe_gather(hipStream_t st, int64_t nch, int32_t slices, int32_t wi
     template void launch_qp_primal<T>(hipStream_t, const DevPlan &, const int32_t *, const QPPrimalArgsT<T> &,  \
                                       T *, double *);                                                            \
     template void launch_w2<T>(hipStream_t, const DevPlan &, const CsrT<T> &, const W2ArgsT<T> &);              \
+    template void launch_qp_primal_fq<T>(hipStream_t, const DevPlan &, const CsrT<T> &, const T *, const T *, T *, \
+                                         const QPPrimalArgsT<T> &);                                              \
     template void launch_spmv<T>(hipStream_t, const DevPlan &, const CsrT<T> &, const T *, T *, const T *,      \
                                  const T *);                                                                     \
     template void launch_spmv_tempv<T>(hipStream_t, const DevPlan &, const CsrT<T> &, const T *,                \

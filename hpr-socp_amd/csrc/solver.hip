@@ -310,6 +310,9 @@ struct hprsocp_handle {
     double *A_v = nullptr, *AT_v = nullptr, *Q_v = nullptr, *Q_v2 = nullptr;
     int32_t *Q_tpos = nullptr;
     int q_kind = HPRSOCP_Q_CSR, has_quad = 0, Q_is_diag = 0, q_struct_diag = 0;
+    // factored Q and no variable cones: the primal step runs in the F^T product's
+    // tile kernel (launch_qp_primal_fq); HPRSOCP_FUSE_QW=0 keeps them apart
+    int fuse_qw = 0;
     int32_t Frows = 0;
     double *Fdiag = nullptr, *Ftmp = nullptr;
     DevPlan plan_A, plan_AT, plan_Q, plan_F, plan_FT;
@@ -2167,6 +2170,11 @@ static int create_impl(const hprsocp_problem *p, const hprsocp_params *params, h
             if (h->ih[i].Q_is_diag != h->Q_is_diag)
                 throw Err{HPRSOCP_ERR_INVALID, "batch instances must all have a diagonal Q or all a general Q"};
     }
+    {
+        const char *env = getenv("HPRSOCP_FUSE_QW");
+        h->fuse_qw = h->has_quad && h->q_kind == HPRSOCP_Q_FACTORED && h->nvar == 0 && h->nb == 1 &&
+                     h->plan_FT.ntiles > 0 && !(env && atoi(env) == 0);
+    }
     // sigma_0 (allocate_workspace, algorithm.jl:1538-1551)
     for (int i = 0; i < h->nb; ++i) {
         if (P.sigma == -1.0) {
@@ -2359,13 +2367,16 @@ static void do_update_t(hprsocp_handle *h, IterView<T> &V, int64_t kstep, int fu
         return;
     }
     const int sdiag = h->q_struct_diag && h->q_kind == HPRSOCP_Q_CSR;
+    const bool fq = h->fuse_qw && !h->use_act;
     ev(0);
-    if (!sdiag) qmap_t<T>(h, V, V.w, V.qw);
+    if (fq) launch_spmv<T>(st, h->plan_F, V.F, V.w, V.Ftmp);  // F w; F^T runs with the primal step
+    else if (!sdiag) qmap_t<T>(h, V, V.w, V.qw);
     ev(1);
     QPPrimalArgsT<T> a{V.x, V.x_hat, V.x_bar, V.z_bar, V.dx, V.w_bar, V.tempv, V.w, V.last_x, V.c,
                        V.l, V.u, V.aty, V.qw, V.diag_Q, V.fact1, V.fact2, h->isc, kstep,
                        full, sdiag, h->Q_is_diag};
-    launch_qp_primal<T>(st, pAT, h->var_idx, a, V.raw_n, h->part_n);
+    if (fq) launch_qp_primal_fq<T>(st, h->plan_FT, V.FT, V.Ftmp, V.Fdiag, V.qw, a);
+    else launch_qp_primal<T>(st, pAT, h->var_idx, a, V.raw_n, h->part_n);
     ev(2);
     if (m > 0) {
         if (!sdiag) {
@@ -2417,10 +2428,13 @@ static void do_update_delta_t(hprsocp_handle *h, int64_t kstep) {
         ev(2);
         return;
     }
+    const bool fq = h->fuse_qw && !h->use_act;
     ev(0);
-    if (!sdiag) qmap_t<float>(h, V, V.w, V.qw);  // Q dw
+    if (fq) launch_spmv<float>(st, h->plan_F, V.F, V.w, V.Ftmp);  // F dw; F^T runs with the primal step
+    else if (!sdiag) qmap_t<float>(h, V, V.w, V.qw);              // Q dw
     ev(1);
-    launch_dprimal(st, pAT, V.AT, h->var_idx, a, true, h->raw_n, h->part_n);
+    if (fq) launch_dprimal_fq(st, h->plan_FT, V.FT, V.Ftmp, V.Fdiag, V.qw, a);
+    else launch_dprimal(st, pAT, V.AT, h->var_idx, a, true, h->raw_n, h->part_n);
     ev(2);
     if (h->m > 0) {
         if (!sdiag) {  // tempv - last_x = (x_hat - last_x) + sigma (Q dw - Q (w_bar - last_w))
@@ -4133,7 +4147,7 @@ int hprsocp_get_scaled_A(hprsocp_handle *h, double *val) {
 
 int hprsocp_plan_info(hprsocp_handle *h, char *buf, int64_t cap, int64_t *len) {
     ABI_TRY(h, {
-        std::string j = "{\"operators\": [";
+        std::string j = std::string("{\"qw_primal_fused\": ") + (h->fuse_qw ? "1" : "0") + ", \"operators\": [";
         char b[1024];
         // forms forced by the environment are not timed (infinite times): reported as 0
         auto fin = [](float v) { return std::isfinite(v) ? (double)v : 0.0; };

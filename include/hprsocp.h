@@ -328,7 +328,8 @@ int hprsocp_set_scalars(hprsocp_handle *h, double sigma, double lambda_A, double
 /* Scaled A values in CSR order (A_nnz entries). */
 int hprsocp_get_scaled_A(hprsocp_handle *h, double *val);
 /* Launch-plan choices of the autotuner for every operator of the handle, as a
- * JSON object {"operators": [{name, form (plain | slicesP | sortedP |
+ * JSON object {"qw_primal_fused": 0|1 (the primal step runs in the F^T
+ * product's tile kernel), "operators": [{name, form (plain | slicesP | sortedP |
  * sortedP-fused | sortedP-rounds), tuned, cached, nnz_target, g_divisor, huge_target,
  * huge_whole, stream, tiles, candidates, ms_default, ms, ms_form, block,
  * form32, block32, ms32}, ...]}: the tile-plan configuration

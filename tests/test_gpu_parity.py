@@ -524,3 +524,30 @@ def test_sorted_whole_rounds_bit_identical(name, precision, monkeypatch, gpu_lib
         assert a[2].iter == b[2].iter and a[2].primal_obj == b[2].primal_obj
         np.testing.assert_array_equal(a[2].x, b[2].x)
         np.testing.assert_array_equal(a[2].y, b[2].y)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32_refine", "fp32_delta"])
+def test_factored_q_primal_fusion_bit_identical(monkeypatch, prec):
+    """With a factored Q and no variable cones the primal step runs in the F^T
+    product's tile kernel; its iterates and solve equal the separate Q w product
+    followed by the primal step bit for bit."""
+    model = synthetic.make_c1(scale=0.02, seed=21, q_form="factored")
+    outs, infos, res = [], [], []
+    for fuse in ("0", "1"):
+        monkeypatch.setenv("HPRSOCP_FUSE_QW", fuse)
+        p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-8, precision=prec)
+        with hprsocp.Solver(model, p) as g:
+            g.estimate_lambda()
+            infos.append(g.plan_info()["qw_primal_fused"])
+            for k in range(30):
+                g.update(k, k % 10 == 9)
+            outs.append({v: g.get(v).copy() for v in ("x", "y", "x_bar", "y_bar")})
+        with hprsocp.Solver(model, p) as g:
+            r = g.solve()
+            res.append((r.status, r.iter, r.primal_obj, r.x.copy()))
+    assert infos == [0, 1]
+    for v in outs[0]:
+        assert np.array_equal(outs[0][v], outs[1][v]), v
+    assert res[0][:3] == res[1][:3]
+    assert np.array_equal(res[0][3], res[1][3])
