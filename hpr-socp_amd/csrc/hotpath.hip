@@ -611,12 +611,40 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
     if (!op.bind(t.inst)) return;  // retired instance (uniform over the workgroup)
     const int tid = threadIdx.x;
     const int nrows = t.re - t.rb;
+    // linear rows: raw value, box projection and update, one row per thread
+    auto linear_rows = [&](V a) {
+        if (tid < nrows) {
+            const int i = t.rb + tid;
+            const V r = op.raw(i, a, false);
+            op.finish(i, r, op.lin_proj(i, r));
+        }
+    };
+    // HUGE segment, phase 1: raw values and a partial |xi|^2
+    auto huge_phase1 = [&](V a) {
+        double sq = 0.0;
+        if (tid < nrows) {
+            const int i = t.rb + tid;
+            const V r = op.raw(i, a, true);
+            raw_buf[i] = r;
+            if (i != huge.start[t.c0]) sq = (double)r * (double)r;
+        }
+        sq = block_sum(sq, sh);
+        if (tid == 0) partials[t.c1] = sq;
+    };
     // raw_in_acc: acc holds the rows' raw values (not their products) for the cone loop
     bool raw_in_acc = false;
     if constexpr (SRC == SRC_CSR) {
         if (t.g == 0) tile_spmv_stream(M, vin, t, acc, prod, rps);
         else tile_spmv(M, vin, t, acc, sh);
         __syncthreads();
+        if (!CONES || t.kind == TILE_LINEAR) {
+            linear_rows(tid < nrows ? acc[tid] : V(0));
+            return;
+        }
+        if (t.kind == TILE_HUGE) {
+            huge_phase1(tid < nrows ? acc[tid] : V(0));
+            return;
+        }
     } else {
         // one row per thread: the row's product stays in a register, and a row
         // that no other thread reads (linear rows, HUGE segments) takes no LDS
@@ -633,24 +661,11 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
             }
         }
         if (!CONES || t.kind == TILE_LINEAR) {
-            if (tid < nrows) {
-                const int i = t.rb + tid;
-                const V r = op.raw(i, a, false);
-                op.finish(i, r, op.lin_proj(i, r));
-            }
+            linear_rows(a);
             return;
         }
         if (t.kind == TILE_HUGE) {
-            // HUGE segment, phase 1: raw values and a partial |xi|^2
-            double sq = 0.0;
-            if (tid < nrows) {
-                const int i = t.rb + tid;
-                const V r = op.raw(i, a, true);
-                raw_buf[i] = r;
-                if (i != huge.start[t.c0]) sq = (double)r * (double)r;
-            }
-            sq = block_sum(sq, sh);
-            if (tid == 0) partials[t.c1] = sq;
+            huge_phase1(a);
             return;
         }
         // cone rows: raw values formed here, read by the cones' lane groups
@@ -658,46 +673,26 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
         raw_in_acc = true;
         __syncthreads();
     }
-    if (!CONES || t.kind == TILE_LINEAR) {
-        if (tid < nrows) {
-            const int i = t.rb + tid;
-            const V r = op.raw(i, acc[tid], false);
-            op.finish(i, r, op.lin_proj(i, r));
+    // CONES tile: one lane group of g2 lanes per cone
+    const int G2 = t.g2;
+    const int lane = tid & (G2 - 1), grp = tid / G2, ngrp = kBlock / G2;
+    for (int c = t.c0 + grp; c < t.c1; c += ngrp) {
+        const int s = cone_idx[c], e = cone_idx[c + 1];
+        double sq = 0.0, head = 0.0;
+        for (int i = s + lane; i < e; i += G2) {
+            const V r = raw_in_acc ? acc[i - t.rb] : op.raw(i, acc[i - t.rb], true);
+            acc[i - t.rb] = r;
+            if (i == s) head = (double)r;
+            else sq += (double)r * (double)r;
         }
-        return;
-    }
-    if (t.kind == TILE_CONES) {
-        const int G2 = t.g2;
-        const int lane = tid & (G2 - 1), grp = tid / G2, ngrp = kBlock / G2;
-        for (int c = t.c0 + grp; c < t.c1; c += ngrp) {
-            const int s = cone_idx[c], e = cone_idx[c + 1];
-            double sq = 0.0, head = 0.0;
-            for (int i = s + lane; i < e; i += G2) {
-                const V r = raw_in_acc ? acc[i - t.rb] : op.raw(i, acc[i - t.rb], true);
-                acc[i - t.rb] = r;
-                if (i == s) head = (double)r;
-                else sq += (double)r * (double)r;
-            }
-            for (int o = G2 >> 1; o > 0; o >>= 1) sq += __shfl_xor(sq, o, G2);
-            head = __shfl(head, 0, G2);
-            const ConeCase cc = cone_case(head, sqrt(sq));
-            for (int i = s + lane; i < e; i += G2) {
-                const V r = acc[i - t.rb];
-                op.finish(i, r, cone_apply(cc, i == s, r));
-            }
+        for (int o = G2 >> 1; o > 0; o >>= 1) sq += __shfl_xor(sq, o, G2);
+        head = __shfl(head, 0, G2);
+        const ConeCase cc = cone_case(head, sqrt(sq));
+        for (int i = s + lane; i < e; i += G2) {
+            const V r = acc[i - t.rb];
+            op.finish(i, r, cone_apply(cc, i == s, r));
         }
-        return;
     }
-    // HUGE segment, phase 1: raw values and a partial |xi|^2
-    double sq = 0.0;
-    if (tid < nrows) {
-        const int i = t.rb + tid;
-        const V r = op.raw(i, acc[tid], true);
-        raw_buf[i] = r;
-        if (i != huge.start[t.c0]) sq = (double)r * (double)r;
-    }
-    sq = block_sum(sq, sh);
-    if (tid == 0) partials[t.c1] = sq;
 }
 
 // HUGE segment, phase 2: the cone's |xi| from all partials, then the projection.

@@ -1357,7 +1357,7 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
         HIPCHK(hipEventCreate(&e1));
     }
     // every candidate is timed; the fastest wins unless a form earlier in the
-    // preference order (8 sorted, 8, 4 sorted, 4, plain, slice-fused) is within 2% of it,
+    // preference order (see the ranks below) is within 2% of it,
     // which keeps the choice stable between runs when forms time alike
     struct Form {
         CsrView M;
@@ -1369,12 +1369,22 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
     };
     std::vector<Form> forms;
     forms.push_back(Form{M, plan, env ? INFINITY : time_one(M, plan), 4, false});  // plain
+    int32_t natural_tiles[2] = {-1, -1};  // tiles of the natural column-sorted tiling, P = 4 / 8
     for (const auto &cand : cands) {
         const int32_t P = cand.P;
         CsrView S = M;
         if (!build_slices(h, S, hm, dval, P, cand.sorted, cand.fused * sorted_slots(h),
                           cand.rounds ? std::max(1, sorted_slots(h) / P) : 0))
             continue;
+        if (cand.sorted && !cand.fused && (P == 4 || P == 8)) {
+            int32_t &nat = natural_tiles[P == 8];
+            if (!cand.rounds) {
+                nat = S.s_ntiles;
+            } else if (S.s_ntiles == nat) {  // the natural tiling fills whole rounds already
+                free_slices(h, S);
+                continue;
+            }
+        }
         DevPlan splan = make_plan(h, {}, hm.nrows, lin_end, cidx, ncones, row_inst);
         float t = env ? 0.0f : time_one(S, splan);
         if (env) {
@@ -1386,8 +1396,10 @@ void tune_slices(hprsocp_handle *h, CsrView &M, DevPlan &plan, const HostCsr &hm
             printf("XCD slicing autotune %s: %d slices%s%s%s (%d tiles, %d threads) %.4f ms (plain %.4f ms)\n", name,
                    P, cand.sorted ? " column-sorted" : "", S.s_fused ? " slice-fused" : "",
                    cand.rounds ? " whole rounds" : "", S.s_ntiles, S.s_block, t, forms[0].t);
-        // preference order on near-ties: 8 sorted, 8, 4 sorted, 4, plain, slice-fused, whole rounds
-        const int rank = cand.rounds ? 7 : cand.fused ? 4 + cand.fused : (P >= 8 ? 0 : 2) + (cand.sorted ? 0 : 1);
+        // preference order on near-ties: 8 sorted in whole rounds (the natural 8-sorted tiling
+        // with its last round filled: as fast or faster inside the iteration, where the two
+        // often time alike alone), 8 sorted, 8, 4 sorted, 4, plain, slice-fused
+        const int rank = cand.rounds ? -1 : cand.fused ? 4 + cand.fused : (P >= 8 ? 0 : 2) + (cand.sorted ? 0 : 1);
         forms.push_back(Form{S, splan, t, rank, true, cand.rounds});
     }
     float tmin = INFINITY;
