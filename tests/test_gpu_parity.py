@@ -551,3 +551,29 @@ def test_factored_q_primal_fusion_bit_identical(monkeypatch, prec):
         assert np.array_equal(outs[0][v], outs[1][v]), v
     assert res[0][:3] == res[1][:3]
     assert np.array_equal(res[0][3], res[1][3])
+
+
+@pytest.mark.gpu
+def test_factored_q_primal_fusion_without_diagonal_and_with_variable_cones(monkeypatch):
+    """Q = F'F with no diagonal part: the fused primal step (no D w term) equals
+    the separate kernels bit for bit.  With variable cones the steps stay apart."""
+    import dataclasses
+    base = synthetic.make_c1(scale=0.02, seed=22, q_form="factored")
+    model = dataclasses.replace(base, Q=("factored", base.Q[1], None))
+    outs = []
+    for fuse in ("0", "1"):
+        monkeypatch.setenv("HPRSOCP_FUSE_QW", fuse)
+        p = hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6, precision="fp64")
+        with hprsocp.Solver(model, p) as g:
+            g.estimate_lambda()
+            assert g.plan_info()["qw_primal_fused"] == int(fuse)
+            for k in range(30):
+                g.update(k, k % 10 == 9)
+            outs.append({v: g.get(v).copy() for v in ("x", "y", "x_bar", "y_bar")})
+    for v in outs[0]:
+        assert np.array_equal(outs[0][v], outs[1][v]), v
+    monkeypatch.delenv("HPRSOCP_FUSE_QW")
+    cones = synthetic.make_small_qp(seed=5, q="factored")
+    assert len(cones.SOC_var_idx) > 1
+    with hprsocp.Solver(cones, hprsocp.HPRSOCP_parameters(verbose=False, stoptol=1e-6)) as g:
+        assert g.plan_info()["qw_primal_fused"] == 0
