@@ -11,6 +11,7 @@
 // projection in registers; HUGE tiles store raw values and a partial |xi|^2,
 // and k_huge_finish projects once all partials of the cone exist.
 #include <algorithm>
+#include <type_traits>
 
 #include "devutil.h"
 #include "hotpath.h"
@@ -274,15 +275,22 @@ struct OpDual {
         return cone ? (acc - a.rhs[i]) - f1 * a.y[i] : acc - f1 * a.y[i];
     }
     __device__ T lin_proj(int i, T r) const { return jclamp(r, a.AL[i], a.AU[i]); }
-    __device__ void finish(int i, T raw, T p) const {
-        const T yi = a.y[i];
+    __device__ void finish(int i, T raw, T p) const { finish_with(i, raw, p, a.y[i], a.last_y[i]); }
+    // cone rows: y and last_y are read with the raw value, before the cone's
+    // reduction, and handed to the lane that finishes the row (kStash)
+    static constexpr bool kStash = true;
+    __device__ void stash(int i, T &s0, T &s1) const {
+        s0 = a.y[i];
+        s1 = a.last_y[i];
+    }
+    __device__ void finish_with(int i, T raw, T p, T yi, T lyi) const {
         const T yb = f2 * (p - raw);
         if (a.full) {
             a.s[i] = p;
             a.dy[i] = yb - yi;
         }
         if (a.full || a.write_ybar) a.y_bar[i] = yb;
-        a.y[i] = h2 * (T(2) * yb - yi) + h1 * a.last_y[i];
+        a.y[i] = h2 * (T(2) * yb - yi) + h1 * lyi;
     }
 };
 
@@ -595,6 +603,13 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_sorted_fused(CsrT<T> M, const T 
 // SRC: 0 = no SpMV, 1 = CSR SpMV inside the tile, 2 = sum of the XCD-slice partials
 enum : int { SRC_NONE = 0, SRC_CSR = 1, SRC_SLICES = 2 };
 
+// Op::kStash: the epilogue can take operands of a cone row read early (stash /
+// finish_with) instead of reading them after the cone's reduction.
+template <class Op, class = void>
+struct op_stashes : std::false_type {};
+template <class Op>
+struct op_stashes<Op, std::void_t<decltype(Op::kStash)>> : std::bool_constant<Op::kStash> {};
+
 // T: SpMV arithmetic; Op::V: raw and projected values (T, or double for the delta form).
 template <typename T, class Op, int SRC, bool CONES>
 __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles, CsrT<T> M, const T *__restrict__ vin,
@@ -603,6 +618,9 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
                                                  double *__restrict__ partials) {
     using V = typename Op::V;
     __shared__ V acc[kTileRows];
+    // cone rows' stashed epilogue operands (one-row-per-thread sources only)
+    constexpr bool kStashRows = CONES && SRC != SRC_CSR && op_stashes<Op>::value;
+    __shared__ V st0[kStashRows ? kTileRows : 1], st1[kStashRows ? kTileRows : 1];
     __shared__ double sh[4];
     __shared__ T prod[SRC == SRC_CSR ? kStreamNnz : 1];
     __shared__ int32_t rps[SRC == SRC_CSR ? kTileRows + 1 : 1];
@@ -669,7 +687,10 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
             return;
         }
         // cone rows: raw values formed here, read by the cones' lane groups
-        if (tid < nrows) acc[tid] = op.raw(t.rb + tid, a, true);
+        if (tid < nrows) {
+            acc[tid] = op.raw(t.rb + tid, a, true);
+            if constexpr (kStashRows) op.stash(t.rb + tid, st0[tid], st1[tid]);
+        }
         raw_in_acc = true;
         __syncthreads();
     }
@@ -690,7 +711,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
         const ConeCase cc = cone_case(head, sqrt(sq));
         for (int i = s + lane; i < e; i += G2) {
             const V r = acc[i - t.rb];
-            op.finish(i, r, cone_apply(cc, i == s, r));
+            if constexpr (kStashRows) op.finish_with(i, r, cone_apply(cc, i == s, r), st0[i - t.rb], st1[i - t.rb]);
+            else op.finish(i, r, cone_apply(cc, i == s, r));
         }
     }
 }
@@ -869,9 +891,16 @@ struct OpDDual {
     // rya holds A last_x - rhs (SOC rows) - f1 last_y
     __device__ double raw(int i, double acc, bool) const { return a.rya[i] + (acc - f1 * (double)a.dy[i]); }
     __device__ double lin_proj(int i, double r) const { return jclamp(r, a.AL[i], a.AU[i]); }
-    __device__ void finish(int i, double raw, double p) const {
-        const double dyb = f2 * (p - raw) - a.ya[i];  // y_bar - last_y
-        a.dy[i] = (float)(h2 * (2.0 * dyb - (double)a.dy[i]));
+    __device__ void finish(int i, double raw, double p) const { finish_with(i, raw, p, a.ya[i], (double)a.dy[i]); }
+    // cone rows: last_y and dy read before the cone's reduction (as OpDual)
+    static constexpr bool kStash = true;
+    __device__ void stash(int i, double &s0, double &s1) const {
+        s0 = a.ya[i];
+        s1 = (double)a.dy[i];
+    }
+    __device__ void finish_with(int i, double raw, double p, double yai, double dyi) const {
+        const double dyb = f2 * (p - raw) - yai;  // y_bar - last_y
+        a.dy[i] = (float)(h2 * (2.0 * dyb - dyi));
         a.dyb[i] = (float)dyb;
     }
 };
