@@ -275,15 +275,20 @@ struct OpDual {
         return cone ? (acc - a.rhs[i]) - f1 * a.y[i] : acc - f1 * a.y[i];
     }
     __device__ T lin_proj(int i, T r) const { return jclamp(r, a.AL[i], a.AU[i]); }
-    __device__ void finish(int i, T raw, T p) const { finish_with(i, raw, p, a.y[i], a.last_y[i]); }
-    // cone rows: y and last_y are read with the raw value, before the cone's
-    // reduction, and handed to the lane that finishes the row (kStash)
-    static constexpr bool kStash = true;
-    __device__ void stash(int i, T &s0, T &s1) const {
-        s0 = a.y[i];
-        s1 = a.last_y[i];
+    __device__ void finish(int i, T raw, T p) const {
+        T sv[kStash];
+        stash(i, sv);
+        finish_with(i, raw, p, sv);
     }
-    __device__ void finish_with(int i, T raw, T p, T yi, T lyi) const {
+    // cone rows: y and last_y are read with the raw value, before the cone's
+    // reduction, and handed to the lane that finishes the row
+    static constexpr int kStash = 2;
+    __device__ void stash(int i, T *sv) const {
+        sv[0] = a.y[i];
+        sv[1] = a.last_y[i];
+    }
+    __device__ void finish_with(int i, T raw, T p, const T *sv) const {
+        const T yi = sv[0], lyi = sv[1];
         const T yb = f2 * (p - raw);
         if (a.full) {
             a.s[i] = p;
@@ -315,15 +320,30 @@ struct OpQPPrimal {
     __device__ T raw(int i, T, bool) const { return a.x[i] + sigma * ((-qw(i) + a.aty[i]) - a.c[i]); }
     __device__ T lin_proj(int i, T r) const { return jclamp(r, a.l[i], a.u[i]); }
     __device__ void finish(int i, T raw, T p) const {
-        const T xi = a.x[i], wi = a.w[i];
+        T sv[kStash];
+        stash(i, sv);
+        finish_with(i, raw, p, sv);
+    }
+    // cone rows: x, w, last_x and the per-coordinate Q factors are read with the
+    // raw value, before the cone's reduction
+    static constexpr int kStash = 5;
+    __device__ void stash(int i, T *sv) const {
+        sv[0] = a.x[i];
+        sv[1] = a.w[i];
+        sv[2] = a.last_x[i];
+        sv[3] = a.vec_fact ? a.fact1[i] : T(0);
+        sv[4] = a.vec_fact ? a.fact2[i] : T(0);
+    }
+    __device__ void finish_with(int i, T raw, T p, const T *sv) const {
+        const T xi = sv[0], wi = sv[1];
         const T xh = T(2) * p - xi;
-        const T wb = a.vec_fact ? a.fact1[i] * wi + a.fact2[i] * xh : f1s * wi + f2s * xh;
+        const T wb = a.vec_fact ? sv[3] * wi + sv[4] * xh : f1s * wi + f2s * xh;
         if (a.full) {
             a.x_bar[i] = p;
             a.z_bar[i] = (p - raw) / sigma;
             a.dx[i] = p - xi;
         }
-        a.x[i] = h2 * xh + h1 * a.last_x[i];
+        a.x[i] = h2 * xh + h1 * sv[2];
         a.x_hat[i] = xh;
         a.w_bar[i] = wb;
         if (a.diag) a.tempv[i] = xh + sigma * (a.dq[i] * wi - a.dq[i] * wb);
@@ -603,12 +623,12 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_sorted_fused(CsrT<T> M, const T 
 // SRC: 0 = no SpMV, 1 = CSR SpMV inside the tile, 2 = sum of the XCD-slice partials
 enum : int { SRC_NONE = 0, SRC_CSR = 1, SRC_SLICES = 2 };
 
-// Op::kStash: the epilogue can take operands of a cone row read early (stash /
-// finish_with) instead of reading them after the cone's reduction.
+// Op::kStash > 0: the epilogue can take kStash operands of a cone row read
+// early (stash / finish_with) instead of reading them after the cone's reduction.
 template <class Op, class = void>
-struct op_stashes : std::false_type {};
+struct op_stash_n : std::integral_constant<int, 0> {};
 template <class Op>
-struct op_stashes<Op, std::void_t<decltype(Op::kStash)>> : std::bool_constant<Op::kStash> {};
+struct op_stash_n<Op, std::void_t<decltype(Op::kStash)>> : std::integral_constant<int, Op::kStash> {};
 
 // T: SpMV arithmetic; Op::V: raw and projected values (T, or double for the delta form).
 template <typename T, class Op, int SRC, bool CONES>
@@ -619,8 +639,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
     using V = typename Op::V;
     __shared__ V acc[kTileRows];
     // cone rows' stashed epilogue operands (one-row-per-thread sources only)
-    constexpr bool kStashRows = CONES && SRC != SRC_CSR && op_stashes<Op>::value;
-    __shared__ V st0[kStashRows ? kTileRows : 1], st1[kStashRows ? kTileRows : 1];
+    constexpr int kStashN = CONES && SRC != SRC_CSR ? op_stash_n<Op>::value : 0;
+    __shared__ V stv[kStashN > 0 ? kStashN : 1][kStashN > 0 ? kTileRows : 1];
     __shared__ double sh[4];
     __shared__ T prod[SRC == SRC_CSR ? kStreamNnz : 1];
     __shared__ int32_t rps[SRC == SRC_CSR ? kTileRows + 1 : 1];
@@ -689,7 +709,12 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
         // cone rows: raw values formed here, read by the cones' lane groups
         if (tid < nrows) {
             acc[tid] = op.raw(t.rb + tid, a, true);
-            if constexpr (kStashRows) op.stash(t.rb + tid, st0[tid], st1[tid]);
+            if constexpr (kStashN > 0) {
+                V sv[kStashN > 0 ? kStashN : 1];
+                op.stash(t.rb + tid, sv);
+#pragma unroll
+                for (int k = 0; k < kStashN; ++k) stv[k][tid] = sv[k];
+            }
         }
         raw_in_acc = true;
         __syncthreads();
@@ -711,8 +736,14 @@ __global__ __launch_bounds__(kBlock) void k_tile(const Tile *__restrict__ tiles,
         const ConeCase cc = cone_case(head, sqrt(sq));
         for (int i = s + lane; i < e; i += G2) {
             const V r = acc[i - t.rb];
-            if constexpr (kStashRows) op.finish_with(i, r, cone_apply(cc, i == s, r), st0[i - t.rb], st1[i - t.rb]);
-            else op.finish(i, r, cone_apply(cc, i == s, r));
+            if constexpr (kStashN > 0) {
+                V sv[kStashN > 0 ? kStashN : 1];
+#pragma unroll
+                for (int k = 0; k < kStashN; ++k) sv[k] = stv[k][i - t.rb];
+                op.finish_with(i, r, cone_apply(cc, i == s, r), sv);
+            } else {
+                op.finish(i, r, cone_apply(cc, i == s, r));
+            }
         }
     }
 }
@@ -891,14 +922,19 @@ struct OpDDual {
     // rya holds A last_x - rhs (SOC rows) - f1 last_y
     __device__ double raw(int i, double acc, bool) const { return a.rya[i] + (acc - f1 * (double)a.dy[i]); }
     __device__ double lin_proj(int i, double r) const { return jclamp(r, a.AL[i], a.AU[i]); }
-    __device__ void finish(int i, double raw, double p) const { finish_with(i, raw, p, a.ya[i], (double)a.dy[i]); }
-    // cone rows: last_y and dy read before the cone's reduction (as OpDual)
-    static constexpr bool kStash = true;
-    __device__ void stash(int i, double &s0, double &s1) const {
-        s0 = a.ya[i];
-        s1 = (double)a.dy[i];
+    __device__ void finish(int i, double raw, double p) const {
+        double sv[kStash];
+        stash(i, sv);
+        finish_with(i, raw, p, sv);
     }
-    __device__ void finish_with(int i, double raw, double p, double yai, double dyi) const {
+    // cone rows: last_y and dy read before the cone's reduction (as OpDual)
+    static constexpr int kStash = 2;
+    __device__ void stash(int i, double *sv) const {
+        sv[0] = a.ya[i];
+        sv[1] = (double)a.dy[i];
+    }
+    __device__ void finish_with(int i, double raw, double p, const double *sv) const {
+        const double yai = sv[0], dyi = sv[1];
         const double dyb = f2 * (p - raw) - yai;  // y_bar - last_y
         a.dy[i] = (float)(h2 * (2.0 * dyb - dyi));
         a.dyb[i] = (float)dyb;
